@@ -1,0 +1,166 @@
+"""Benchmark of the LBM time step on one MI355X.
+
+Workload (default): D3Q19 MRT + Smagorinsky LES (Cs 0.17, tau0 0.505)
+flow past a sphere, 256^3 FP32 SoA, sphere R = 24 at (64, 128, 128) with
+BFL curved links, Zou-He velocity inlet u = 0.05 with seeded 1% turbulence,
+convective outflow, periodic y/z.  One step = one fused pull-stream-collide
+launch over the whole domain; MLUPS counts every node, as the reference does
+(sphere_3d_wake.cpp:81-92).
+
+Prints ONE JSON line.  The roofline entry divides the algorithmic bytes of
+a step (one read and one write of every population: 152 B/node for D3Q19
+FP32) by the average duration of the step kernel, measured with HIP events
+on the domain's stream over the timed launches.  The CPU baseline times the
+oracle's unfused restatement of the reference step on the host cores on a
+short sample of the same workload.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "D3Q19 MLUPS & %HBM peak; batched-serve vs plain sims/s + p99 at stated err"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def sphere_workload(grid_cls, n=256, R=24.0, storage=0, lattice=19, plane_pad=-1):
+    g = grid_cls(n, n, n, 0.505, lattice=lattice, storage=storage, plane_pad=plane_pad)
+    g.set_periodic_axes(True, True)
+    g.initialize(1.0, 0.05, 0.0, 0.0)
+    g.set_les_coupled(True)
+    g.set_inflow_west(0.05, 0.0)
+    g.set_inflow_turbulence(0.01, 2)
+    g.set_convective_outflow_east(0.05)
+    g.set_solid_sphere(64.0, 128.0, 128.0, R)
+    nl = g.build_curved_sphere_links(64.0, 128.0, 128.0, R)
+    return g, nl
+
+
+def cpu_baseline(n, steps, R=24.0):
+    """Oracle (unfused restatement of the reference step) on the host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+
+    L = ob.lib(32)
+    o = ob.OracleGrid(3, n, n, n, tau=0.505, bits=32)
+    o.set_periodic_axes(True, True)
+    o.initialize(1.0, 0.05, 0.0, 0.0)
+    o.set_les_coupled(True)
+    o.set_inflow_west(0.05, 0.0)
+    o.set_inflow_turbulence(0.01, 2)
+    o.set_convective_outflow_east(0.05)
+    c = 64.0 * n / 256
+    o.set_solid_sphere(c, n / 2, n / 2, R * n / 256)
+    o.build_curved_sphere_links(c, n / 2, n / 2, R * n / 256)
+    o.step("MRT", use_les=True, smag=0.17)  # warm-up
+    t0 = time.perf_counter()
+    o.run(steps, "MRT", use_les=True, smag=0.17)
+    dt = time.perf_counter() - t0
+    return n ** 3 * steps / dt / 1e6, L.oracle_threads(), dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--storage", choices=["fp32", "fp16", "bf16"], default="fp32")
+    ap.add_argument("--plane-pad", type=int, default=-1)
+    ap.add_argument("--block-x", type=int, default=0)
+    ap.add_argument("--cpu-steps", type=int, default=4, help="oracle steps timed for the CPU baseline (0: skip)")
+    ap.add_argument("--cpu-n", type=int, default=256)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import wake_amd
+
+    storage = {"fp32": 0, "fp16": 1, "bf16": 2}[args.storage]
+    bpn = {0: 152, 1: 76, 2: 76}[storage]
+    n = args.n
+    N = n ** 3
+    log(f"setup {n}^3 sphere workload ({args.storage})")
+    t0 = time.perf_counter()
+    g, nlinks = sphere_workload(wake_amd.LatticeGrid3D, n, 24.0 * n / 256, storage, plane_pad=args.plane_pad)
+    if args.block_x:
+        g.set_block_x(args.block_x)
+    log(f"setup done in {time.perf_counter() - t0:.1f}s, {nlinks} BFL links, device bytes {g.device_bytes() / 1e9:.2f} GB")
+    g.run(args.warmup, "MRT", use_les=True, smag=0.17)
+    g.synchronize()
+    log(f"warmup {args.warmup} steps done")
+    t0 = time.perf_counter()
+    ms = g.run(args.steps, "MRT", use_les=True, smag=0.17, timed=True)
+    g.synchronize()
+    wall = time.perf_counter() - t0
+    log(f"timed {args.steps} steps: events {ms:.1f} ms, wall {wall * 1e3:.1f} ms")
+    mlups = N * args.steps / wall / 1e6
+    kernel_ms = ms / args.steps
+    achieved = bpn * N / (kernel_ms * 1e-3) / 1e9
+    # momentum exchange on the final state (C_D, C_L), FP64 on the device
+    R = 24.0 * n / 256
+    F = g.momentum_exchange(64.0 * n / 256, n / 2, n / 2, 2 * R)
+    q = 0.5 * 0.05 ** 2 * math.pi * R * R
+    cd, cl = F[0] / q, F[1] / q
+    ok = all(math.isfinite(v) for v in (cd, cl)) and math.isfinite(g.compute_total_mass())
+    log(f"C_D {cd:.4f} C_L {cl:.5f} finite={ok}")
+
+    cpu = None
+    if args.cpu_steps > 0:
+        log(f"CPU baseline: oracle {args.cpu_n}^3, {args.cpu_steps} steps")
+        v, cores, dt = cpu_baseline(args.cpu_n, args.cpu_steps, 24.0)
+        cpu = {"value": round(v, 2), "unit": "MLUPS", "cores": cores, "kind": "port",
+               "sample": f"oracle (unfused reference step sequence, FP32, OpenMP) on the same {args.cpu_n}^3 "
+                         f"sphere workload, {args.cpu_steps} timed steps after 1 warm-up ({dt:.1f} s)"}
+        log(f"CPU baseline {v:.1f} MLUPS on {cores} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if storage == 0 else ("fp16-storage/fp32" if storage == 1 else "bf16-storage/fp32"),
+        "data": "synthetic",
+        "config": {
+            "workload": f"D3Q19 MRT + coupled Smagorinsky LES flow past a sphere, {n}^3, R={R:g} at "
+                        f"({64 * n // 256},{n // 2},{n // 2}), BFL links, Zou-He inlet u=0.05 + 1% seeded "
+                        f"turbulence, convective outflow, periodic y/z",
+            "lattice": "D3Q19", "collision": "MRT+LES(Cs=0.17,tau0=0.505)", "nodes": N,
+            "bfl_links": nlinks, "storage": args.storage, "C_D": round(cd, 5), "C_L": round(cl, 6),
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_node": bpn,
+            "kernel_ms": round(kernel_ms, 5),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

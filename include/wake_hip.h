@@ -1,0 +1,231 @@
+/*
+ * wake_hip.h -- C ABI of the MI355X (gfx950) lattice-Boltzmann step.
+ *
+ * Part 1 replaces the reference's LBM kernels one for one: every function
+ * has the name, parameter list and argument meaning of the ISPC export it
+ * replaces (cited as file:line in the reference tree), takes host pointers
+ * in the reference's SoA layout (population q of node i at q * count + i,
+ * node i = (z * ny + y) * nx + x), and runs on the GPU through staged device
+ * buffers.  Like the ISPC exports they return nothing; a failure of the GPU
+ * runtime is reported on stderr and by wake_hip_last_error().  The
+ * reference's consumers declare these names themselves in
+ * `namespace ispc { extern "C" { ... } }`, so linking libwake_hip.so in place
+ * of the ISPC objects is the drop-in.
+ *
+ * Part 2 is the device-resident path behind the reference's C++ grid API
+ * (include/wake/grid/lattice_grid_3d.hpp:21-123 and lattice_grid_2d.hpp:14-113,
+ * which the C++ classes in include/wake/grid/ keep): populations stay in HBM
+ * between steps and one fused kernel per step pulls, bounces back, applies
+ * BFL links and open boundaries, and collides.  Functions return 0 on success
+ * and a non-zero code on failure, with the message in wake_hip_last_error().
+ *
+ * Plain pointers and sizes only.  Floating point is float (the reference's
+ * real_t, include/wake/core/types.hpp:12); cell types are the reference's
+ * uint8 codes (types.hpp:69-77): 0 Fluid, 1 Solid, 2 Inflow, 3 Outflow,
+ * 4 Periodic, 5 Symmetry, 6 MovingWall.
+ */
+#ifndef WAKE_HIP_H
+#define WAKE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ----------------------------------------------------------------------
+ * Library
+ * -------------------------------------------------------------------- */
+/* Message of the last failure on this thread ("" if none). */
+const char* wake_hip_last_error(void);
+/* Clear the last failure; returns the previous error code (0 = none). */
+int wake_hip_clear_error(void);
+/* Number of visible GPUs, or a negative value if the runtime failed. */
+int wake_hip_device_count(void);
+/* The D3Q19 MRT back-transform table the kernels compile in (19 x 19 row
+ * major): inv(M) rounded to 7 decimals, src/ispc/d3q19_collision_mrt.ispc:4-24.
+ * Host-only; needs no GPU. */
+void wake_hip_d3q19_minv(float out[361]);
+
+/* ----------------------------------------------------------------------
+ * Part 1: the reference's LBM kernel exports (host pointers)
+ * -------------------------------------------------------------------- */
+/* src/ispc/d2q9_moments.ispc:4 */
+void d2q9_compute_macroscopic_moments(const float f[], float rho[], float ux[], float uy[], int count);
+/* src/ispc/d2q9_moments.ispc:35 */
+void d2q9_compute_macroscopic_moments_forced(const float f[], float rho[], float ux[], float uy[], float gx,
+                                             float gy, int count);
+/* src/ispc/d2q9_moments.ispc:68 */
+void d2q9_compute_velocity_magnitude(const float ux[], const float uy[], float speed[], int count);
+/* src/ispc/d2q9_collision_bgk.ispc:5 */
+void d2q9_collide_bgk(const float f_in[], float f_out[], const uint8_t cell_type[], float rho[], float ux[],
+                      float uy[], float tau, float gx, float gy, int count);
+/* src/ispc/d2q9_collision_trt.ispc:7 */
+void d2q9_collide_trt(const float f_in[], float f_out[], const uint8_t cell_type[], float rho[], float ux[],
+                      float uy[], float tau_s, float tau_a, float gx, float gy, int count);
+/* src/ispc/d2q9_collision_mrt.ispc:6 */
+void d2q9_collide_mrt(const float f_in[], float f_out[], const uint8_t cell_type[], float rho[], float ux[],
+                      float uy[], float tau_shear, float tau_bulk, float gx, float gy, int count);
+/* src/ispc/d2q9_streaming.ispc:4 */
+void d2q9_stream_periodic(const float f_src[], float f_dst[], int nx, int ny);
+/* src/ispc/d2q9_streaming.ispc:33 */
+void d2q9_stream_with_boundaries(const float f_src[], float f_dst[], const uint8_t cell_type[], int nx, int ny,
+                                 float uw_x, float uw_y);
+/* src/ispc/d2q9_boundaries.ispc:5 */
+void d2q9_apply_bounce_back(float f[], const uint8_t cell_type[], float uw_x, float uw_y, int count);
+/* src/ispc/d2q9_boundaries.ispc:43 */
+void d2q9_apply_zou_he_inflow_west(float f[], float rho[], float ux[], float uy[], float u_in_x, float u_in_y,
+                                   int nx, int ny);
+/* src/ispc/d2q9_boundaries.ispc:85 */
+void d2q9_apply_zou_he_outflow_east(float f[], float rho[], float ux[], float uy[], float rho_out, int nx,
+                                    int ny);
+/* src/ispc/d2q9_boundaries.ispc:125 */
+void d2q9_apply_convective_outflow_east(float f[], const float f_prev[], float u_conv, int nx, int ny);
+/* src/ispc/d2q9_les.ispc:6 */
+void d2q9_compute_smagorinsky_tau(const float f[], const float rho[], const float ux[], const float uy[],
+                                  float tau_eff[], float tau_0, float smagorinsky_constant, int count);
+/* src/ispc/forces_momentum_exchange.ispc:6 (the ISPC `uniform float &`
+ * outputs are pointers at the ABI level) */
+void d2q9_compute_momentum_exchange_forces(const float f_post_collision[], const uint8_t cell_type[], int nx,
+                                           int ny, float center_x, float center_y, float max_radius,
+                                           float* fx_out, float* fy_out, float* torque_z_out);
+
+/* src/ispc/d3q19_moments.ispc:4 */
+void d3q19_compute_macroscopic_moments(const float f[], float rho[], float ux[], float uy[], float uz[],
+                                       int count);
+/* src/ispc/d3q19_moments.ispc:35 */
+void d3q19_compute_macroscopic_moments_forced(const float f[], float rho[], float ux[], float uy[], float uz[],
+                                              float gx, float gy, float gz, int count);
+/* src/ispc/d3q19_moments.ispc:69 */
+void d3q19_compute_velocity_magnitude(const float ux[], const float uy[], const float uz[], float speed[],
+                                      int count);
+/* src/ispc/d3q19_collision_bgk.ispc:4 */
+void d3q19_collide_bgk(const float f_in[], float f_out[], const uint8_t cell_type[], float rho[], float ux[],
+                       float uy[], float uz[], float tau, float gx, float gy, float gz, int count);
+/* src/ispc/d3q19_collision_trt.ispc:5 */
+void d3q19_collide_trt(const float f_in[], float f_out[], const uint8_t cell_type[], float rho[], float ux[],
+                       float uy[], float uz[], float tau_s, float tau_a, float gx, float gy, float gz, int count);
+/* src/ispc/d3q19_collision_mrt.ispc:26 */
+void d3q19_collide_mrt(const float f_in[], float f_out[], const uint8_t cell_type[], float rho[], float ux[],
+                       float uy[], float uz[], float tau_shear, float tau_bulk, float gx, float gy, float gz,
+                       int count);
+/* src/ispc/d3q19_streaming.ispc:4 */
+void d3q19_stream_periodic(const float f_src[], float f_dst[], int nx, int ny, int nz);
+/* src/ispc/d3q19_streaming.ispc:39 */
+void d3q19_stream_with_boundaries(const float f_src[], float f_dst[], const uint8_t cell_type[], int nx, int ny,
+                                  int nz, float uw_x, float uw_y, float uw_z);
+/* src/ispc/d3q19_les.ispc:7 */
+void d3q19_compute_smagorinsky_tau(const float f[], const float rho[], const float ux[], const float uy[],
+                                   const float uz[], float tau_eff[], float tau_0, float smagorinsky_constant,
+                                   int count);
+/* src/ispc/d3q19_curved.ispc:7 */
+void d3q19_apply_curved_bfl(float f_dst[], const float f_src[], const int fluid_indices[], const int q_dirs[],
+                            const float delta_fractions[], const float uw_x[], const float uw_y[],
+                            const float uw_z[], int nx, int ny, int nz, int num_boundary_links);
+/* src/ispc/forces_momentum_exchange.ispc:71 */
+void d3q19_compute_momentum_exchange_forces(const float f_post_collision[], const uint8_t cell_type[], int nx,
+                                            int ny, int nz, float center_x, float center_y, float center_z,
+                                            float max_radius, float* fx_out, float* fy_out, float* fz_out);
+
+/* src/ispc/d3q27_collision_mrt.ispc:5 (TRT; pairs populations with the
+ * reference's opposite table, include/wake/core/lattice_models.hpp:104-109) */
+void d3q27_collide(const float f_in[], float f_out[], const uint8_t cell_type[], float rho[], float ux[],
+                   float uy[], float uz[], float tau_s, float tau_a, float gx, float gy, float gz, int count);
+/* src/ispc/d3q27_streaming.ispc:4 */
+void d3q27_stream_periodic(const float f_src[], float f_dst[], int nx, int ny, int nz);
+/* src/ispc/d3q27_moments.ispc:4 */
+void d3q27_compute_macroscopic_moments(const float f[], float rho[], float ux[], float uy[], float uz[],
+                                       int count);
+
+/* ----------------------------------------------------------------------
+ * Part 2: device-resident domains (the LatticeGrid2D / LatticeGrid3D path)
+ * -------------------------------------------------------------------- */
+typedef struct wake_domain wake_domain;
+
+enum wake_collision { WAKE_BGK = 0, WAKE_TRT = 1, WAKE_MRT = 2 };
+enum wake_storage { WAKE_STORE_F32 = 0, WAKE_STORE_F16 = 1, WAKE_STORE_BF16 = 2 };
+enum wake_field {
+  WAKE_FIELD_RHO = 0,
+  WAKE_FIELD_UX = 1,
+  WAKE_FIELD_UY = 2,
+  WAKE_FIELD_UZ = 3,
+  WAKE_FIELD_SPEED = 4,
+  WAKE_FIELD_F_SRC = 5, /* Q planes of nx*ny*nz: the post-stream populations */
+  WAKE_FIELD_F_DST = 6  /* Q planes: the post-collision populations of the last step */
+};
+
+/* LatticeGrid3D(nx, ny, nz, tau) (lattice_grid_3d.cpp:46-64) for dim = 3,
+ * LatticeGrid2D(nx, ny, tau) (lattice_grid_2d.cpp:46-62) for dim = 2.
+ * lattice: 9 (2D), 19 or 27 (3D).  storage: WAKE_STORE_F32, or FP16 / BF16
+ * storage of f_q - w_q with FP32 arithmetic.  plane_pad < 0 picks the
+ * default padding of the population plane stride. */
+int wake_domain_create(int dim, int lattice, int nx, int ny, int nz, float tau, int storage, long long plane_pad,
+                       wake_domain** out);
+void wake_domain_destroy(wake_domain* d);
+/* lattice_grid_3d.hpp:31 initialize */
+int wake_domain_initialize(wake_domain* d, float rho, float ux, float uy, float uz);
+/* lattice_grid_3d.hpp:32 initialize_custom, with the per-node values given as
+ * arrays of nx*ny*nz (uz may be NULL) */
+int wake_domain_initialize_fields(wake_domain* d, const float* rho, const float* ux, const float* uy,
+                                  const float* uz);
+/* Seeded uniform noise U(-a, a) on every population (counter = node*Q + q). */
+int wake_domain_add_noise(wake_domain* d, float amplitude, uint64_t seed);
+/* Replace the post-stream state by Q planes of nx*ny*nz (reference layout). */
+int wake_domain_set_populations(wake_domain* d, const float* f);
+/* lattice_grid_3d.hpp:50 set_cell_type / :51 get_cell_type */
+int wake_domain_set_cell_type(wake_domain* d, int x, int y, int z, int type);
+int wake_domain_get_cell_type(wake_domain* d, int x, int y, int z);
+/* lattice_grid_3d.hpp:47 set_periodic */
+int wake_domain_set_periodic(wake_domain* d, int periodic);
+/* Wall streaming with y and/or z wrapped instead of reflected (x always wraps). */
+int wake_domain_set_periodic_axes(wake_domain* d, int per_y, int per_z);
+/* lattice_grid_3d.hpp:53-56, lattice_grid_2d.hpp:47-51 */
+int wake_domain_set_solid_box(wake_domain* d, int x0, int y0, int z0, int x1, int y1, int z1);
+int wake_domain_set_solid_sphere(wake_domain* d, float cx, float cy, float cz, float r);
+int wake_domain_set_solid_circle(wake_domain* d, float cx, float cy, float r);
+int wake_domain_set_solid_cylinder_z(wake_domain* d, float cx, float cy, float r);
+int wake_domain_set_moving_wall_top(wake_domain* d, float u_wall_x, float u_wall_z);
+/* lattice_grid_2d.hpp:49-50 set_inflow_west / set_outflow_east (3D: x = 0 and
+ * x = nx - 1 planes), plus a parabolic inlet, seeded inlet turbulence and a
+ * convective outlet. */
+int wake_domain_set_inflow_west(wake_domain* d, float u_in_x, float u_in_y);
+int wake_domain_set_inflow_parabolic(wake_domain* d, float u_max);
+int wake_domain_set_inflow_turbulence(wake_domain* d, float amplitude, uint64_t seed);
+int wake_domain_set_outflow_east(wake_domain* d, float rho_out);
+int wake_domain_set_convective_outflow_east(wake_domain* d, float u_conv);
+/* use_les steps feed the Smagorinsky tau into the collision when coupled
+ * (the reference computes it and discards it, lattice_grid_3d.cpp:312-318). */
+int wake_domain_set_les_coupled(wake_domain* d, int coupled);
+int wake_domain_set_relaxation_time(wake_domain* d, float tau);
+/* lattice_grid_3d.hpp:59 build_curved_sphere_links; *nlinks = link count */
+int wake_domain_build_curved_sphere_links(wake_domain* d, float cx, float cy, float cz, float r, int* nlinks);
+/* the link list in the reference's order (fluid index, q, delta) */
+int wake_domain_get_links(wake_domain* d, int* fluid_index, int* q_dir, float* delta);
+/* lattice_grid_3d.hpp:35-42 step(collision, gx, gy, gz, use_les, smag) */
+int wake_domain_step(wake_domain* d, int collision, float gx, float gy, float gz, int use_les, float smag);
+/* `steps` steps; *ms = device time of the launches (hip events on the
+ * domain's stream) or 0 when ms is NULL. */
+int wake_domain_run(wake_domain* d, int steps, int collision, float gx, float gy, float gz, int use_les,
+                    float smag, float* ms);
+int wake_domain_synchronize(wake_domain* d);
+/* lattice_grid_3d.hpp:72-78 rho_data ... f_src_data, copied into out */
+int wake_domain_read(wake_domain* d, int field, float* out);
+int wake_domain_read_cell_types(wake_domain* d, uint8_t* out);
+/* lattice_grid_3d.hpp:44 update_macroscopic */
+int wake_domain_update_macroscopic(wake_domain* d);
+/* lattice_grid_3d.hpp:86-88: total mass, total kinetic energy, max speed over
+ * Fluid nodes, reduced in FP64 on the device */
+int wake_domain_diagnostics(wake_domain* d, double out[3]);
+/* Momentum exchange on the post-stream populations, FP64 (the call the
+ * scenarios make, sphere_3d_wake.cpp:98-105): out = (Fx, Fy, Fz) in 3D and
+ * (Fx, Fy, torque_z) in 2D. */
+int wake_domain_momentum_exchange(wake_domain* d, float cx, float cy, float cz, float r_max, double out[3]);
+/* out: nx, ny, nz, Q, plane stride, current step, storage, device bytes */
+int wake_domain_info(wake_domain* d, long long out[8]);
+/* work-group width along x for the step kernel (64, 128, 256; 0 = auto) */
+int wake_domain_set_block_x(wake_domain* d, int block_x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WAKE_HIP_H */

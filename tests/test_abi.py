@@ -1,0 +1,88 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function
+include/wake_hip.h declares, compiles in the same D3Q19 MRT table as the
+oracle, and the device path fails loudly when no GPU is usable."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "wake_hip.h")
+LIB = os.path.join(ROOT, "wake_amd", "libwake_hip.so")
+
+
+def declared_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*([a-z_0-9]+)\s*\(", txt, flags=re.M)
+    return sorted(set(n for n in names if n not in ("if", "while")))
+
+
+def exported_symbols():
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True, check=True).stdout
+    return set(line.split()[-1] for line in out.splitlines() if " T " in line)
+
+
+def test_library_exports_every_declared_function():
+    decl = declared_functions()
+    assert len(decl) >= 60
+    missing = [d for d in decl if d not in exported_symbols()]
+    assert not missing, missing
+
+
+def test_kernel_exports_match_reference_names():
+    # the 28 LBM exports of src/ispc that the hot path replaces
+    names = ["d2q9_compute_macroscopic_moments", "d2q9_compute_macroscopic_moments_forced",
+             "d2q9_compute_velocity_magnitude", "d2q9_collide_bgk", "d2q9_collide_trt", "d2q9_collide_mrt",
+             "d2q9_stream_periodic", "d2q9_stream_with_boundaries", "d2q9_apply_bounce_back",
+             "d2q9_apply_zou_he_inflow_west", "d2q9_apply_zou_he_outflow_east",
+             "d2q9_apply_convective_outflow_east", "d2q9_compute_smagorinsky_tau",
+             "d2q9_compute_momentum_exchange_forces", "d3q19_compute_macroscopic_moments",
+             "d3q19_compute_macroscopic_moments_forced", "d3q19_compute_velocity_magnitude", "d3q19_collide_bgk",
+             "d3q19_collide_trt", "d3q19_collide_mrt", "d3q19_stream_periodic", "d3q19_stream_with_boundaries",
+             "d3q19_compute_smagorinsky_tau", "d3q19_apply_curved_bfl", "d3q19_compute_momentum_exchange_forces",
+             "d3q27_collide", "d3q27_stream_periodic", "d3q27_compute_macroscopic_moments"]
+    ex = exported_symbols()
+    assert all(n in ex for n in names)
+    import wake_amd._lib as wl
+    assert set(names) == set(wl.KERNEL_SIGS)
+
+
+def test_compiled_mrt_table_equals_oracle():
+    L = ctypes.CDLL(LIB)
+    dev = np.zeros(361, np.float32)
+    L.wake_hip_d3q19_minv(dev.ctypes.data_as(ctypes.c_void_p))
+    ora = np.zeros(361, np.float32)
+    ob.lib(32).oracle_d3q19_minv(ob.ptr(ora))
+    assert np.array_equal(dev.view(np.uint32), ora.view(np.uint32))
+
+
+def test_package_imports_and_fails_loudly_without_gpu():
+    import wake_amd
+
+    if wake_amd._lib.lib().wake_hip_device_count() > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(RuntimeError):
+        wake_amd.LatticeGrid3D(8, 8, 8, 0.8)
+    with pytest.raises(RuntimeError):
+        f = np.zeros(9 * 4, np.float32)
+        r = np.zeros(4, np.float32)
+        wake_amd.kernels.kernel("d2q9_compute_macroscopic_moments", f, r, r.copy(), r.copy(), 4)
+
+
+def test_missing_library_raises(tmp_path, monkeypatch):
+    import importlib
+
+    import wake_amd._lib as wl
+
+    monkeypatch.setattr(wl, "_lib", None)
+    monkeypatch.setattr(wl, "LIB_PATH", str(tmp_path / "nope.so"))
+    with pytest.raises(ImportError):
+        wl.lib()
+    monkeypatch.undo()
+    importlib.reload(wl)

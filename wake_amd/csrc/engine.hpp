@@ -1,0 +1,95 @@
+// engine.hpp -- host/device contract of the device-resident LBM step.
+//
+// A domain's populations live in HBM as Q planes of `stride` elements
+// (plane q of node n at q * stride + n, n = (z * ny + y) * nx + x, the
+// reference's SoA index with a padded plane stride).  Two buffers ping-pong.
+// The fused kernel pulls the post-collision populations A_{k-1} of the
+// previous launch, forms the post-stream state S_k in registers (streaming,
+// halfway bounce-back, moving walls, BFL curved links, open boundaries), and
+// collides it into A_k.  S_k itself -- the reference's f_src after step k --
+// is only materialised on demand by the read-back kernel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace wake_hip {
+
+enum Storage : int { kStoreF32 = 0, kStoreF16 = 1, kStoreBF16 = 2 };
+
+// Per-node class byte, derived on the device from the reference's cell
+// types and the boundary configuration.
+enum ClassBits : uint8_t {
+  kClsSolid = 1,   // collision copies the node (reference: CELL_SOLID)
+  kClsFrozen = 2,  // the pull copies the node's own populations (solid, wall streaming)
+  kClsSlow = 4,    // some pull source is out of range, solid or a moving wall
+  kClsBfl = 8,     // the node owns BFL curved-wall links (aux[n] indexes bfl[])
+  kClsInlet = 16,  // Zou-He velocity inlet node
+  kClsOutlet = 32  // Zou-He pressure or convective outlet node
+};
+
+// Curved-wall (BFL) links of one fluid node: bit q of mask marks a link from
+// the node along c_q into the wall, with wall fraction delta[q] and the
+// wall-velocity projection c_q . u_w in ciuw[q].
+struct BflNode {
+  uint32_t mask;
+  float delta[27];
+  float ciuw[27];
+};
+
+struct StepArgs {
+  int nx, ny, nz;
+  long long stride;          // population plane stride in elements (>= nx*ny*nz)
+  int periodic_mode;         // reference stream_periodic: the pull ignores cell types
+  int per_y, per_z;          // boundary mode: wrap y / z instead of reflecting
+  int bfl_per_y, bfl_per_z;  // wrap of the BFL upstream read x_f - c_q
+  float tau, gx, gy, gz;     // collision: shear tau and Guo body force
+  float cs;                  // Smagorinsky constant (LES-coupled collision)
+  float mom[27];             // moving-wall term 2 w_q rho0 3 c_q.u_w per pulled q
+  int inflow;                // 1: Zou-He velocity inlet at x = 0
+  int inflow_parabolic;      // profile 4 umax s (H - s) / H^2, s = y - 1/2, H = ny - 2
+  float uin_x, uin_y, umax, turb_amp;
+  unsigned long long turb_seed;
+  unsigned long long s_step;  // reference step index k of the S_k being formed
+  int outflow;                // 1: Zou-He pressure (2D), 2: convective
+  float rho_out, u_conv;
+  int by0, by1, bz0, bz1;     // rows an open boundary acts on
+  int commit_prev;            // convective: store S_k(outlet) as next step's f_prev
+  float* f_prev;              // convective outlet store: Q planes of ny*nz
+  const uint8_t* flags;       // reference cell types
+  const uint8_t* cls;         // ClassBits
+  const int* aux;             // per node: index into bfl[] (valid where kClsBfl)
+  const BflNode* bfl;
+  // read-back outputs
+  float* rho;
+  float* ux;
+  float* uy;
+  float* uz;
+  float* speed;
+  float* s_out;               // optional: S_k planes, fp32, stride nx*ny*nz
+};
+
+// Launchers (step_kernels.hip).  `lattice` is 9, 19 or 27.
+hipError_t launch_build_class(int lattice, const StepArgs& a, uint8_t* cls, hipStream_t s);
+hipError_t launch_mark_bfl(const int* nodes, int count, uint8_t* cls, int* aux, hipStream_t s);
+// pull=false: collide-only from the materialised S state.  block_x is the
+// work-group width along x (64, 128 or 256).
+hipError_t launch_step(int lattice, int storage, int collision, bool les, bool pull, const StepArgs& a,
+                       const void* a_in, void* a_out, int block_x, hipStream_t s);
+// Materialise S_k (commit=true updates the convective outlet store) into
+// s_state (storage format, stride a.stride), or fields/planes for read-back.
+hipError_t launch_materialize(int lattice, int storage, const StepArgs& a, const void* a_in, void* s_state,
+                              hipStream_t s);
+hipError_t launch_readback(int lattice, int storage, bool from_s, const StepArgs& a, const void* buf,
+                           hipStream_t s);
+// Momentum exchange over a link list (node, q) on S_k, accumulated in FP64
+// in a fixed order: out[0..2] = F (3D) or (Fx, Fy, torque) (2D).
+hipError_t launch_mem(int lattice, int storage, bool from_s, const StepArgs& a, const void* buf,
+                      const int* link_node, const int* link_q, int nlinks, float cx, float cy, float cz,
+                      double* out, hipStream_t s);
+// FP64 sums over Fluid nodes of rho and rho |u|^2 / 2, and max |u|.
+hipError_t launch_diagnostics(const uint8_t* flags, const float* rho, const float* speed, long long n,
+                              double* partial, int nblocks, double* out3, hipStream_t s);
+
+}  // namespace wake_hip

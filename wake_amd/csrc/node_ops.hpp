@@ -1,0 +1,357 @@
+// node_ops.hpp -- per-node LBM operators on register arrays (device code).
+//
+// Every operator here works on the Q populations of one node held in
+// registers; the callers decide where they come from (a pull gather, a plain
+// load) and where they go.  The arithmetic follows the reference's ISPC
+// kernels term for term (cited per function) except that products with the
+// lattice constants 0 and +-1 are resolved at compile time, so a D3Q19 node
+// costs only its non-trivial FLOPs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+#include <utility>
+
+#include "lattice.hpp"
+
+namespace wake_hip {
+
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  [&]<int... I>(std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+  }(std::make_integer_sequence<int, N>{});
+}
+
+// c * v for a lattice component c in {-1, 0, 1}, folded at compile time.
+template <int C>
+__device__ __forceinline__ float cmul(float v) {
+  if constexpr (C == 1) return v;
+  else if constexpr (C == -1) return -v;
+  else return 0.0f;
+}
+
+// c_q . (ux, uy, uz) summing only the non-zero components, in x, y, z order.
+template <class L, int q>
+__device__ __forceinline__ float cdot(float ux, float uy, float uz) {
+  constexpr int a = L::cx[q], b = L::cy[q], c = L::cz[q];
+  if constexpr (a == 0 && b == 0 && c == 0) return 0.0f;
+  else if constexpr (b == 0 && c == 0) return cmul<a>(ux);
+  else if constexpr (a == 0 && c == 0) return cmul<b>(uy);
+  else if constexpr (a == 0 && b == 0) return cmul<c>(uz);
+  else if constexpr (c == 0) return cmul<a>(ux) + cmul<b>(uy);
+  else if constexpr (b == 0) return cmul<a>(ux) + cmul<c>(uz);
+  else if constexpr (a == 0) return cmul<b>(uy) + cmul<c>(uz);
+  else return cmul<a>(ux) + cmul<b>(uy) + cmul<c>(uz);
+}
+
+// Density and momentum sums (d3q19_moments.ispc:12-24 order: rho then j).
+template <class L>
+__device__ __forceinline__ void density_momentum(const float (&f)[L::Q], float& rho, float& jx, float& jy,
+                                                 float& jz) {
+  rho = 0.0f;
+  jx = 0.0f;
+  jy = 0.0f;
+  jz = 0.0f;
+  static_for<L::Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    rho += f[q];
+    if constexpr (L::cx[q] == 1) jx += f[q];
+    if constexpr (L::cx[q] == -1) jx -= f[q];
+    if constexpr (L::cy[q] == 1) jy += f[q];
+    if constexpr (L::cy[q] == -1) jy -= f[q];
+    if constexpr (L::cz[q] == 1) jz += f[q];
+    if constexpr (L::cz[q] == -1) jz -= f[q];
+  });
+}
+
+// Velocity with the Guo half-force shift (d3q19_moments.ispc:60-64).
+__device__ __forceinline__ void guo_velocity(float rho, float jx, float jy, float jz, float gx, float gy, float gz,
+                                             float& ux, float& uy, float& uz) {
+  const float inv_rho = 1.0f / rho;
+  ux = (jx + 0.5f * rho * gx) * inv_rho;
+  uy = (jy + 0.5f * rho * gy) * inv_rho;
+  uz = (jz + 0.5f * rho * gz) * inv_rho;
+}
+
+// Second-order equilibrium (lbm_d3q19.isph:78-88).
+template <class L, int q>
+__device__ __forceinline__ float feq(float rho, float ux, float uy, float uz, float usq) {
+  const float cu = cdot<L, q>(ux, uy, uz);
+  return L::w[q] * rho * (1.0f + 3.0f * cu + 4.5f * cu * cu - 1.5f * usq);
+}
+
+// Smagorinsky relaxation time from the non-equilibrium stress
+// (d3q19_les.ispc:7-66, d2q9_les.ispc:6-57):
+//   Pi = sum c c (f - feq), Q = sqrt(2 Pi:Pi),
+//   tau = (tau0 + sqrt(tau0^2 + 18 sqrt(2) Cs^2 Q / rho)) / 2.
+template <class L>
+__device__ __forceinline__ float smagorinsky_tau(const float (&f)[L::Q], float rho, float ux, float uy, float uz,
+                                                 float tau0, float cs) {
+  const float usq = ux * ux + uy * uy + uz * uz;
+  float pxx = 0.0f, pyy = 0.0f, pzz = 0.0f, pxy = 0.0f, pyz = 0.0f, pxz = 0.0f;
+  static_for<L::Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int a = L::cx[q], b = L::cy[q], c = L::cz[q];
+    const float fn = f[q] - feq<L, q>(rho, ux, uy, uz, usq);
+    if constexpr (a != 0) pxx += fn;
+    if constexpr (b != 0) pyy += fn;
+    if constexpr (c != 0) pzz += fn;
+    if constexpr (a * b == 1) pxy += fn;
+    if constexpr (a * b == -1) pxy -= fn;
+    if constexpr (b * c == 1) pyz += fn;
+    if constexpr (b * c == -1) pyz -= fn;
+    if constexpr (a * c == 1) pxz += fn;
+    if constexpr (a * c == -1) pxz -= fn;
+  });
+  float tensor_sum;
+  if constexpr (L::D == 2)
+    tensor_sum = pxx * pxx + 2.0f * pxy * pxy + pyy * pyy;
+  else
+    tensor_sum = (pxx * pxx + pyy * pyy + pzz * pzz) + 2.0f * (pxy * pxy + pyz * pyz + pxz * pxz);
+  const float q_stress = sqrtf(2.0f * tensor_sum);
+  const float prefactor = 18.0f * 1.4142135623730951f * (cs * cs);
+  const float disc = tau0 * tau0 + (prefactor / rho) * q_stress;
+  return 0.5f * (tau0 + sqrtf(disc));
+}
+
+// TRT antisymmetric tau from the grid (lattice_grid_3d.cpp:328-330), Lambda = 1/4.
+__device__ __host__ __forceinline__ float trt_tau_a(float tau_s) {
+  float diff = tau_s - 0.5f;
+  diff = diff < 1e-4f ? 1e-4f : diff;
+  return 0.5f + 0.25f / diff;
+}
+
+struct Macro {
+  float rho, ux, uy, uz;
+};
+
+// BGK with Guo forcing, in place (d3q19_collision_bgk.ispc:4-75,
+// d2q9_collision_bgk.ispc:5-86).
+template <class L>
+__device__ __forceinline__ Macro collide_bgk(float (&f)[L::Q], float tau, float gx, float gy, float gz) {
+  const float omega = 1.0f / tau;
+  const float pre = 1.0f - 0.5f * omega;
+  float rho, jx, jy, jz, ux, uy, uz;
+  density_momentum<L>(f, rho, jx, jy, jz);
+  guo_velocity(rho, jx, jy, jz, gx, gy, gz, ux, uy, uz);
+  const float u_dot_g = ux * gx + uy * gy + uz * gz;
+  const float usq = ux * ux + uy * uy + uz * uz;
+  static_for<L::Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    const float cu = cdot<L, q>(ux, uy, uz);
+    const float cg = cdot<L, q>(gx, gy, gz);
+    const float fe = L::w[q] * rho * (1.0f + 3.0f * cu + 4.5f * cu * cu - 1.5f * usq);
+    const float Fi = pre * L::w[q] * rho * (3.0f * (cg - u_dot_g) + 9.0f * cu * cg);
+    f[q] = f[q] - omega * (f[q] - fe) + Fi;
+  });
+  return {rho, ux, uy, uz};
+}
+
+// TRT with split Guo forcing, in place (d3q19_collision_trt.ispc:5-93,
+// d3q27_collision_mrt.ispc:5-91).  OPP selects the pairing table.
+template <class L, bool REF_PAIRS = false>
+__device__ __forceinline__ Macro collide_trt(float (&f)[L::Q], float tau_s, float tau_a, float gx, float gy,
+                                             float gz) {
+  const float omega_s = 1.0f / tau_s, omega_a = 1.0f / tau_a;
+  const float pre_s = 1.0f - 0.5f * omega_s, pre_a = 1.0f - 0.5f * omega_a;
+  float rho, jx, jy, jz, ux, uy, uz;
+  density_momentum<L>(f, rho, jx, jy, jz);
+  guo_velocity(rho, jx, jy, jz, gx, gy, gz, ux, uy, uz);
+  const float u_dot_g = ux * gx + uy * gy + uz * gz;
+  const float usq = ux * ux + uy * uy + uz * uz;
+  float fe[L::Q];
+  static_for<L::Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    fe[q] = feq<L, q>(rho, ux, uy, uz, usq);
+  });
+  float out[L::Q];
+  static_for<L::Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int o = REF_PAIRS ? L::opp_ref[q] : L::opp[q];
+    const float f_sym = 0.5f * (f[q] + f[o]);
+    const float f_asym = 0.5f * (f[q] - f[o]);
+    const float fe_sym = 0.5f * (fe[q] + fe[o]);
+    const float fe_asym = 0.5f * (fe[q] - fe[o]);
+    const float cu = cdot<L, q>(ux, uy, uz);
+    const float cg = cdot<L, q>(gx, gy, gz);
+    const float F_asym = pre_a * L::w[q] * rho * (3.0f * cg);
+    const float F_sym = pre_s * L::w[q] * rho * (-3.0f * u_dot_g + 9.0f * cu * cg);
+    out[q] = f[q] - omega_s * (f_sym - fe_sym) - omega_a * (f_asym - fe_asym) + (F_sym + F_asym);
+  });
+  static_for<L::Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    f[q] = out[q];
+  });
+  return {rho, ux, uy, uz};
+}
+
+// D2Q9 MRT (Lallemand-Luo), in place (d2q9_collision_mrt.ispc:6-137).
+__device__ __forceinline__ Macro collide_mrt_d2q9(float (&f)[9], float tau_shear, float tau_bulk, float gx,
+                                                  float gy) {
+  const float s_e = 1.0f / tau_bulk, s_eps = 1.4f, s_q = 1.6f, s_p = 1.0f / tau_shear;
+  const float f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3], f4 = f[4], f5 = f[5], f6 = f[6], f7 = f[7], f8 = f[8];
+  float m0 = f0 + f1 + f2 + f3 + f4 + f5 + f6 + f7 + f8;
+  float m1 = -4.0f * f0 - (f1 + f2 + f3 + f4) + 2.0f * (f5 + f6 + f7 + f8);
+  float m2 = 4.0f * f0 - 2.0f * (f1 + f2 + f3 + f4) + (f5 + f6 + f7 + f8);
+  float m3 = (f1 - f3) + (f5 - f6 - f7 + f8);
+  float m4 = -2.0f * (f1 - f3) + (f5 - f6 - f7 + f8);
+  float m5 = (f2 - f4) + (f5 + f6 - f7 - f8);
+  float m6 = -2.0f * (f2 - f4) + (f5 + f6 - f7 - f8);
+  float m7 = (f1 - f2 + f3 - f4);
+  float m8 = (f5 - f6 + f7 - f8);
+  const float rho = m0;
+  const float inv_rho = 1.0f / rho;
+  const float ux = (m3 + 0.5f * rho * gx) * inv_rho;
+  const float uy = (m5 + 0.5f * rho * gy) * inv_rho;
+  const float ux2 = ux * ux, uy2 = uy * uy, u2 = ux2 + uy2;
+  const float meq1 = rho * (-2.0f + 3.0f * u2);
+  const float meq2 = rho * (1.0f - 3.0f * u2);
+  const float meq4 = -rho * ux;
+  const float meq6 = -rho * uy;
+  const float meq7 = rho * (ux2 - uy2);
+  const float meq8 = rho * (ux * uy);
+  const float F_pre = 1.0f - 0.5f * s_p;
+  const float F_m3 = rho * gx, F_m5 = rho * gy;
+  const float F_m1 = 6.0f * (ux * F_m3 + uy * F_m5);
+  const float F_m2 = -6.0f * (ux * F_m3 + uy * F_m5);
+  const float F_m7 = 2.0f * (ux * F_m3 - uy * F_m5);
+  const float F_m8 = (ux * F_m5 + uy * F_m3);
+  m1 = m1 - s_e * (m1 - meq1) + F_pre * F_m1;
+  m2 = m2 - s_eps * (m2 - meq2) + F_pre * F_m2;
+  m3 = m3 + F_m3;
+  m4 = m4 - s_q * (m4 - meq4) + F_pre * (-F_m3);
+  m5 = m5 + F_m5;
+  m6 = m6 - s_q * (m6 - meq6) + F_pre * (-F_m5);
+  m7 = m7 - s_p * (m7 - meq7) + F_pre * F_m7;
+  m8 = m8 - s_p * (m8 - meq8) + F_pre * F_m8;
+  const float a9 = 1.0f / 9.0f, a18 = 1.0f / 18.0f, a36 = 1.0f / 36.0f, a6 = 1.0f / 6.0f, a12 = 1.0f / 12.0f;
+  f[0] = a9 * m0 - a9 * m1 + a9 * m2;
+  f[1] = a9 * m0 - a36 * m1 - a18 * m2 + a6 * m3 - a6 * m4 + 0.25f * m7;
+  f[2] = a9 * m0 - a36 * m1 - a18 * m2 + a6 * m5 - a6 * m6 - 0.25f * m7;
+  f[3] = a9 * m0 - a36 * m1 - a18 * m2 - a6 * m3 + a6 * m4 + 0.25f * m7;
+  f[4] = a9 * m0 - a36 * m1 - a18 * m2 - a6 * m5 + a6 * m6 - 0.25f * m7;
+  f[5] = a9 * m0 + a18 * m1 + a36 * m2 + a6 * m3 + a12 * m4 + a6 * m5 + a12 * m6 + 0.25f * m8;
+  f[6] = a9 * m0 + a18 * m1 + a36 * m2 - a6 * m3 - a12 * m4 + a6 * m5 + a12 * m6 - 0.25f * m8;
+  f[7] = a9 * m0 + a18 * m1 + a36 * m2 - a6 * m3 - a12 * m4 - a6 * m5 - a12 * m6 + 0.25f * m8;
+  f[8] = a9 * m0 + a18 * m1 + a36 * m2 + a6 * m3 + a12 * m4 - a6 * m5 - a12 * m6 - 0.25f * m8;
+  return {rho, ux, uy, 0.0f};
+}
+
+// A D3Q19 moment is "odd" when its basis polynomial changes sign under
+// c -> -c (the momenta, the energy fluxes and the third-order m16..m18).
+struct D3Q19OddMoment {
+  static constexpr bool v[19] = {false, false, false, true,  true,  true,  true,  true,  true, false,
+                                 false, false, false, false, false, false, true,  true,  true};
+};
+
+// D3Q19 MRT (d'Humieres), in place (d3q19_collision_mrt.ispc:26-176).  The
+// back transform applies the rounded inverse table: the odd-moment part of
+// rows q and opp(q) differs only in sign, so each opposite pair shares one
+// even and one odd partial sum.
+__device__ __forceinline__ Macro collide_mrt_d3q19(float (&f)[19], float tau_shear, float tau_bulk, float gx,
+                                                   float gy, float gz) {
+  const float s_e = 1.0f / tau_bulk, s_eps = 1.4f, s_q = 1.2f, s_nu = 1.0f / tau_shear, s_pi = 1.4f, s_m = 1.98f;
+  float m[19];
+  m[0] = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8] + f[9] + f[10] + f[11] + f[12] + f[13] +
+         f[14] + f[15] + f[16] + f[17] + f[18];
+  const float faces = f[1] + f[2] + f[3] + f[4] + f[5] + f[6];
+  const float edges = f[7] + f[8] + f[9] + f[10] + f[11] + f[12] + f[13] + f[14] + f[15] + f[16] + f[17] + f[18];
+  m[1] = -30.0f * f[0] - 11.0f * faces + 8.0f * edges;
+  m[2] = 12.0f * f[0] - 4.0f * faces + edges;
+  const float ex = f[7] - f[8] + f[9] - f[10] + f[11] - f[12] + f[13] - f[14];
+  const float ey = f[7] + f[8] - f[9] - f[10] + f[15] - f[16] + f[17] - f[18];
+  const float ez = f[11] + f[12] - f[13] - f[14] + f[15] + f[16] - f[17] - f[18];
+  m[3] = (f[1] - f[2]) + ex;
+  m[4] = -4.0f * (f[1] - f[2]) + ex;
+  m[5] = (f[3] - f[4]) + ey;
+  m[6] = -4.0f * (f[3] - f[4]) + ey;
+  m[7] = (f[5] - f[6]) + ez;
+  m[8] = -4.0f * (f[5] - f[6]) + ez;
+  const float exy = f[7] + f[8] + f[9] + f[10], exz = f[11] + f[12] + f[13] + f[14];
+  const float eyz = f[15] + f[16] + f[17] + f[18];
+  m[9] = 2.0f * (f[1] + f[2]) - (f[3] + f[4] + f[5] + f[6]) + (exy + exz) - 2.0f * eyz;
+  m[10] = -4.0f * (f[1] + f[2]) + 2.0f * (f[3] + f[4] + f[5] + f[6]) + (exy + exz) - 2.0f * eyz;
+  m[11] = (f[3] + f[4]) - (f[5] + f[6]) + exy - exz;
+  m[12] = -2.0f * (f[3] + f[4]) + 2.0f * (f[5] + f[6]) + exy - exz;
+  m[13] = (f[7] - f[8] - f[9] + f[10]);
+  m[14] = (f[15] - f[16] - f[17] + f[18]);
+  m[15] = (f[11] - f[12] - f[13] + f[14]);
+  m[16] = (f[7] - f[8] + f[9] - f[10] - f[11] + f[12] - f[13] + f[14]);
+  m[17] = (-f[7] - f[8] + f[9] + f[10] + f[15] - f[16] + f[17] - f[18]);
+  m[18] = (-f[11] - f[12] + f[13] + f[14] + f[15] + f[16] - f[17] - f[18]);
+
+  const float rho = m[0];
+  const float inv_rho = 1.0f / rho;
+  const float ux = (m[3] + 0.5f * rho * gx) * inv_rho;
+  const float uy = (m[5] + 0.5f * rho * gy) * inv_rho;
+  const float uz = (m[7] + 0.5f * rho * gz) * inv_rho;
+  const float ux2 = ux * ux, uy2 = uy * uy, uz2 = uz * uz, u2 = ux2 + uy2 + uz2;
+  const float meq9 = rho * (2.0f * ux2 - uy2 - uz2);
+  const float meq11 = rho * (uy2 - uz2);
+  m[1] = m[1] - s_e * (m[1] - rho * (-11.0f + 19.0f * u2));
+  m[2] = m[2] - s_eps * (m[2] - rho * (3.0f - 11.0f / 2.0f * u2));
+  m[3] = m[3] + rho * gx;
+  m[4] = m[4] - s_q * (m[4] - (-2.0f / 3.0f * rho * ux)) - (2.0f / 3.0f) * rho * gx;
+  m[5] = m[5] + rho * gy;
+  m[6] = m[6] - s_q * (m[6] - (-2.0f / 3.0f * rho * uy)) - (2.0f / 3.0f) * rho * gy;
+  m[7] = m[7] + rho * gz;
+  m[8] = m[8] - s_q * (m[8] - (-2.0f / 3.0f * rho * uz)) - (2.0f / 3.0f) * rho * gz;
+  m[9] = m[9] - s_nu * (m[9] - meq9);
+  m[10] = m[10] - s_pi * (m[10] - (-0.5f * meq9));
+  m[11] = m[11] - s_nu * (m[11] - meq11);
+  m[12] = m[12] - s_pi * (m[12] - (-0.5f * meq11));
+  m[13] = m[13] - s_nu * (m[13] - rho * (ux * uy));
+  m[14] = m[14] - s_nu * (m[14] - rho * (uy * uz));
+  m[15] = m[15] - s_nu * (m[15] - rho * (ux * uz));
+  m[16] = m[16] - s_m * m[16];
+  m[17] = m[17] - s_m * m[17];
+  m[18] = m[18] - s_m * m[18];
+
+  // Back transform (odd moments: see D3Q19OddMoment).
+  static_for<19>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int o = D3Q19::opp[q];
+    if constexpr (q == 0) {
+      float v = 0.0f;
+      static_for<19>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        constexpr float c = kD3Q19Minv.minv[0][k];
+        if constexpr (c != 0.0f) v += c * m[k];
+      });
+      f[0] = v;
+    } else if constexpr (q < o) {
+      float ev = 0.0f, od = 0.0f;
+      static_for<19>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        constexpr float c = kD3Q19Minv.minv[q][k];
+        if constexpr (c != 0.0f) {
+          if constexpr (D3Q19OddMoment::v[k]) od += c * m[k];
+          else ev += c * m[k];
+        }
+      });
+      f[q] = ev + od;
+      f[o] = ev - od;
+    }
+  });
+  return {rho, ux, uy, uz};
+}
+
+// Dispatch one collision on a node.  tau_eff is the shear tau (the scalar tau
+// of the step, or the per-node Smagorinsky value in LES-coupled mode).
+template <class L, int COLL, bool REF_PAIRS = false>
+__device__ __forceinline__ Macro collide(float (&f)[L::Q], float tau_eff, float gx, float gy, float gz) {
+  if constexpr (COLL == kBGK) {
+    return collide_bgk<L>(f, tau_eff, gx, gy, gz);
+  } else if constexpr (COLL == kTRT || L::Q == 27) {
+    return collide_trt<L, REF_PAIRS>(f, tau_eff, trt_tau_a(tau_eff), gx, gy, gz);
+  } else if constexpr (L::Q == 9) {
+    return collide_mrt_d2q9(f, tau_eff, kTauBulk2D, gx, gy);
+  } else {
+    return collide_mrt_d3q19(f, tau_eff, kTauBulk3D, gx, gy, gz);
+  }
+}
+
+}  // namespace wake_hip

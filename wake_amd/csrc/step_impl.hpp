@@ -1,0 +1,461 @@
+// step_impl.hpp -- fused LBM time step for gfx950 (device-resident path).
+//
+// One lane per node, lanes contiguous along x, one work-group row per (y, z):
+// every pull load of a wavefront is 64 consecutive elements of one
+// population plane (a +-1 element shift for c_x = -+1), so the 19 (or 9, 27)
+// plane streams are fully coalesced and each population element of the
+// previous step is read exactly once.  y and z are work-group uniform, so the
+// domain-edge logic is scalar.  Bulk nodes take a branch-free pull; nodes
+// next to walls, solids, curved links or open boundaries (flagged once per
+// geometry change in the class byte) take the general path that restates the
+// reference's separate streaming, BFL and Zou-He passes:
+//   d3q19_streaming.ispc:39-104, d2q9_streaming.ispc:33-95 (wall streaming)
+//   d3q19_streaming.ispc:4-35 (periodic streaming, cell types ignored)
+//   d3q19_curved.ispc:7-71 (BFL, read from the post-collision buffer)
+//   d2q9_boundaries.ispc:43-121 (Zou-He inflow / outflow)
+// followed by the collision of lattice_grid_3d.cpp:313-339.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "engine.hpp"
+#include "lattice.hpp"
+#include "node_ops.hpp"
+
+namespace wake_hip {
+
+// ----------------------------------------------------------------- storage
+struct StF32 {
+  using T = float;
+  template <class L, int q>
+  __device__ __forceinline__ static float ld(const T* __restrict__ p, long long i) {
+    return p[i];
+  }
+  template <class L, int q>
+  __device__ __forceinline__ static void st(T* __restrict__ p, long long i, float v) {
+    __builtin_nontemporal_store(v, p + i);
+  }
+};
+
+// FP16 storage of the deviation f_q - w_q; all arithmetic stays FP32.
+struct StF16 {
+  using T = _Float16;
+  template <class L, int q>
+  __device__ __forceinline__ static float ld(const T* __restrict__ p, long long i) {
+    return L::w[q] + static_cast<float>(p[i]);
+  }
+  template <class L, int q>
+  __device__ __forceinline__ static void st(T* __restrict__ p, long long i, float v) {
+    __builtin_nontemporal_store(static_cast<_Float16>(v - L::w[q]), p + i);
+  }
+};
+
+// BF16 storage of the deviation f_q - w_q (round to nearest even).
+struct StBF16 {
+  using T = uint16_t;
+  template <class L, int q>
+  __device__ __forceinline__ static float ld(const T* __restrict__ p, long long i) {
+    return L::w[q] + __uint_as_float(static_cast<uint32_t>(p[i]) << 16);
+  }
+  template <class L, int q>
+  __device__ __forceinline__ static void st(T* __restrict__ p, long long i, float v) {
+    const uint32_t b = __float_as_uint(v - L::w[q]);
+    const uint32_t r = (b + 0x7FFFu + ((b >> 16) & 1u)) >> 16;
+    __builtin_nontemporal_store(static_cast<uint16_t>(r), p + i);
+  }
+};
+
+// ------------------------------------------------------------- RNG (inlet)
+// splitmix64 finaliser of seed ^ counter; top 24 bits as a uniform in [0,1).
+__device__ __forceinline__ float rng_sym(unsigned long long seed, unsigned long long counter) {
+  unsigned long long x = seed ^ counter;
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  x = x ^ (x >> 31);
+  // 2 U - 1, exact in FP32
+  return static_cast<float>(static_cast<uint32_t>(x >> 40)) * (1.0f / 8388608.0f) - 1.0f;
+}
+
+__device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+// ------------------------------------------------------------------ gather
+// S of node (x, y, z) from the post-collision buffer A: streaming with the
+// reference's bounce-back rules, then the BFL overrides.
+template <class L, class St>
+__device__ __forceinline__ void gather(const StepArgs& a, const typename St::T* __restrict__ A, int x, int y,
+                                       int z, int n, uint8_t c, float (&f)[L::Q]) {
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const long long S = a.stride;
+  const int xm = x == 0 ? nx - 1 : x - 1;
+  const int xp = x == nx - 1 ? 0 : x + 1;
+  if (!(c & (kClsFrozen | kClsSlow | kClsBfl))) {
+    static_for<L::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
+      const int sx = cx == 0 ? x : (cx > 0 ? xm : xp);
+      const int sy = cy == 0 ? y : wrap(y - cy, ny);
+      const int sz = cz == 0 ? z : wrap(z - cz, nz);
+      f[q] = St::template ld<L, q>(A, q * S + static_cast<long long>((sz * ny + sy) * nx + sx));
+    });
+    return;
+  }
+  if (c & kClsFrozen) {
+    static_for<L::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      f[q] = St::template ld<L, q>(A, q * S + n);
+    });
+  } else {
+    static_for<L::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q], o = L::opp[q];
+      const int sx = cx == 0 ? x : (cx > 0 ? xm : xp);
+      int sy = y - cy, sz = z - cz;
+      bool in = true;
+      if constexpr (cy != 0) {
+        if (a.per_y) sy = wrap(sy, ny);
+        else in = in && sy >= 0 && sy < ny;
+      }
+      if constexpr (cz != 0) {
+        if (a.per_z) sz = wrap(sz, nz);
+        else in = in && sz >= 0 && sz < nz;
+      }
+      if (!in) {
+        f[q] = St::template ld<L, o>(A, o * S + n);  // halfway bounce-back at a domain wall
+      } else {
+        const int sn = (sz * ny + sy) * nx + sx;
+        const uint8_t t = a.periodic_mode ? static_cast<uint8_t>(kFluid) : a.flags[sn];
+        if (t == kSolid) f[q] = St::template ld<L, o>(A, o * S + n);
+        else if (t == kMovingWall) f[q] = St::template ld<L, o>(A, o * S + n) + a.mom[q];
+        else f[q] = St::template ld<L, q>(A, q * S + sn);
+      }
+    });
+  }
+  if (c & kClsBfl) {
+    const BflNode* r = a.bfl + a.aux[n];
+    const uint32_t mask = r->mask;
+    static_for<L::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      if constexpr (q > 0) {
+        if (mask & (1u << q)) {
+          constexpr int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q], o = L::opp[q];
+          const float delta = r->delta[q];
+          const float ciuw = r->ciuw[q];
+          const float f_i_xf = St::template ld<L, q>(A, q * S + n);
+          float val;
+          if (delta < 0.5f) {
+            const int xf = cx == 0 ? x : (cx > 0 ? xm : xp);
+            int yf = y - cy, zf = z - cz;
+            if (a.bfl_per_y) yf = wrap(yf, ny);
+            if (a.bfl_per_z) zf = wrap(zf, nz);
+            float f_i_xff = f_i_xf;
+            if (yf >= 0 && yf < ny && zf >= 0 && zf < nz)
+              f_i_xff = St::template ld<L, q>(A, q * S + static_cast<long long>((zf * ny + yf) * nx + xf));
+            const float mom = 2.0f * L::w[q] * 1.0f * 3.0f * ciuw;
+            val = 2.0f * delta * f_i_xf + (1.0f - 2.0f * delta) * f_i_xff - mom;
+          } else {
+            const float f_opp_xf = St::template ld<L, o>(A, o * S + n);
+            const float inv_2delta = 1.0f / (2.0f * delta);
+            const float mom = (1.0f / delta) * L::w[q] * 1.0f * 3.0f * ciuw;
+            val = inv_2delta * f_i_xf + (1.0f - inv_2delta) * f_opp_xf - mom;
+          }
+          f[o] = val;
+        }
+      }
+    });
+  }
+}
+
+// --------------------------------------------------------- open boundaries
+template <class L>
+__device__ __forceinline__ void apply_inlet(const StepArgs& a, float (&f)[L::Q], int y, int n) {
+  float ux = a.uin_x;
+  if (a.inflow_parabolic) {
+    const float s = static_cast<float>(y) - 0.5f;
+    const float H = static_cast<float>(a.ny - 2);
+    ux = 4.0f * a.umax * s * (H - s) / (H * H);
+  }
+  if (a.turb_amp != 0.0f) {
+    const unsigned long long N = static_cast<unsigned long long>(a.nx) * a.ny * a.nz;
+    ux = ux * (1.0f + a.turb_amp * rng_sym(a.turb_seed, a.s_step * N + static_cast<unsigned long long>(n)));
+  }
+  const float uy = a.uin_y;
+  if constexpr (L::Q == 9) {
+    // d2q9_boundaries.ispc:43-81
+    const float rho_in = (f[0] + f[2] + f[4] + 2.0f * (f[3] + f[6] + f[7])) / (1.0f - ux);
+    const float f1 = f[3] + (2.0f / 3.0f) * rho_in * ux;
+    const float f5 = f[7] - 0.5f * (f[2] - f[4]) + (1.0f / 6.0f) * rho_in * ux + 0.5f * rho_in * uy;
+    const float f8 = f[6] + 0.5f * (f[2] - f[4]) + (1.0f / 6.0f) * rho_in * ux - 0.5f * rho_in * uy;
+    f[1] = f1;
+    f[5] = f5;
+    f[8] = f8;
+  } else {
+    // 3D non-equilibrium bounce-back with equal transverse corrections
+    float s0 = 0.0f, sm = 0.0f, ty = 0.0f, tz = 0.0f;
+    int cnt_y = 0, cnt_z = 0;
+    static_for<L::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      if constexpr (L::cx[q] == 0) {
+        s0 += f[q];
+        if constexpr (L::cy[q] == 1) ty += f[q];
+        if constexpr (L::cy[q] == -1) ty -= f[q];
+        if constexpr (L::cz[q] == 1) tz += f[q];
+        if constexpr (L::cz[q] == -1) tz -= f[q];
+      } else if constexpr (L::cx[q] < 0) {
+        sm += f[q];
+      } else {
+        cnt_y += L::cy[q] * L::cy[q];
+        cnt_z += L::cz[q] * L::cz[q];
+      }
+    });
+    const float uz = 0.0f;
+    const float rho = (s0 + 2.0f * sm) / (1.0f - ux);
+    const float Ny = (ty - (2.0f / 3.0f) * rho * uy) / static_cast<float>(cnt_y);
+    const float Nz = (tz - (2.0f / 3.0f) * rho * uz) / static_cast<float>(cnt_z);
+    float g[L::Q];
+    static_for<L::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      g[q] = f[q];
+    });
+    static_for<L::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      if constexpr (L::cx[q] == 1) {
+        const float cu = cdot<L, q>(ux, uy, uz);
+        f[q] = g[L::opp[q]] + 6.0f * L::w[q] * rho * cu - (cmul<L::cy[q]>(Ny) + cmul<L::cz[q]>(Nz));
+      }
+    });
+  }
+}
+
+// d2q9_boundaries.ispc:85-121 (pressure outlet, u_y = 0).
+__device__ __forceinline__ void apply_zh_outflow_d2q9(const StepArgs& a, float (&f)[9]) {
+  const float rho_out = a.rho_out;
+  const float u_out_x = -1.0f + (f[0] + f[2] + f[4] + 2.0f * (f[1] + f[5] + f[8])) / rho_out;
+  const float u_out_y = 0.0f;
+  const float f3 = f[1] - (2.0f / 3.0f) * rho_out * u_out_x;
+  const float f6 = f[8] - 0.5f * (f[2] - f[4]) - (1.0f / 6.0f) * rho_out * u_out_x + 0.5f * rho_out * u_out_y;
+  const float f7 = f[5] + 0.5f * (f[2] - f[4]) - (1.0f / 6.0f) * rho_out * u_out_x - 0.5f * rho_out * u_out_y;
+  f[3] = f3;
+  f[6] = f6;
+  f[7] = f7;
+}
+
+// Full post-stream state of a node: gather + open boundaries.  The convective
+// outlet needs the post-stream state of its upstream neighbour, which the
+// outlet lane gathers itself.
+template <class L, class St>
+__device__ __forceinline__ void form_s(const StepArgs& a, const typename St::T* __restrict__ A, int x, int y, int z,
+                                       int n, uint8_t c, bool commit, float (&f)[L::Q]) {
+  gather<L, St>(a, A, x, y, z, n, c, f);
+  if (c & kClsInlet) apply_inlet<L>(a, f, y, n);
+  if (c & kClsOutlet) {
+    if constexpr (L::Q == 9) {
+      if (a.outflow == 1) apply_zh_outflow_d2q9(a, f);
+    }
+    if (a.outflow == 2) {
+      float nb[L::Q];
+      const uint8_t cn = a.cls[n - 1];
+      gather<L, St>(a, A, x - 1, y, z, n - 1, cn, nb);
+      if (cn & kClsInlet) apply_inlet<L>(a, nb, y, n - 1);
+      const float alpha = a.u_conv / (1.0f + a.u_conv);
+      const float beta = 1.0f / (1.0f + a.u_conv);
+      const long long plane = static_cast<long long>(a.ny) * a.nz;
+      const long long row = static_cast<long long>(z) * a.ny + y;
+      static_for<L::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const float v = beta * a.f_prev[q * plane + row] + alpha * nb[q];
+        f[q] = v;
+        if (commit) a.f_prev[q * plane + row] = v;
+      });
+    }
+  }
+}
+
+// ----------------------------------------------------------------- kernels
+template <class L>
+__global__ void k_build_class(const StepArgs a, uint8_t* __restrict__ cls) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y, z = blockIdx.z;
+  if (x >= a.nx) return;
+  const int n = (z * a.ny + y) * a.nx + x;
+  const uint8_t t = a.flags[n];
+  uint8_t c = 0;
+  if (t == kSolid) c |= a.periodic_mode ? kClsSolid : (kClsSolid | kClsFrozen);
+  if (!a.periodic_mode && t != kSolid) {
+    bool slow = false;
+    static_for<L::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      if constexpr (q > 0) {
+        int sx = wrap(x - L::cx[q], a.nx), sy = y - L::cy[q], sz = z - L::cz[q];
+        bool in = true;
+        if (a.per_y) sy = wrap(sy, a.ny);
+        else in = in && sy >= 0 && sy < a.ny;
+        if (a.per_z) sz = wrap(sz, a.nz);
+        else in = in && sz >= 0 && sz < a.nz;
+        if (!in) slow = true;
+        else {
+          const uint8_t nt = a.flags[(sz * a.ny + sy) * a.nx + sx];
+          if (nt == kSolid || nt == kMovingWall) slow = true;
+        }
+      }
+    });
+    if (slow) c |= kClsSlow;
+  }
+  const bool in_rows = y >= a.by0 && y < a.by1 && z >= a.bz0 && z < a.bz1;
+  if (a.inflow && x == 0 && in_rows) c |= kClsInlet;
+  if (a.outflow && x == a.nx - 1 && in_rows) c |= kClsOutlet;
+  cls[n] = c;
+}
+
+template <class L, class St, int COLL, bool LES, bool PULL>
+__global__ __launch_bounds__(256) void k_step(const StepArgs a, const typename St::T* __restrict__ ain,
+                                              typename St::T* __restrict__ aout) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y, z = blockIdx.z;
+  if (x >= a.nx) return;
+  const int n = (z * a.ny + y) * a.nx + x;
+  const uint8_t c = a.cls[n];
+  const long long S = a.stride;
+  float f[L::Q];
+  if constexpr (PULL) {
+    form_s<L, St>(a, ain, x, y, z, n, c, true, f);
+  } else {
+    static_for<L::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      f[q] = St::template ld<L, q>(ain, q * S + n);
+    });
+  }
+  if (!(c & kClsSolid)) {
+    float tau = a.tau;
+    if constexpr (LES) {
+      float rho, jx, jy, jz, ux, uy, uz;
+      density_momentum<L>(f, rho, jx, jy, jz);
+      guo_velocity(rho, jx, jy, jz, a.gx, a.gy, a.gz, ux, uy, uz);
+      tau = smagorinsky_tau<L>(f, rho, ux, uy, uz, a.tau, a.cs);
+    }
+    collide<L, COLL>(f, tau, a.gx, a.gy, a.gz);
+  }
+  static_for<L::Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    St::template st<L, q>(aout, q * S + n, f[q]);
+  });
+}
+
+// S_k into the state buffer (storage format), committing the convective store.
+template <class L, class St>
+__global__ __launch_bounds__(256) void k_materialize(const StepArgs a, const typename St::T* __restrict__ ain,
+                                                     typename St::T* __restrict__ sout) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y, z = blockIdx.z;
+  if (x >= a.nx) return;
+  const int n = (z * a.ny + y) * a.nx + x;
+  float f[L::Q];
+  form_s<L, St>(a, ain, x, y, z, n, a.cls[n], true, f);
+  static_for<L::Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    St::template st<L, q>(sout, q * a.stride + n, f[q]);
+  });
+}
+
+// Moments of S_k with the Guo shift (d3q19_moments.ispc:35-82): rho, u, |u|
+// (skipped when a.rho is null) and optionally the S planes in the reference
+// layout (stride N, fp32).
+template <class L, class St, bool FROM_S>
+__global__ __launch_bounds__(256) void k_readback(const StepArgs a, const typename St::T* __restrict__ buf) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y, z = blockIdx.z;
+  if (x >= a.nx) return;
+  const int n = (z * a.ny + y) * a.nx + x;
+  float f[L::Q];
+  if constexpr (FROM_S) {
+    static_for<L::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      f[q] = St::template ld<L, q>(buf, q * a.stride + n);
+    });
+  } else {
+    form_s<L, St>(a, buf, x, y, z, n, a.cls[n], false, f);
+  }
+  if (a.rho) {
+    float rho, jx, jy, jz, ux, uy, uz;
+    density_momentum<L>(f, rho, jx, jy, jz);
+    guo_velocity(rho, jx, jy, jz, a.gx, a.gy, a.gz, ux, uy, uz);
+    if constexpr (L::D == 2) uz = 0.0f;
+    a.rho[n] = rho;
+    a.ux[n] = ux;
+    a.uy[n] = uy;
+    if (a.uz) a.uz[n] = uz;
+    a.speed[n] = L::D == 2 ? sqrtf(ux * ux + uy * uy) : sqrtf(ux * ux + uy * uy + uz * uz);
+  }
+  if (a.s_out) {
+    const long long N = static_cast<long long>(a.nx) * a.ny * a.nz;
+    static_for<L::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      a.s_out[q * N + n] = f[q];
+    });
+  }
+}
+
+// Momentum exchange on S_k over a link list, FP64, one work-group, fixed
+// reduction order (forces_momentum_exchange.ispc:6-138 restricted to links).
+template <class L, class St, bool FROM_S>
+__global__ __launch_bounds__(256) void k_mem(const StepArgs a, const typename St::T* __restrict__ buf,
+                                             const int* __restrict__ link_node, const int* __restrict__ link_q,
+                                             int nlinks, float ccx, float ccy, float ccz, double* __restrict__ out) {
+  __shared__ double red[3][256];
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int i = threadIdx.x; i < nlinks; i += blockDim.x) {
+    const int n = link_node[i];
+    const int qd = link_q[i];
+    const int x = n % a.nx;
+    const int y = (n / a.nx) % a.ny;
+    const int z = n / (a.nx * a.ny);
+    float f[L::Q];
+    if constexpr (FROM_S) {
+      static_for<L::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        f[q] = St::template ld<L, q>(buf, q * a.stride + n);
+      });
+    } else {
+      form_s<L, St>(a, buf, x, y, z, n, a.cls[n], false, f);
+    }
+    float fv = 0.0f;
+    int cx = 0, cy = 0, cz = 0;
+    static_for<L::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      if (qd == q) {
+        fv = f[q];
+        cx = L::cx[q];
+        cy = L::cy[q];
+        cz = L::cz[q];
+      }
+    });
+    const float dfx = 2.0f * fv * static_cast<float>(cx);
+    const float dfy = 2.0f * fv * static_cast<float>(cy);
+    if constexpr (L::D == 2) {
+      const float rx = (static_cast<float>(x) + 0.5f * static_cast<float>(cx)) - ccx;
+      const float ry = (static_cast<float>(y) + 0.5f * static_cast<float>(cy)) - ccy;
+      s0 += dfx;
+      s1 += dfy;
+      s2 += static_cast<double>(rx * dfy - ry * dfx);
+    } else {
+      s0 += dfx;
+      s1 += dfy;
+      s2 += static_cast<double>(2.0f * fv * static_cast<float>(cz));
+    }
+  }
+  red[0][threadIdx.x] = s0;
+  red[1][threadIdx.x] = s1;
+  red[2][threadIdx.x] = s2;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int k = 0; k < 3; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k) out[k] = red[k][0];
+}
+
+}  // namespace wake_hip

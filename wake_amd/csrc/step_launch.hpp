@@ -1,0 +1,140 @@
+// step_launch.hpp -- per-lattice launchers of the step kernels.  Each lattice
+// is instantiated in its own translation unit (step_d2q9.hip, step_d3q19.hip,
+// step_d3q27.hip) so the kernel variants compile in parallel.
+#pragma once
+
+#include "step_impl.hpp"
+
+namespace wake_hip {
+namespace detail {
+
+inline dim3 grid_of(const StepArgs& a, int bx) { return dim3((a.nx + bx - 1) / bx, a.ny, a.nz); }
+
+inline int pick_block(int nx, int block_x) {
+  if (block_x > 0) return block_x;
+  if (nx >= 256) return 256;
+  if (nx >= 128) return 128;
+  return 64;
+}
+
+template <class F>
+hipError_t with_storage(int storage, F&& f) {
+  switch (storage) {
+    case kStoreF32: return f(StF32{});
+    case kStoreF16: return f(StF16{});
+    case kStoreBF16: return f(StBF16{});
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <class L, class St, int COLL>
+hipError_t step_dispatch(bool les, bool pull, const StepArgs& a, const void* in, void* out, int bx, hipStream_t s) {
+  using T = typename St::T;
+  const T* i = static_cast<const T*>(in);
+  T* o = static_cast<T*>(out);
+  const dim3 g = grid_of(a, bx);
+  if (les) {
+    if (pull) k_step<L, St, COLL, true, true><<<g, bx, 0, s>>>(a, i, o);
+    else k_step<L, St, COLL, true, false><<<g, bx, 0, s>>>(a, i, o);
+  } else {
+    if (pull) k_step<L, St, COLL, false, true><<<g, bx, 0, s>>>(a, i, o);
+    else k_step<L, St, COLL, false, false><<<g, bx, 0, s>>>(a, i, o);
+  }
+  return hipGetLastError();
+}
+
+template <class L>
+struct Launch {
+  static hipError_t build_class(const StepArgs& a, uint8_t* cls, hipStream_t s) {
+    const int bx = pick_block(a.nx, 0);
+    k_build_class<L><<<grid_of(a, bx), bx, 0, s>>>(a, cls);
+    return hipGetLastError();
+  }
+
+  static hipError_t step(int storage, int collision, bool les, bool pull, const StepArgs& a, const void* a_in,
+                         void* a_out, int block_x, hipStream_t s) {
+    const int bx = pick_block(a.nx, block_x);
+    return with_storage(storage, [&](auto st) {
+      using St = decltype(st);
+      if (collision == kBGK) return step_dispatch<L, St, kBGK>(les, pull, a, a_in, a_out, bx, s);
+      if constexpr (L::Q == 27) {
+        return step_dispatch<L, St, kTRT>(les, pull, a, a_in, a_out, bx, s);
+      } else {
+        if (collision == kTRT) return step_dispatch<L, St, kTRT>(les, pull, a, a_in, a_out, bx, s);
+        return step_dispatch<L, St, kMRT>(les, pull, a, a_in, a_out, bx, s);
+      }
+    });
+  }
+
+  static hipError_t materialize(int storage, const StepArgs& a, const void* a_in, void* s_state, hipStream_t s) {
+    const int bx = pick_block(a.nx, 0);
+    return with_storage(storage, [&](auto st) {
+      using St = decltype(st);
+      using T = typename St::T;
+      k_materialize<L, St><<<grid_of(a, bx), bx, 0, s>>>(a, static_cast<const T*>(a_in), static_cast<T*>(s_state));
+      return hipGetLastError();
+    });
+  }
+
+  static hipError_t readback(int storage, bool from_s, const StepArgs& a, const void* buf, hipStream_t s) {
+    const int bx = pick_block(a.nx, 0);
+    return with_storage(storage, [&](auto st) {
+      using St = decltype(st);
+      using T = typename St::T;
+      if (from_s) k_readback<L, St, true><<<grid_of(a, bx), bx, 0, s>>>(a, static_cast<const T*>(buf));
+      else k_readback<L, St, false><<<grid_of(a, bx), bx, 0, s>>>(a, static_cast<const T*>(buf));
+      return hipGetLastError();
+    });
+  }
+
+  static hipError_t mem(int storage, bool from_s, const StepArgs& a, const void* buf, const int* link_node,
+                        const int* link_q, int nlinks, float cx, float cy, float cz, double* out, hipStream_t s) {
+    return with_storage(storage, [&](auto st) {
+      using St = decltype(st);
+      using T = typename St::T;
+      if (from_s)
+        k_mem<L, St, true><<<1, 256, 0, s>>>(a, static_cast<const T*>(buf), link_node, link_q, nlinks, cx, cy, cz,
+                                             out);
+      else
+        k_mem<L, St, false><<<1, 256, 0, s>>>(a, static_cast<const T*>(buf), link_node, link_q, nlinks, cx, cy, cz,
+                                              out);
+      return hipGetLastError();
+    });
+  }
+};
+
+// Per-lattice entry points, defined in step_d2q9.hip / step_d3q19.hip / step_d3q27.hip.
+#define WAKE_DECLARE_LATTICE(NAME)                                                                             \
+  hipError_t build_class_##NAME(const StepArgs& a, uint8_t* cls, hipStream_t s);                               \
+  hipError_t step_##NAME(int storage, int collision, bool les, bool pull, const StepArgs& a, const void* a_in, \
+                         void* a_out, int block_x, hipStream_t s);                                             \
+  hipError_t materialize_##NAME(int storage, const StepArgs& a, const void* a_in, void* s_state, hipStream_t s); \
+  hipError_t readback_##NAME(int storage, bool from_s, const StepArgs& a, const void* buf, hipStream_t s);      \
+  hipError_t mem_##NAME(int storage, bool from_s, const StepArgs& a, const void* buf, const int* link_node,    \
+                        const int* link_q, int nlinks, float cx, float cy, float cz, double* out, hipStream_t s);
+
+#define WAKE_DEFINE_LATTICE(NAME, LAT)                                                                          \
+  hipError_t build_class_##NAME(const StepArgs& a, uint8_t* cls, hipStream_t s) {                               \
+    return Launch<LAT>::build_class(a, cls, s);                                                                 \
+  }                                                                                                             \
+  hipError_t step_##NAME(int storage, int collision, bool les, bool pull, const StepArgs& a, const void* a_in,  \
+                         void* a_out, int block_x, hipStream_t s) {                                             \
+    return Launch<LAT>::step(storage, collision, les, pull, a, a_in, a_out, block_x, s);                        \
+  }                                                                                                             \
+  hipError_t materialize_##NAME(int storage, const StepArgs& a, const void* a_in, void* s_state, hipStream_t s) { \
+    return Launch<LAT>::materialize(storage, a, a_in, s_state, s);                                              \
+  }                                                                                                             \
+  hipError_t readback_##NAME(int storage, bool from_s, const StepArgs& a, const void* buf, hipStream_t s) {      \
+    return Launch<LAT>::readback(storage, from_s, a, buf, s);                                                   \
+  }                                                                                                             \
+  hipError_t mem_##NAME(int storage, bool from_s, const StepArgs& a, const void* buf, const int* link_node,     \
+                        const int* link_q, int nlinks, float cx, float cy, float cz, double* out, hipStream_t s) { \
+    return Launch<LAT>::mem(storage, from_s, a, buf, link_node, link_q, nlinks, cx, cy, cz, out, s);            \
+  }
+
+WAKE_DECLARE_LATTICE(d2q9)
+WAKE_DECLARE_LATTICE(d3q19)
+WAKE_DECLARE_LATTICE(d3q27)
+
+}  // namespace detail
+}  // namespace wake_hip
