@@ -1477,17 +1477,23 @@ static void bc_range(const oracle_grid* g, int* y0, int* y1, int* z0, int* z1) {
   else { *z0 = pz ? 0 : 1; *z1 = pz ? g->nz : g->nz - 1; }
 }
 
-/* EXTENSION: D3Q19 / D3Q27 Zou-He velocity inlet on the x = 0 plane.  Each
- * unknown (c_x = +1) population is the non-equilibrium bounce-back of its
- * geometric opposite plus a transverse correction shared equally by the
- * unknowns (Hecht & Harting 2010 for D3Q19):
+/* EXTENSION: D3Q19 / D3Q27 regularized Zou-He velocity inlet on the x = 0
+ * plane.  The unknown (c_x = +1) populations are first completed by
+ * non-equilibrium bounce-back of their geometric opposite with a transverse
+ * correction shared equally by the unknowns (Hecht & Harting 2010):
  *   rho  = (sum_{cx=0} f + 2 sum_{cx=-1} f) / (1 - u_x)
  *   N_t  = (sum_{cx=0} f c_t - (2/3) rho u_t) / sum_{unknown} c_t^2
  *   f_q  = f_opp(q) + 6 w_q rho (c_q . u) - (c_qy N_y + c_qz N_z)
- * which gives sum f = rho and sum f c = rho u exactly.  pinned:
- * test_oracle_zou_he (both identities after the update). */
+ * and then the whole node is regularized (Latt & Chopard 2006): with
+ * Pi = sum_q c_q c_q (f_q - feq_q),
+ *   f_q = feq_q + 4.5 w_q (c_q c_q - I/3) : Pi.
+ * Regularization keeps rho, rho u and Pi and removes the non-hydrodynamic
+ * content that the plain closure feeds into an MRT collision (which
+ * diverges at the inlet under perturbed inflow).  pinned:
+ * test_oracle_zou_he_3d_identities (sum f = rho, sum f c = rho u) and
+ * test_oracle_regularized_inlet (Pi preserved, third-order moments zero). */
 static void zh3d_velocity_node(const lat_t* L, real* f, size_t total, size_t idx, real ux, real uy, real uz) {
-  real fv[27];
+  real fv[27], fe[27];
   for (int q = 0; q < L->Q; ++q) fv[q] = f[q * total + idx];
   real s0 = R(0), sm = R(0), ty = R(0), tz = R(0);
   int cnt_y = 0, cnt_z = 0;
@@ -1509,7 +1515,26 @@ static void zh3d_velocity_node(const lat_t* L, real* f, size_t total, size_t idx
   for (int q = 0; q < L->Q; ++q) {
     if (L->cx[q] != 1) continue;
     real cu = (real)L->cx[q] * ux + (real)L->cy[q] * uy + (real)L->cz[q] * uz;
-    f[q * total + idx] = fv[L->opp[q]] + R(6) * L->w[q] * rho * cu - ((real)L->cy[q] * Ny + (real)L->cz[q] * Nz);
+    fv[q] = fv[L->opp[q]] + R(6) * L->w[q] * rho * cu - ((real)L->cy[q] * Ny + (real)L->cz[q] * Nz);
+  }
+  real pxx = R(0), pyy = R(0), pzz = R(0), pxy = R(0), pyz = R(0), pxz = R(0);
+  for (int q = 0; q < L->Q; ++q) {
+    fe[q] = feq_of(L, q, rho, ux, uy, uz);
+    real n = fv[q] - fe[q];
+    real cx = (real)L->cx[q], cy = (real)L->cy[q], cz = (real)L->cz[q];
+    pxx += cx * cx * n;
+    pyy += cy * cy * n;
+    pzz += cz * cz * n;
+    pxy += cx * cy * n;
+    pyz += cy * cz * n;
+    pxz += cx * cz * n;
+  }
+  const real third = R(1) / R(3);
+  for (int q = 0; q < L->Q; ++q) {
+    real cx = (real)L->cx[q], cy = (real)L->cy[q], cz = (real)L->cz[q];
+    real t = ((cx * cx - third) * pxx + (cy * cy - third) * pyy + (cz * cz - third) * pzz) +
+             R(2) * ((cx * cy) * pxy + (cy * cz) * pyz + (cx * cz) * pxz);
+    f[q * total + idx] = fe[q] + R(4.5) * L->w[q] * t;
   }
 }
 

@@ -445,6 +445,35 @@ def test_oracle_zou_he_3d_identities():
                 assert abs(j[0] - r * 0.05) < 1e-13 and abs(j[1]) < 1e-13 and abs(j[2]) < 1e-13
 
 
+def test_oracle_regularized_inlet():
+    # EXTENSION: the 3D inlet node is the projection feq + 4.5 w (cc - I/3):Pi;
+    # projecting again changes nothing and the third-order moments vanish.
+    g = OracleGrid(3, 8, 6, 6, tau=0.6, bits=64)
+    g.set_periodic_axes(True, True)
+    g.set_inflow_west(0.05, 0.0)
+    g.set_inflow_turbulence(0.01, 4)
+    g.add_noise(1e-3, 5)
+    g.step("MRT")
+    f = g.f_src()
+    C = D3Q19_C.astype(np.float64)
+    W = D3Q19_W
+    for z in range(6):
+        for y in range(6):
+            fi = f[:, (z * 6 + y) * 8]
+            rho = fi.sum()
+            u = C.T @ fi / rho
+            cu = C @ u
+            feq = W * rho * (1 + 3 * cu + 4.5 * cu * cu - 1.5 * (u @ u))
+            Pi = np.einsum("qa,qb,q->ab", C, C, fi - feq)
+            Qt = np.einsum("qa,qb->qab", C, C) - np.eye(3) / 3
+            reg = feq + 4.5 * W * np.einsum("qab,ab->q", Qt, Pi)
+            assert np.allclose(fi, reg, atol=1e-15)
+            m16 = (C[:, 0] * (C[:, 1] ** 2 - C[:, 2] ** 2)) @ fi
+            m17 = (C[:, 1] * (C[:, 2] ** 2 - C[:, 0] ** 2)) @ fi
+            m18 = (C[:, 2] * (C[:, 1] ** 2 - C[:, 0] ** 2)) @ fi
+            assert max(abs(m16), abs(m17), abs(m18)) < 1e-15
+
+
 def test_oracle_convective_fixed_point():
     nx, ny = 5, 4
     N = nx * ny

@@ -200,9 +200,7 @@ int wake_domain_update_macroscopic(wake_domain* d) {
 }
 int wake_domain_diagnostics(wake_domain* d, double out[3]) {
   return guard(d, [&](wake_hip::Domain& g) {
-    out[0] = g.total_mass();
-    out[1] = g.total_kinetic_energy();
-    out[2] = g.max_speed();
+    g.diagnostics(out);
   });
 }
 int wake_domain_momentum_exchange(wake_domain* d, float cx, float cy, float cz, float r_max, double out[3]) {

@@ -706,34 +706,31 @@ const float* Domain::f_dst() {
   return h_fdst_.data();
 }
 
-double Domain::total_mass() {
+void Domain::diagnostics(double out[3]) {
   ensure_fields();
+  sync_device_config();  // the reduction reads the device cell types
   check(launch_diagnostics(d_flags_, d_rho_, d_speed_, static_cast<long long>(N_), d_partial_, 1024, d_out_, stream_),
         "diagnostics");
-  double out[3];
-  check(hipMemcpyAsync(out, d_out_, sizeof(out), hipMemcpyDeviceToHost, stream_), "d2h");
+  check(hipMemcpyAsync(out, d_out_, 3 * sizeof(double), hipMemcpyDeviceToHost, stream_), "d2h");
   check(hipStreamSynchronize(stream_), "diagnostics");
-  return out[0];
+}
+
+double Domain::total_mass() {
+  double d[3];
+  diagnostics(d);
+  return d[0];
 }
 
 double Domain::total_kinetic_energy() {
-  ensure_fields();
-  check(launch_diagnostics(d_flags_, d_rho_, d_speed_, static_cast<long long>(N_), d_partial_, 1024, d_out_, stream_),
-        "diagnostics");
-  double out[3];
-  check(hipMemcpyAsync(out, d_out_, sizeof(out), hipMemcpyDeviceToHost, stream_), "d2h");
-  check(hipStreamSynchronize(stream_), "diagnostics");
-  return out[1];
+  double d[3];
+  diagnostics(d);
+  return d[1];
 }
 
 double Domain::max_speed() {
-  ensure_fields();
-  check(launch_diagnostics(d_flags_, d_rho_, d_speed_, static_cast<long long>(N_), d_partial_, 1024, d_out_, stream_),
-        "diagnostics");
-  double out[3];
-  check(hipMemcpyAsync(out, d_out_, sizeof(out), hipMemcpyDeviceToHost, stream_), "d2h");
-  check(hipStreamSynchronize(stream_), "diagnostics");
-  return out[2];
+  double d[3];
+  diagnostics(d);
+  return d[2];
 }
 
 // forces_momentum_exchange.ispc:6-138: links from Fluid nodes to in-range

@@ -91,6 +91,8 @@ class Domain {
   const float* f_dst();
   const uint8_t* cell_type() const { return flags_.data(); }
   void update_macroscopic();
+  // FP64 sums over Fluid nodes of rho and rho |u|^2 / 2, and max |u|.
+  void diagnostics(double out[3]);
   double total_mass();
   double total_kinetic_energy();
   double max_speed();

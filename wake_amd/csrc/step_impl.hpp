@@ -192,7 +192,8 @@ __device__ __forceinline__ void apply_inlet(const StepArgs& a, float (&f)[L::Q],
     f[5] = f5;
     f[8] = f8;
   } else {
-    // 3D non-equilibrium bounce-back with equal transverse corrections
+    // 3D: regularized Zou-He (non-equilibrium bounce-back completion with
+    // equal transverse corrections, then projection on feq + second order)
     float s0 = 0.0f, sm = 0.0f, ty = 0.0f, tz = 0.0f;
     int cnt_y = 0, cnt_z = 0;
     static_for<L::Q>([&](auto qc) {
@@ -214,17 +215,44 @@ __device__ __forceinline__ void apply_inlet(const StepArgs& a, float (&f)[L::Q],
     const float rho = (s0 + 2.0f * sm) / (1.0f - ux);
     const float Ny = (ty - (2.0f / 3.0f) * rho * uy) / static_cast<float>(cnt_y);
     const float Nz = (tz - (2.0f / 3.0f) * rho * uz) / static_cast<float>(cnt_z);
+    // non-equilibrium bounce-back completion of the c_x = +1 populations
     float g[L::Q];
-    static_for<L::Q>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      g[q] = f[q];
-    });
     static_for<L::Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       if constexpr (L::cx[q] == 1) {
         const float cu = cdot<L, q>(ux, uy, uz);
-        f[q] = g[L::opp[q]] + 6.0f * L::w[q] * rho * cu - (cmul<L::cy[q]>(Ny) + cmul<L::cz[q]>(Nz));
+        g[q] = f[L::opp[q]] + 6.0f * L::w[q] * rho * cu - (cmul<L::cy[q]>(Ny) + cmul<L::cz[q]>(Nz));
+      } else {
+        g[q] = f[q];
       }
+    });
+    // regularization: f = feq + 4.5 w (c c - I/3) : Pi_neq
+    const float usq = ux * ux + uy * uy + uz * uz;
+    float fe[L::Q];
+    float pxx = 0.0f, pyy = 0.0f, pzz = 0.0f, pxy = 0.0f, pyz = 0.0f, pxz = 0.0f;
+    static_for<L::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr int a = L::cx[q], b = L::cy[q], c = L::cz[q];
+      fe[q] = feq<L, q>(rho, ux, uy, uz, usq);
+      const float n = g[q] - fe[q];
+      if constexpr (a != 0) pxx += n;
+      if constexpr (b != 0) pyy += n;
+      if constexpr (c != 0) pzz += n;
+      if constexpr (a * b == 1) pxy += n;
+      if constexpr (a * b == -1) pxy -= n;
+      if constexpr (b * c == 1) pyz += n;
+      if constexpr (b * c == -1) pyz -= n;
+      if constexpr (a * c == 1) pxz += n;
+      if constexpr (a * c == -1) pxz -= n;
+    });
+    const float third = 1.0f / 3.0f;
+    static_for<L::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr int a = L::cx[q], b = L::cy[q], c = L::cz[q];
+      const float t = ((static_cast<float>(a * a) - third) * pxx + (static_cast<float>(b * b) - third) * pyy +
+                       (static_cast<float>(c * c) - third) * pzz) +
+                      2.0f * (cmul<a * b>(pxy) + cmul<b * c>(pyz) + cmul<a * c>(pxz));
+      f[q] = fe[q] + 4.5f * L::w[q] * t;
     });
   }
 }
