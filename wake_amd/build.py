@@ -25,7 +25,7 @@ DEVICE_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++20", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
-]
+] + os.environ.get("WAKE_EXTRA_DEVICE_FLAGS", "").split()
 # Host code: the flag and link builders must round exactly as written
 # (bit-exact cell types and BFL fractions), so no FP contraction.
 HOST_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-ffp-contract=off", f"--offload-arch={ARCH}",
@@ -33,7 +33,7 @@ HOST_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-ffp-contract=off", f"--offload-arc
 
 SOURCES = [
     "step_d3q19.hip", "step_d3q27.hip", "step_d2q9.hip", "step_kernels.hip",
-    "ispc_compat.hip", "batch.hip", "domain.cpp", "capi.cpp", "lattice_grid.cpp", "batch_host.cpp",
+    "ispc_compat.hip", "batch.hip", "domain.cpp", "capi.cpp", "lattice_grid.cpp", "runner.cpp", "batch_host.cpp",
 ]
 
 
@@ -82,7 +82,29 @@ def build(force=False, jobs=None, verbose=True):
             raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
         if verbose:
             print(f"[wake_amd.build] linked {LIB}", flush=True)
+    build_api_check(force=force, verbose=verbose)
     return LIB
+
+
+API_CHECK_SRC = os.path.join(ROOT, "tests", "cpp", "grid_api_check.cpp")
+API_CHECK_BIN = os.path.join(ROOT, "tests", "cpp", "grid_api_check")
+
+
+def build_api_check(force=False, verbose=True):
+    """Compile the C++ API check program against the in-tree library."""
+    if not os.path.exists(API_CHECK_SRC):
+        return None
+    if (not force and os.path.exists(API_CHECK_BIN)
+            and os.path.getmtime(API_CHECK_BIN) >= max(os.path.getmtime(API_CHECK_SRC), os.path.getmtime(LIB))):
+        return API_CHECK_BIN
+    cmd = ["hipcc", "-O2", "-std=c++20", "-I", os.path.join(ROOT, "include"), API_CHECK_SRC, "-o", API_CHECK_BIN,
+           "-L", HERE, "-lwake_hip", f"-Wl,-rpath,$ORIGIN/../../wake_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"api check build failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"[wake_amd.build] built {API_CHECK_BIN}", flush=True)
+    return API_CHECK_BIN
 
 
 if __name__ == "__main__":

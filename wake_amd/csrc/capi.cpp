@@ -4,7 +4,7 @@
 #include <exception>
 #include <string>
 
-#include "../../include/wake_hip.h"
+#include "wake_hip.h"
 #include "domain.hpp"
 #include "lattice.hpp"
 

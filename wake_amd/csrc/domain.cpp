@@ -383,6 +383,8 @@ void Domain::set_outflow_east(float rho_out) {
 }
 
 void Domain::set_convective_outflow_east(float u_conv) {
+  // the outlet blends in its upstream neighbour x = nx - 2
+  if (nx_ < 2) throw HipError("wake: convective outflow needs nx >= 2");
   before_config_change();
   outflow_ = 2;
   u_conv_ = u_conv;
@@ -588,8 +590,12 @@ void Domain::step(int collision, float gx, float gy, float gz, bool use_les, flo
     s_buf_ = -1;
   } else {
     const int out = 1 - a_buf_;
+    a.split_planes = (a.inflow || a.outflow) ? 1 : 0;
     check(launch_step(lattice_, storage_, collision, les, true, a, buf_[a_buf_], buf_[out], block_x_, stream_),
           "step");
+    if (a.split_planes)
+      check(launch_step_planes(lattice_, storage_, collision, les, a, buf_[a_buf_], buf_[out], stream_),
+            "step planes");
     a_buf_ = out;
   }
   g_last_[0] = gx;

@@ -56,6 +56,7 @@ struct StepArgs {
   float rho_out, u_conv;
   int by0, by1, bz0, bz1;     // rows an open boundary acts on
   int commit_prev;            // convective: store S_k(outlet) as next step's f_prev
+  int split_planes;           // k_step leaves inlet/outlet nodes to k_step_planes
   float* f_prev;              // convective outlet store: Q planes of ny*nz
   const uint8_t* flags;       // reference cell types
   const uint8_t* cls;         // ClassBits
@@ -77,6 +78,11 @@ hipError_t launch_mark_bfl(const int* nodes, int count, uint8_t* cls, int* aux, 
 // work-group width along x (64, 128 or 256).
 hipError_t launch_step(int lattice, int storage, int collision, bool les, bool pull, const StepArgs& a,
                        const void* a_in, void* a_out, int block_x, hipStream_t s);
+// The inlet (x = 0) and outlet (x = nx - 1) nodes of a pull step, when
+// a.split_planes is set: same gather, open boundaries and collision as
+// k_step, one lane per (y, z) of each open plane.
+hipError_t launch_step_planes(int lattice, int storage, int collision, bool les, const StepArgs& a,
+                              const void* a_in, void* a_out, hipStream_t s);
 // Materialise S_k (commit=true updates the convective outlet store) into
 // s_state (storage format, stride a.stride), or fields/planes for read-back.
 hipError_t launch_materialize(int lattice, int storage, const StepArgs& a, const void* a_in, void* s_state,

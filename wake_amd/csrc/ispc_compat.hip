@@ -13,7 +13,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/wake_hip.h"
+#include "wake_hip.h"
 #include "lattice.hpp"
 #include "node_ops.hpp"
 

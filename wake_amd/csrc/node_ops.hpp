@@ -189,9 +189,13 @@ __device__ __forceinline__ Macro collide_trt(float (&f)[L::Q], float tau_s, floa
 }
 
 // D2Q9 MRT (Lallemand-Luo), in place (d2q9_collision_mrt.ispc:6-137).
+// With LES = true the shear rate comes from the Smagorinsky tau, the stress
+// taken from the moment deviations (sum c^2 (f - feq) = dm1 / 3,
+// Pi_xx - Pi_yy = dm7, Pi_xy = dm8) as in collide_mrt_d3q19.
+template <bool LES = false>
 __device__ __forceinline__ Macro collide_mrt_d2q9(float (&f)[9], float tau_shear, float tau_bulk, float gx,
-                                                  float gy) {
-  const float s_e = 1.0f / tau_bulk, s_eps = 1.4f, s_q = 1.6f, s_p = 1.0f / tau_shear;
+                                                  float gy, float cs = 0.0f) {
+  const float s_e = 1.0f / tau_bulk, s_eps = 1.4f, s_q = 1.6f;
   const float f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3], f4 = f[4], f5 = f[5], f6 = f[6], f7 = f[7], f8 = f[8];
   float m0 = f0 + f1 + f2 + f3 + f4 + f5 + f6 + f7 + f8;
   float m1 = -4.0f * f0 - (f1 + f2 + f3 + f4) + 2.0f * (f5 + f6 + f7 + f8);
@@ -213,6 +217,19 @@ __device__ __forceinline__ Macro collide_mrt_d2q9(float (&f)[9], float tau_shear
   const float meq6 = -rho * uy;
   const float meq7 = rho * (ux2 - uy2);
   const float meq8 = rho * (ux * uy);
+  float s_p;
+  if constexpr (LES) {
+    const float dc2 = (m1 - meq1) * (1.0f / 3.0f);
+    const float dm7 = m7 - meq7;
+    const float pxx = 0.5f * (dc2 + dm7), pyy = 0.5f * (dc2 - dm7), pxy = m8 - meq8;
+    const float tensor_sum = pxx * pxx + 2.0f * pxy * pxy + pyy * pyy;
+    const float q_stress = sqrtf(2.0f * tensor_sum);
+    const float prefactor = 18.0f * 1.4142135623730951f * (cs * cs);
+    const float disc = tau_shear * tau_shear + (prefactor / rho) * q_stress;
+    s_p = 1.0f / (0.5f * (tau_shear + sqrtf(disc)));
+  } else {
+    s_p = 1.0f / tau_shear;
+  }
   const float F_pre = 1.0f - 0.5f * s_p;
   const float F_m3 = rho * gx, F_m5 = rho * gy;
   const float F_m1 = 6.0f * (ux * F_m3 + uy * F_m5);
@@ -251,9 +268,18 @@ struct D3Q19OddMoment {
 // back transform applies the rounded inverse table: the odd-moment part of
 // rows q and opp(q) differs only in sign, so each opposite pair shares one
 // even and one odd partial sum.
+//
+// With LES = true the shear rate comes from the Smagorinsky tau of this node:
+// the non-equilibrium stress is read off the moment deviations the collision
+// already has (meq = M feq exactly for these moments):
+//   sum c^2 (f - feq) = dm1 / 19,   Pi_xx = (dm9 + dm1 / 19) / 3,
+//   Pi_yy, Pi_zz = ((dm1 / 19 - Pi_xx) +- dm11) / 2,
+//   Pi_xy, Pi_yz, Pi_xz = dm13, dm14, dm15,
+// which is sum_q c c (f - feq) of d3q19_les.ispc:139-159 without forming feq.
+template <bool LES = false>
 __device__ __forceinline__ Macro collide_mrt_d3q19(float (&f)[19], float tau_shear, float tau_bulk, float gx,
-                                                   float gy, float gz) {
-  const float s_e = 1.0f / tau_bulk, s_eps = 1.4f, s_q = 1.2f, s_nu = 1.0f / tau_shear, s_pi = 1.4f, s_m = 1.98f;
+                                                   float gy, float gz, float cs = 0.0f) {
+  const float s_e = 1.0f / tau_bulk, s_eps = 1.4f, s_q = 1.2f, s_pi = 1.4f, s_m = 1.98f;
   float m[19];
   m[0] = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8] + f[9] + f[10] + f[11] + f[12] + f[13] +
          f[14] + f[15] + f[16] + f[17] + f[18];
@@ -291,7 +317,27 @@ __device__ __forceinline__ Macro collide_mrt_d3q19(float (&f)[19], float tau_she
   const float ux2 = ux * ux, uy2 = uy * uy, uz2 = uz * uz, u2 = ux2 + uy2 + uz2;
   const float meq9 = rho * (2.0f * ux2 - uy2 - uz2);
   const float meq11 = rho * (uy2 - uz2);
-  m[1] = m[1] - s_e * (m[1] - rho * (-11.0f + 19.0f * u2));
+  const float meq1 = rho * (-11.0f + 19.0f * u2);
+  float s_nu;
+  if constexpr (LES) {
+    const float dm1 = m[1] - meq1;
+    const float dc2 = dm1 * (1.0f / 19.0f);
+    const float pxx = (m[9] - meq9 + dc2) * (1.0f / 3.0f);
+    const float dm11 = m[11] - meq11;
+    const float pyy = 0.5f * ((dc2 - pxx) + dm11);
+    const float pzz = 0.5f * ((dc2 - pxx) - dm11);
+    const float pxy = m[13] - rho * (ux * uy);
+    const float pyz = m[14] - rho * (uy * uz);
+    const float pxz = m[15] - rho * (ux * uz);
+    const float tensor_sum = (pxx * pxx + pyy * pyy + pzz * pzz) + 2.0f * (pxy * pxy + pyz * pyz + pxz * pxz);
+    const float q_stress = sqrtf(2.0f * tensor_sum);
+    const float prefactor = 18.0f * 1.4142135623730951f * (cs * cs);
+    const float disc = tau_shear * tau_shear + (prefactor / rho) * q_stress;
+    s_nu = 1.0f / (0.5f * (tau_shear + sqrtf(disc)));
+  } else {
+    s_nu = 1.0f / tau_shear;
+  }
+  m[1] = m[1] - s_e * (m[1] - meq1);
   m[2] = m[2] - s_eps * (m[2] - rho * (3.0f - 11.0f / 2.0f * u2));
   m[3] = m[3] + rho * gx;
   m[4] = m[4] - s_q * (m[4] - (-2.0f / 3.0f * rho * ux)) - (2.0f / 3.0f) * rho * gx;
@@ -351,6 +397,22 @@ __device__ __forceinline__ Macro collide(float (&f)[L::Q], float tau_eff, float 
     return collide_mrt_d2q9(f, tau_eff, kTauBulk2D, gx, gy);
   } else {
     return collide_mrt_d3q19(f, tau_eff, kTauBulk3D, gx, gy, gz);
+  }
+}
+
+// Collision with LES-coupled shear tau (tau0 is the base tau).  MRT reads the
+// stress from its moments; BGK and TRT form it from feq.
+template <class L, int COLL>
+__device__ __forceinline__ Macro collide_les(float (&f)[L::Q], float tau0, float cs, float gx, float gy, float gz) {
+  if constexpr (COLL == kMRT && L::Q == 19) {
+    return collide_mrt_d3q19<true>(f, tau0, kTauBulk3D, gx, gy, gz, cs);
+  } else if constexpr (COLL == kMRT && L::Q == 9) {
+    return collide_mrt_d2q9<true>(f, tau0, kTauBulk2D, gx, gy, cs);
+  } else {
+    float rho, jx, jy, jz, ux, uy, uz;
+    density_momentum<L>(f, rho, jx, jy, jz);
+    guo_velocity(rho, jx, jy, jz, gx, gy, gz, ux, uy, uz);
+    return collide<L, COLL>(f, smagorinsky_tau<L>(f, rho, ux, uy, uz, tau0, cs), gx, gy, gz);
   }
 }
 

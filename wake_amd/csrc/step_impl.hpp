@@ -27,11 +27,30 @@
 namespace wake_hip {
 
 // ----------------------------------------------------------------- storage
+// Code-generation knobs of the step kernel (same work, different schedule):
+// WAKE_NT_LOADS marks the population loads non-temporal; WAKE_WAVES_PER_EU
+// asks the compiler for that many waves per SIMD (register budget).
+#ifndef WAKE_NT_LOADS
+#define WAKE_NT_LOADS 0
+#endif
+#ifndef WAKE_WAVES_PER_EU
+#define WAKE_WAVES_PER_EU 0
+#endif
+
+template <class T>
+__device__ __forceinline__ T ld_pop(const T* __restrict__ p, long long i) {
+#if WAKE_NT_LOADS
+  return __builtin_nontemporal_load(p + i);
+#else
+  return p[i];
+#endif
+}
+
 struct StF32 {
   using T = float;
   template <class L, int q>
   __device__ __forceinline__ static float ld(const T* __restrict__ p, long long i) {
-    return p[i];
+    return ld_pop(p, i);
   }
   template <class L, int q>
   __device__ __forceinline__ static void st(T* __restrict__ p, long long i, float v) {
@@ -44,7 +63,7 @@ struct StF16 {
   using T = _Float16;
   template <class L, int q>
   __device__ __forceinline__ static float ld(const T* __restrict__ p, long long i) {
-    return L::w[q] + static_cast<float>(p[i]);
+    return L::w[q] + static_cast<float>(ld_pop(p, i));
   }
   template <class L, int q>
   __device__ __forceinline__ static void st(T* __restrict__ p, long long i, float v) {
@@ -57,7 +76,7 @@ struct StBF16 {
   using T = uint16_t;
   template <class L, int q>
   __device__ __forceinline__ static float ld(const T* __restrict__ p, long long i) {
-    return L::w[q] + __uint_as_float(static_cast<uint32_t>(p[i]) << 16);
+    return L::w[q] + __uint_as_float(static_cast<uint32_t>(ld_pop(p, i)) << 16);
   }
   template <class L, int q>
   __device__ __forceinline__ static void st(T* __restrict__ p, long long i, float v) {
@@ -91,45 +110,50 @@ __device__ __forceinline__ void gather(const StepArgs& a, const typename St::T* 
   const long long S = a.stride;
   const int xm = x == 0 ? nx - 1 : x - 1;
   const int xp = x == nx - 1 ? 0 : x + 1;
-  if (!(c & (kClsFrozen | kClsSlow | kClsBfl))) {
-    static_for<L::Q>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      constexpr int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
-      const int sx = cx == 0 ? x : (cx > 0 ? xm : xp);
-      const int sy = cy == 0 ? y : wrap(y - cy, ny);
-      const int sz = cz == 0 ? z : wrap(z - cz, nz);
-      f[q] = St::template ld<L, q>(A, q * S + static_cast<long long>((sz * ny + sy) * nx + sx));
-    });
-    return;
-  }
+  // Bulk pull, issued for every node together with the class byte (wrapped
+  // addresses are always in bounds); special nodes patch it below, so these
+  // loads never wait on the class byte.
+  static_for<L::Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
+    const int sx = cx == 0 ? x : (cx > 0 ? xm : xp);
+    const int sy = cy == 0 ? y : wrap(y - cy, ny);
+    const int sz = cz == 0 ? z : wrap(z - cz, nz);
+    f[q] = St::template ld<L, q>(A, q * S + static_cast<long long>((sz * ny + sy) * nx + sx));
+  });
+  if (!(c & (kClsFrozen | kClsSlow | kClsBfl))) return;
   if (c & kClsFrozen) {
     static_for<L::Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       f[q] = St::template ld<L, q>(A, q * S + n);
     });
-  } else {
+  } else if (c & kClsSlow) {
     static_for<L::Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       constexpr int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q], o = L::opp[q];
-      const int sx = cx == 0 ? x : (cx > 0 ? xm : xp);
-      int sy = y - cy, sz = z - cz;
-      bool in = true;
-      if constexpr (cy != 0) {
-        if (a.per_y) sy = wrap(sy, ny);
-        else in = in && sy >= 0 && sy < ny;
-      }
-      if constexpr (cz != 0) {
-        if (a.per_z) sz = wrap(sz, nz);
-        else in = in && sz >= 0 && sz < nz;
-      }
-      if (!in) {
-        f[q] = St::template ld<L, o>(A, o * S + n);  // halfway bounce-back at a domain wall
-      } else {
-        const int sn = (sz * ny + sy) * nx + sx;
-        const uint8_t t = a.periodic_mode ? static_cast<uint8_t>(kFluid) : a.flags[sn];
-        if (t == kSolid) f[q] = St::template ld<L, o>(A, o * S + n);
-        else if (t == kMovingWall) f[q] = St::template ld<L, o>(A, o * S + n) + a.mom[q];
-        else f[q] = St::template ld<L, q>(A, q * S + sn);
+      if constexpr (q > 0) {
+        const int sx = cx == 0 ? x : (cx > 0 ? xm : xp);
+        int sy = y - cy, sz = z - cz;
+        bool in = true;
+        if constexpr (cy != 0) {
+          if (a.per_y) sy = wrap(sy, ny);
+          else in = in && sy >= 0 && sy < ny;
+        }
+        if constexpr (cz != 0) {
+          if (a.per_z) sz = wrap(sz, nz);
+          else in = in && sz >= 0 && sz < nz;
+        }
+        // Halfway bounce-back (domain wall, solid or moving-wall source)
+        // replaces the pulled value with the node's own opposite population.
+        const int sn = in ? (sz * ny + sy) * nx + sx : n;
+        const uint8_t t = (in && !a.periodic_mode) ? a.flags[sn] : static_cast<uint8_t>(kFluid);
+        const bool bounce = !in || t == kSolid || t == kMovingWall;
+        float v = f[q];
+        if (bounce) {
+          v = St::template ld<L, o>(A, o * S + n);
+          if (t == kMovingWall) v += a.mom[q];
+        }
+        f[q] = v;
       }
     });
   }
@@ -139,20 +163,21 @@ __device__ __forceinline__ void gather(const StepArgs& a, const typename St::T* 
     static_for<L::Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       if constexpr (q > 0) {
+        constexpr int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q], o = L::opp[q];
+        float val = f[o];
         if (mask & (1u << q)) {
-          constexpr int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q], o = L::opp[q];
           const float delta = r->delta[q];
           const float ciuw = r->ciuw[q];
           const float f_i_xf = St::template ld<L, q>(A, q * S + n);
-          float val;
           if (delta < 0.5f) {
             const int xf = cx == 0 ? x : (cx > 0 ? xm : xp);
             int yf = y - cy, zf = z - cz;
             if (a.bfl_per_y) yf = wrap(yf, ny);
             if (a.bfl_per_z) zf = wrap(zf, nz);
-            float f_i_xff = f_i_xf;
-            if (yf >= 0 && yf < ny && zf >= 0 && zf < nz)
-              f_i_xff = St::template ld<L, q>(A, q * S + static_cast<long long>((zf * ny + yf) * nx + xf));
+            const bool inside = yf >= 0 && yf < ny && zf >= 0 && zf < nz;
+            const float f_i_xff = inside ? St::template ld<L, q>(A, q * S + static_cast<long long>(
+                                                                        ((inside ? zf : z) * ny + (inside ? yf : y)) * nx + xf))
+                                         : f_i_xf;
             const float mom = 2.0f * L::w[q] * 1.0f * 3.0f * ciuw;
             val = 2.0f * delta * f_i_xf + (1.0f - 2.0f * delta) * f_i_xff - mom;
           } else {
@@ -161,8 +186,8 @@ __device__ __forceinline__ void gather(const StepArgs& a, const typename St::T* 
             const float mom = (1.0f / delta) * L::w[q] * 1.0f * 3.0f * ciuw;
             val = inv_2delta * f_i_xf + (1.0f - inv_2delta) * f_opp_xf - mom;
           }
-          f[o] = val;
         }
+        f[o] = val;
       }
     });
   }
@@ -337,37 +362,91 @@ __global__ void k_build_class(const StepArgs a, uint8_t* __restrict__ cls) {
   cls[n] = c;
 }
 
+#if WAKE_WAVES_PER_EU > 0
+#define WAKE_STEP_ATTR __attribute__((amdgpu_waves_per_eu(WAKE_WAVES_PER_EU, WAKE_WAVES_PER_EU)))
+#else
+#define WAKE_STEP_ATTR
+#endif
+
+// Nodes per thread along z (WAKE_ZK): the pull of node z+1 is issued before
+// node z collides, so each lane keeps two nodes' loads in flight.
+#ifndef WAKE_ZK
+#define WAKE_ZK 1
+#endif
+
 template <class L, class St, int COLL, bool LES, bool PULL>
-__global__ __launch_bounds__(256) void k_step(const StepArgs a, const typename St::T* __restrict__ ain,
-                                              typename St::T* __restrict__ aout) {
+__global__ __launch_bounds__(256) WAKE_STEP_ATTR void k_step(const StepArgs a, const typename St::T* __restrict__ ain,
+                                                             typename St::T* __restrict__ aout) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y, z = blockIdx.z;
+  const int y = blockIdx.y, z0 = blockIdx.z * WAKE_ZK;
   if (x >= a.nx) return;
-  const int n = (z * a.ny + y) * a.nx + x;
-  const uint8_t c = a.cls[n];
   const long long S = a.stride;
-  float f[L::Q];
-  if constexpr (PULL) {
-    form_s<L, St>(a, ain, x, y, z, n, c, true, f);
-  } else {
+  auto fetch = [&](int z, float (&f)[L::Q], uint8_t& c, int& n) {
+    n = (z * a.ny + y) * a.nx + x;
+    c = a.cls[n];
+    if constexpr (PULL) {
+      // with split planes the open-boundary nodes are only gathered here
+      // (and skipped below); k_step_planes forms and commits them
+      const uint8_t cb = a.split_planes ? static_cast<uint8_t>(c & ~(kClsInlet | kClsOutlet)) : c;
+      form_s<L, St>(a, ain, x, y, z, n, cb, true, f);
+    } else {
+      static_for<L::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        f[q] = St::template ld<L, q>(ain, q * S + n);
+      });
+    }
+  };
+  float fn[L::Q];
+  uint8_t cn;
+  int nn;
+  fetch(z0, fn, cn, nn);
+#pragma unroll
+  for (int k = 0; k < WAKE_ZK; ++k) {
+    const int z = z0 + k;
+    if (k > 0 && z >= a.nz) break;
+    float f[L::Q];
     static_for<L::Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
-      f[q] = St::template ld<L, q>(ain, q * S + n);
+      f[q] = fn[q];
+    });
+    const uint8_t c = cn;
+    const int n = nn;
+    if (k + 1 < WAKE_ZK && z + 1 < a.nz) fetch(z + 1, fn, cn, nn);
+    // open-boundary nodes belong to k_step_planes, so no bulk wave diverges
+    if (PULL && a.split_planes && (c & (kClsInlet | kClsOutlet))) continue;
+    if (!(c & kClsSolid)) {
+      if constexpr (LES) collide_les<L, COLL>(f, a.tau, a.cs, a.gx, a.gy, a.gz);
+      else collide<L, COLL>(f, a.tau, a.gx, a.gy, a.gz);
+    }
+    static_for<L::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      St::template st<L, q>(aout, q * S + n, f[q]);
     });
   }
+}
+
+// Inlet and outlet planes of a split pull step: blockIdx.y = 0 for the
+// x = 0 plane, 1 for x = nx - 1; lanes run over (y, z) of the plane.
+template <class L, class St, int COLL, bool LES>
+__global__ __launch_bounds__(256) void k_step_planes(const StepArgs a, const typename St::T* __restrict__ ain,
+                                                     typename St::T* __restrict__ aout) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.ny * a.nz) return;
+  const int y = i % a.ny, z = i / a.ny;
+  if (blockIdx.y == 1 && a.nx == 1) return;  // one plane is both: formed once
+  const int x = blockIdx.y == 0 ? 0 : a.nx - 1;
+  const int n = (z * a.ny + y) * a.nx + x;
+  const uint8_t c = a.cls[n];
+  if (!(c & (kClsInlet | kClsOutlet))) return;
+  float f[L::Q];
+  form_s<L, St>(a, ain, x, y, z, n, c, true, f);
   if (!(c & kClsSolid)) {
-    float tau = a.tau;
-    if constexpr (LES) {
-      float rho, jx, jy, jz, ux, uy, uz;
-      density_momentum<L>(f, rho, jx, jy, jz);
-      guo_velocity(rho, jx, jy, jz, a.gx, a.gy, a.gz, ux, uy, uz);
-      tau = smagorinsky_tau<L>(f, rho, ux, uy, uz, a.tau, a.cs);
-    }
-    collide<L, COLL>(f, tau, a.gx, a.gy, a.gz);
+    if constexpr (LES) collide_les<L, COLL>(f, a.tau, a.cs, a.gx, a.gy, a.gz);
+    else collide<L, COLL>(f, a.tau, a.gx, a.gy, a.gz);
   }
   static_for<L::Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    St::template st<L, q>(aout, q * S + n, f[q]);
+    St::template st<L, q>(aout, q * a.stride + n, f[q]);
   });
 }
 
