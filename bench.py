@@ -7,6 +7,11 @@ convective outflow, periodic y/z.  One step = one fused pull-stream-collide
 launch over the whole domain; MLUPS counts every node, as the reference does
 (sphere_3d_wake.cpp:81-92).
 
+`--workload batched-wakes` runs the batched multi-domain path instead: 512
+D2Q9 MRT cylinder-wake domains of uneven size, one launch per step over all
+of them, and compares simulations per second with the plain
+one-domain-at-a-time path.
+
 Prints ONE JSON line.  The roofline entry divides the algorithmic bytes of
 a step (one read and one write of every population: 152 B/node for D3Q19
 FP32) by the average duration of the step kernel, measured with HIP events
@@ -68,10 +73,135 @@ def cpu_baseline(n, steps, R=24.0):
     return n ** 3 * steps / dt / 1e6, L.oracle_threads(), dt
 
 
+def batched_wakes(wa, ndom, seed=4, umax=0.05, storage=0):
+    """512 (ndom) independent D2Q9 MRT channel flows past a cylinder: NX =
+    128 + 32 U{0..28}, NY = NX/4, D = NY/8 at (NX/5, NY/2 + U{-2..2}), solid
+    walls at y = 0 and NY - 1, parabolic Zou-He inlet with peak umax, Zou-He
+    pressure outlet, Re log-uniform in [50, 300] on the mean inlet velocity
+    U = 2/3 umax (tau = 3 U D / Re + 1/2), flow started from the inlet
+    profile.  At D = 4 and Re = 300 the grid Reynolds number U / nu is 75:
+    plain MRT diverges there, so the collision carries the coupled
+    Smagorinsky closure (Cs = 0.17).  Returns the grids and their
+    (cx, cy, D, U, Re) geometry."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    nxs = 128 + 32 * rng.integers(0, 29, ndom)
+    offs = rng.integers(-2, 3, ndom)
+    res = np.exp(rng.uniform(math.log(50.0), math.log(300.0), ndom))
+    U = 2.0 / 3.0 * umax
+    grids, geo = [], []
+    for nx, off, re in zip(nxs, offs, res):
+        nx = int(nx)
+        ny = nx // 4
+        D = ny / 8.0
+        g = wa.LatticeGrid2D(nx, ny, 3.0 * U * D / float(re) + 0.5, storage=storage)
+        H = ny - 2
+        s = np.arange(ny, dtype=np.float64) - 0.5
+        prof = np.clip(4.0 * umax * s * (H - s) / (H * H), 0.0, None)
+        g.initialize_fields(np.ones(nx * ny, np.float32), np.repeat(prof, nx).astype(np.float32),
+                            np.zeros(nx * ny, np.float32))
+        g.set_les_coupled(True)
+        g.set_inflow_parabolic(umax)
+        g.set_outflow_east(1.0)
+        g.set_solid_box(0, 0, nx - 1, 0)
+        g.set_solid_box(0, ny - 1, nx - 1, ny - 1)
+        cx, cy = nx / 5.0, ny / 2.0 + float(off)
+        g.set_solid_circle(cx, cy, D / 2)
+        grids.append(g)
+        geo.append((cx, cy, D, U, float(re)))
+    return grids, geo
+
+
+def main_batched(args):
+    """Batched cylinder wakes: every domain stepped `--steps` times by one
+    launch per step (graph-captured chunks of 100, autotuned launch shape),
+    force history per step, C_D and Strouhal per domain; the plain
+    one-domain-at-a-time path is timed per domain on a short run and scaled
+    to the same step count."""
+    import numpy as np
+
+    import wake_amd
+    from wake_amd import analysis
+
+    storage = {"fp32": 0, "fp16": 1, "bf16": 2}[args.storage]
+    ndom, S = args.domains, args.steps
+    log(f"setup {ndom} D2Q9 cylinder-wake domains ({args.storage})")
+    t0 = time.perf_counter()
+    grids, geo = batched_wakes(wake_amd, ndom, storage=storage)
+    nodes = sum(g.total_nodes() for g in grids)
+    log(f"setup done in {time.perf_counter() - t0:.1f}s, {nodes / 1e6:.1f} M nodes")
+
+    # plain path: each domain alone, timed per domain, scaled to S steps
+    t_plain = 0.0
+    probe = args.plain_probe
+    for g in grids:
+        g.run(2, "MRT", use_les=True, smag=0.17)
+        ms = g.run(probe, "MRT", use_les=True, smag=0.17, timed=True)
+        t_plain += ms / probe * S * 1e-3
+    log(f"plain path: {t_plain:.2f} s projected for {S} steps of every domain ({probe}-step probe each)")
+
+    b = wake_amd.Batch(lattice=9, storage=storage, collision="MRT", use_les=True, chunk=100)
+    slots = [b.add(g, S - 2 - probe, smag=0.17, forces=(cx, cy, 0.0, 1.5 * D))
+             for g, (cx, cy, D, U, re) in zip(grids, geo)]
+    t0 = time.perf_counter()
+    shape = b.tune(probe_steps=4)
+    tune_s = time.perf_counter() - t0
+    left = max(b.remaining(s) for s in slots)
+    log(f"autotuned launch shape block={shape[0]} grid_factor={shape[1]} in {tune_s:.2f}s; {left} steps left")
+    t0 = time.perf_counter()
+    ms = b.run(left)
+    wall = time.perf_counter() - t0
+    mlups = nodes * left / wall / 1e6
+    log(f"batched: {left} steps in {wall:.2f} s wall ({ms / 1e3:.2f} s device), {mlups:.0f} MLUPS")
+    # the same S steps per domain as the plain projection
+    t_batch = wall * S / left
+    cds, sts, bad = [], [], 0
+    for s, (cx, cy, D, U, re) in zip(slots, geo):
+        F = b.forces(s)
+        tail = F[len(F) // 2:]
+        q = 0.5 * U * U * D
+        if not np.all(np.isfinite(tail)):
+            bad += 1
+            continue
+        cds.append(float(np.mean(tail[:, 0])) / q)
+        f = analysis.analyze_frequency((F[-4096:, 1] / q).astype(np.float32), 1.0)
+        sts.append(analysis.strouhal_number(f, D, U))
+    log(f"C_D median {np.median(cds):.3f} [{np.min(cds):.3f}, {np.max(cds):.3f}], "
+        f"St median {np.median(sts):.3f}, non-finite domains {bad}")
+    out = {
+        "metric": METRIC,
+        "value": round(ndom / t_batch, 4),
+        "unit": "sims/s",
+        "n_gpus": 1,
+        "steps": S,
+        "warmup": 2 + probe,
+        "ms_per_step": round(wall * 1e3 / left, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if storage == 0 else ("fp16-storage/fp32" if storage == 1 else "bf16-storage/fp32"),
+        "data": "synthetic",
+        "config": {
+            "workload": f"{ndom} D2Q9 MRT+Smagorinsky(Cs=0.17) cylinder wakes (NX 128-1024 step 32, NY=NX/4, D=NY/8, parabolic "
+                        f"Zou-He inlet umax=0.05, Re 50-300 log-uniform, seed 4), one launch per step over all "
+                        f"domains, graph chunks of 100 steps",
+            "nodes": nodes, "launch_shape": {"block": shape[0], "grid_factor": shape[1]},
+            "batched_mlups": round(mlups, 1), "plain_sims_per_s": round(ndom / t_plain, 4),
+            "speedup_vs_plain": round(t_plain / t_batch, 3),
+            "C_D_median": round(float(np.median(cds)), 4) if cds else None,
+            "St_median": round(float(np.median(sts)), 4) if sts else None,
+            "non_finite_domains": bad, "parallelism": "single GPU",
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--workload", choices=["sphere", "batched-wakes"], default="sphere")
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--storage", choices=["fp32", "fp16", "bf16"], default="fp32")
@@ -79,12 +209,22 @@ def main():
     ap.add_argument("--block-x", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=4, help="oracle steps timed for the CPU baseline (0: skip)")
     ap.add_argument("--cpu-n", type=int, default=256)
+    ap.add_argument("--domains", type=int, default=512)
+    ap.add_argument("--plain-probe", type=int, default=20)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batched-wakes":
+        if args.steps is None:
+            args.steps = 20000
+        main_batched(args)
+        return
+    if args.steps is None:
+        args.steps = 2000
 
     import wake_amd
 
@@ -105,9 +245,10 @@ def main():
     ms = g.run(args.steps, "MRT", use_les=True, smag=0.17, timed=True)
     g.synchronize()
     wall = time.perf_counter() - t0
-    log(f"timed {args.steps} steps: events {ms:.1f} ms, wall {wall * 1e3:.1f} ms")
+    kernel_ms = g.kernel_time()
+    log(f"timed {args.steps} steps: events {ms:.1f} ms, wall {wall * 1e3:.1f} ms, "
+        f"step kernel {kernel_ms * 1e3:.1f} us/launch")
     mlups = N * args.steps / wall / 1e6
-    kernel_ms = ms / args.steps
     achieved = bpn * N / (kernel_ms * 1e-3) / 1e9
     # momentum exchange on the final state (C_D, C_L), FP64 on the device
     R = 24.0 * n / 256
@@ -155,7 +296,9 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
             "bytes_per_node": bpn,
+            "kernel": "k_step (fused pull + MRT/LES collide), HIP events per launch",
             "kernel_ms": round(kernel_ms, 5),
+            "step_ms_events": round(ms / args.steps, 5),
         },
         "cpu_baseline": cpu,
     }

@@ -151,7 +151,8 @@ enum wake_field {
   WAKE_FIELD_UZ = 3,
   WAKE_FIELD_SPEED = 4,
   WAKE_FIELD_F_SRC = 5, /* Q planes of nx*ny*nz: the post-stream populations */
-  WAKE_FIELD_F_DST = 6  /* Q planes: the post-collision populations of the last step */
+  WAKE_FIELD_F_DST = 6, /* Q planes: the post-collision populations of the last step */
+  WAKE_FIELD_TEMPERATURE = 7 /* thermal domains: T of the post-stream temperature populations */
 };
 
 /* LatticeGrid3D(nx, ny, nz, tau) (lattice_grid_3d.cpp:46-64) for dim = 3,
@@ -182,6 +183,12 @@ int wake_domain_set_periodic_axes(wake_domain* d, int per_y, int per_z);
 /* lattice_grid_3d.hpp:53-56, lattice_grid_2d.hpp:47-51 */
 int wake_domain_set_solid_box(wake_domain* d, int x0, int y0, int z0, int x1, int y1, int z1);
 int wake_domain_set_solid_sphere(wake_domain* d, float cx, float cy, float cz, float r);
+/* src/geometry/primitives.cpp:73-100 (GeometryPrimitives::voxelize_ellipsoid)
+ * when axes is NULL.  Otherwise axes is a row-major 3x3 whose rows are the
+ * ellipsoid's unit body axes in world coordinates (a rotated ellipsoid, no
+ * counterpart in the reference). */
+int wake_domain_set_solid_ellipsoid(wake_domain* d, float cx, float cy, float cz, float rx, float ry, float rz,
+                                    const float* axes);
 int wake_domain_set_solid_circle(wake_domain* d, float cx, float cy, float r);
 int wake_domain_set_solid_cylinder_z(wake_domain* d, float cx, float cy, float r);
 int wake_domain_set_moving_wall_top(wake_domain* d, float u_wall_x, float u_wall_z);
@@ -208,6 +215,10 @@ int wake_domain_step(wake_domain* d, int collision, float gx, float gy, float gz
 int wake_domain_run(wake_domain* d, int steps, int collision, float gx, float gy, float gz, int use_les,
                     float smag, float* ms);
 int wake_domain_synchronize(wake_domain* d);
+/* Average device duration (ms) of the fused step kernel per launch over the
+ * last wake_domain_run with ms != NULL (HIP events around each launch on the
+ * domain's stream; the open-boundary plane kernel is not included). */
+int wake_domain_kernel_time(wake_domain* d, float* avg_ms);
 /* lattice_grid_3d.hpp:72-78 rho_data ... f_src_data, copied into out */
 int wake_domain_read(wake_domain* d, int field, float* out);
 int wake_domain_read_cell_types(wake_domain* d, uint8_t* out);
@@ -222,8 +233,63 @@ int wake_domain_diagnostics(wake_domain* d, double out[3]);
 int wake_domain_momentum_exchange(wake_domain* d, float cx, float cy, float cz, float r_max, double out[3]);
 /* out: nx, ny, nz, Q, plane stride, current step, storage, device bytes */
 int wake_domain_info(wake_domain* d, long long out[8]);
+/* D2Q5 temperature coupled to the D2Q9 flow by a Boussinesq acceleration
+ * a_y = gbeta (T - t_ref) per node (2D only; no counterpart in the reference,
+ * whose src/thermal/thermal_solver.cpp:83-123 couples only the mean
+ * temperature).  Temperature starts as rest equilibrium at t_init. */
+int wake_domain_enable_thermal(wake_domain* d, float tau_g, float gbeta, float t_ref, float t_init);
+/* Temperature field (nx * ny, reference node order) as rest equilibrium. */
+int wake_domain_set_temperature_field(wake_domain* d, const float* t);
+/* Anti-bounce-back wall temperature for the solid nodes in the box
+ * (inclusive); NaN makes them adiabatic (bounce-back) again. */
+int wake_domain_set_wall_temperature(wake_domain* d, int x0, int y0, int x1, int y1, float t_wall);
 /* work-group width along x for the step kernel (64, 128, 256; 0 = auto) */
 int wake_domain_set_block_x(wake_domain* d, int block_x);
+
+/* ----------------------------------------------------------------------
+ * Part 3: batched stepping of many domains (no counterpart in the
+ * reference, which steps one grid per call of LatticeGrid*::step,
+ * src/grid/lattice_grid_2d.cpp:195-265 and lattice_grid_3d.cpp:301-380).
+ *
+ * A batch steps every member domain with one kernel launch per step.  The
+ * members share lattice, population storage, collision and LES coupling and
+ * keep everything else of their own (size, cell types, boundaries, tau,
+ * body force).  Each member takes exactly the steps it was added with; a
+ * member's results are those of stepping it alone.  Steps run in chunks of
+ * `chunk` launches captured once as a HIP graph.  A domain must not be
+ * stepped, destroyed or added to another batch while it is a member; after
+ * wake_batch_run returns its fields and populations read as after
+ * wake_domain_step.
+ * -------------------------------------------------------------------- */
+typedef struct wake_batch wake_batch;
+
+/* thermal: members are D2Q9 + D2Q5 temperature domains (lattice 9). */
+int wake_batch_create(int lattice, int storage, int collision, int use_les, int thermal, int chunk,
+                      wake_batch** out);
+void wake_batch_destroy(wake_batch* b);
+/* Add a member for `steps` more steps with body force (gx, gy, gz) and
+ * Smagorinsky constant smag.  track_forces: record the momentum-exchange
+ * force on S after every step over the links around (cx, cy, cz) within
+ * r_max (3 doubles per step: F (3D) or Fx, Fy, torque (2D)).  *slot gets
+ * the member id. */
+int wake_batch_add(wake_batch* b, wake_domain* d, long long steps, float gx, float gy, float gz, float smag,
+                   int track_forces, float cx, float cy, float cz, float r_max, int* slot);
+int wake_batch_remove(wake_batch* b, int slot);
+/* `steps` launches (each member takes min(steps, its remaining steps));
+ * *ms = device time.  use_graph = 0 launches each step directly. */
+int wake_batch_run(wake_batch* b, long long steps, int use_graph, float* ms);
+int wake_batch_remaining(wake_batch* b, int slot, long long* remaining);
+/* Force history since the member was added: *count steps, up to `capacity`
+ * of them copied to out (3 doubles each). */
+int wake_batch_forces(wake_batch* b, int slot, double* out, long long capacity, long long* count);
+/* Autotuner: times probe_steps real steps under each candidate launch shape
+ * (work-group size; one work-group per tile or a persistent grid of
+ * grid_factor x compute units) and keeps the fastest.  The steps count as
+ * steps of the members. */
+int wake_batch_tune(wake_batch* b, int probe_steps, int* block, int* grid_factor);
+int wake_batch_set_launch(wake_batch* b, int block, int grid_factor);
+/* out = {members, total nodes, block, grid_factor} */
+int wake_batch_info(wake_batch* b, long long out[4]);
 
 #ifdef __cplusplus
 }

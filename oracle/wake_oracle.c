@@ -1176,6 +1176,10 @@ struct oracle_grid {
   int nlinks;
   int *l_idx, *l_q;
   real *l_delta, *l_uw;
+  /* EXTENSION: D2Q5 temperature with Boussinesq coupling */
+  int thermal;
+  real tau_g, gbeta, t_ref;
+  real *g_src, *g_dst, *t_wall, *temp;
 };
 
 static void* xcalloc(size_t n, size_t s) {
@@ -1213,6 +1217,7 @@ void oracle_grid_destroy(oracle_grid* g) {
   free(g->f_src); free(g->f_dst); free(g->f_prev); free(g->rho); free(g->ux); free(g->uy); free(g->uz);
   free(g->speed); free(g->tau_eff); free(g->cell);
   free(g->l_idx); free(g->l_q); free(g->l_delta); free(g->l_uw);
+  free(g->g_src); free(g->g_dst); free(g->t_wall); free(g->temp);
   free(g);
 }
 
@@ -1333,6 +1338,38 @@ void oracle_grid_set_solid_sphere(oracle_grid* g, real cx, real cy, real cz, rea
       for (int x = min_x; x <= max_x; ++x) {
         real dx = (real)x - cx, dy = (real)y - cy, dz = (real)z - cz;
         if (dx * dx + dy * dy + dz * dz <= r_sq) oracle_grid_set_cell_type(g, x, y, z, CELL_SOLID);
+      }
+}
+
+/* primitives.cpp:73-100 (axis-aligned: the reference's arithmetic); with
+ * axes != NULL the rows of the row-major 3x3 `axes` are the body axes in
+ * world coordinates, the point's body coordinates are b_k = axes[k] . d and
+ * the box is bounded by the largest semi-axis (no reference counterpart). */
+void oracle_grid_set_solid_ellipsoid(oracle_grid* g, real cx, real cy, real cz, real rx, real ry, real rz,
+                                     const real* axes) {
+  real ex = rx, ey = ry, ez = rz;
+  if (axes) {
+    real m = rx > ry ? rx : ry;
+    m = m > rz ? m : rz;
+    ex = ey = ez = m;
+  }
+  int min_x = imax(0, (int)(cx - ex - R(1))), max_x = imin(g->nx - 1, (int)(cx + ex + R(1)));
+  int min_y = imax(0, (int)(cy - ey - R(1))), max_y = imin(g->ny - 1, (int)(cy + ey + R(1)));
+  int min_z = imax(0, (int)(cz - ez - R(1))), max_z = imin(g->nz - 1, (int)(cz + ez + R(1)));
+  real irx = R(1) / (rx * rx), iry = R(1) / (ry * ry), irz = R(1) / (rz * rz);
+  for (int z = min_z; z <= max_z; ++z)
+    for (int y = min_y; y <= max_y; ++y)
+      for (int x = min_x; x <= max_x; ++x) {
+        real dx = (real)x - cx, dy = (real)y - cy, dz = (real)z - cz;
+        if (axes) {
+          real b0 = axes[0] * dx + axes[1] * dy + axes[2] * dz;
+          real b1 = axes[3] * dx + axes[4] * dy + axes[5] * dz;
+          real b2 = axes[6] * dx + axes[7] * dy + axes[8] * dz;
+          dx = b0;
+          dy = b1;
+          dz = b2;
+        }
+        if (dx * dx * irx + dy * dy * iry + dz * dz * irz <= R(1)) oracle_grid_set_cell_type(g, x, y, z, CELL_SOLID);
       }
 }
 
@@ -1599,7 +1636,165 @@ static void apply_open_boundaries(oracle_grid* g, uint64_t step) {
 }
 
 /* lattice_grid_2d.cpp:195-265 and lattice_grid_3d.cpp:301-380. */
+/* ------------------------------------------------------------------------
+ * EXTENSION (no reference counterpart; the reference's ThermalSolver2D,
+ * src/thermal/thermal_solver.cpp:83-123, is a D2Q9 passive scalar whose
+ * buoyancy reaches the flow only through the mean temperature).  Parity
+ * pinned by test_thermal_* (conduction fixed point, adiabatic energy
+ * conservation, de Vahl Davis cavity Nusselt number).
+ *
+ * D2Q5 temperature g_q (c = 0, +x, +y, -x, -y; w = 1 - 4/6, 1/6 x 4), BGK with
+ * geq_q = w_q T (1 + 3 c_q . u), alpha = (tau_g - 1/2) / 3, coupled to the
+ * D2Q9 flow by a per-node Boussinesq acceleration a_y = gbeta (T - t_ref)
+ * added to the body force of the flow collision; u in geq is the Guo
+ * velocity (j + rho a / 2) / rho of the post-stream state.  A pull source
+ * that is solid (or outside a non-periodic y range) reflects g: adiabatic
+ * bounce-back g_q = g*_opp, or, where the solid node carries a wall
+ * temperature T_w, anti-bounce-back g_q = -g*_opp + 2 w_q T_w. */
+static const int c5x[5] = {0, 1, 0, -1, 0};
+static const int c5y[5] = {0, 0, 1, 0, -1};
+static const int opp5[5] = {0, 3, 4, 1, 2};
+/* w_0 = 1 - 4 w_1 in the working precision, so the weights sum to one and
+ * the equilibrium carries T without a per-step drift */
+static const real w5[5] = {R(1) - R(4) * (R(1) / R(6)), R(1) / R(6), R(1) / R(6), R(1) / R(6), R(1) / R(6)};
+
+void oracle_grid_enable_thermal(oracle_grid* g, real tau_g, real gbeta, real t_ref, real t_init) {
+  if (g->dim != 2) return;
+  if (!g->thermal) {
+    g->g_src = (real*)xcalloc(5 * g->N, sizeof(real));
+    g->g_dst = (real*)xcalloc(5 * g->N, sizeof(real));
+    g->t_wall = (real*)xcalloc(g->N, sizeof(real));
+    g->temp = (real*)xcalloc(g->N, sizeof(real));
+    for (size_t i = 0; i < g->N; ++i) g->t_wall[i] = (real)NAN;
+  }
+  g->thermal = 1;
+  g->tau_g = tau_g;
+  g->gbeta = gbeta;
+  g->t_ref = t_ref;
+  for (size_t i = 0; i < g->N; ++i) {
+    g->temp[i] = t_init;
+    for (int q = 0; q < 5; ++q) g->g_src[q * g->N + i] = g->g_dst[q * g->N + i] = w5[q] * t_init;
+  }
+}
+
+/* Temperature field as equilibrium populations at rest (every node). */
+void oracle_grid_set_temperature_field(oracle_grid* g, const real* t) {
+  if (!g->thermal) return;
+  for (size_t i = 0; i < g->N; ++i) {
+    g->temp[i] = t[i];
+    for (int q = 0; q < 5; ++q) g->g_src[q * g->N + i] = g->g_dst[q * g->N + i] = w5[q] * t[i];
+  }
+}
+
+void oracle_grid_set_wall_temperature(oracle_grid* g, int x0, int y0, int x1, int y1, real t_wall) {
+  if (!g->thermal) return;
+  for (int y = imax(0, y0); y <= imin(g->ny - 1, y1); ++y)
+    for (int x = imax(0, x0); x <= imin(g->nx - 1, x1); ++x) g->t_wall[(size_t)y * g->nx + x] = t_wall;
+}
+
+const real* oracle_grid_temperature(const oracle_grid* g) { return g->temp; }
+
+static void thermal_step_2d(oracle_grid* g, int collision, real gx, real gy, uint64_t step) {
+  const size_t n = g->N;
+  const int nx = g->nx, ny = g->ny, count = (int)n;
+  const int per_y = g->ext_axes && g->per_y;
+  const int periodic = g->is_periodic && !g->has_moving_wall_top;
+  const real omega_g = R(1) / g->tau_g;
+  /* 1. collision of f (per-node Boussinesq force) and g */
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < count; ++i) {
+    if (g->cell[i] == CELL_SOLID) {
+      for (int q = 0; q < 9; ++q) g->f_dst[q * n + i] = g->f_src[q * n + i];
+      for (int q = 0; q < 5; ++q) g->g_dst[q * n + i] = g->g_src[q * n + i];
+      continue;
+    }
+    real T = R(0);
+    for (int q = 0; q < 5; ++q) T += g->g_src[q * n + i];
+    const real gyn = gy + g->gbeta * (T - g->t_ref);
+    real rho = R(0), jx = R(0), jy = R(0);
+    for (int q = 0; q < 9; ++q) {
+      const real v = g->f_src[q * n + i];
+      rho += v;
+      jx += (real)c9x[q] * v;
+      jy += (real)c9y[q] * v;
+    }
+    const real inv_rho = R(1) / rho;
+    const real ux = (jx + R(0.5) * rho * gx) * inv_rho;
+    const real uy = (jy + R(0.5) * rho * gyn) * inv_rho;
+    if (collision == COLL_BGK)
+      d2q9_bgk_node(g->f_src, g->f_dst, n, (size_t)i, NULL, NULL, NULL, g->tau, gx, gyn);
+    else if (collision == COLL_TRT)
+      d2q9_trt_node(g->f_src, g->f_dst, n, (size_t)i, NULL, NULL, NULL, g->tau, trt_tau_a(g->tau), gx, gyn);
+    else
+      d2q9_mrt_node(g->f_src, g->f_dst, n, (size_t)i, NULL, NULL, NULL, g->tau, R(1.63), gx, gyn);
+    for (int q = 0; q < 5; ++q) {
+      const real cu = (real)c5x[q] * ux + (real)c5y[q] * uy;
+      const real geq = w5[q] * T * (R(1) + R(3) * cu);
+      const real v = g->g_src[q * n + i];
+      g->g_dst[q * n + i] = v - omega_g * (v - geq);
+    }
+  }
+  /* 2. streaming of f, open boundaries */
+  if (periodic)
+    oracle_d2q9_stream_periodic(g->f_dst, g->f_src, nx, ny);
+  else
+    oracle_d2q9_stream_general(g->f_dst, g->f_src, g->cell, nx, ny, per_y, g->has_moving_wall_top ? g->uwx : R(0),
+                               R(0));
+  apply_open_boundaries(g, step);
+  /* 3. streaming of g with (anti-)bounce-back */
+#pragma omp parallel for schedule(static)
+  for (int y = 0; y < ny; ++y)
+    for (int x = 0; x < nx; ++x) {
+      const size_t i = (size_t)y * nx + x;
+      if (!periodic && g->cell[i] == CELL_SOLID) {
+        for (int q = 0; q < 5; ++q) g->g_src[q * n + i] = g->g_dst[q * n + i];
+        continue;
+      }
+      for (int q = 0; q < 5; ++q) {
+        const int sx = (x - c5x[q] + nx) % nx;
+        int sy = y - c5y[q];
+        int in = 1;
+        if (periodic || per_y) sy = (sy + ny) % ny;
+        else in = sy >= 0 && sy < ny;
+        const size_t s = in ? (size_t)sy * nx + sx : i;
+        const uint8_t t = (in && !periodic) ? g->cell[s] : CELL_FLUID;
+        if (!in || t == CELL_SOLID || t == CELL_MOVING_WALL) {
+          const real tw = in ? g->t_wall[s] : (real)NAN;
+          const real v = g->g_dst[opp5[q] * n + i];
+          g->g_src[q * n + i] = isnan(tw) ? v : -v + R(2) * w5[q] * tw;
+        } else {
+          g->g_src[q * n + i] = g->g_dst[q * n + s];
+        }
+      }
+    }
+  /* 4. temperature and moments with the per-node force */
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < count; ++i) {
+    real T = R(0);
+    for (int q = 0; q < 5; ++q) T += g->g_src[q * n + i];
+    g->temp[i] = T;
+    const real gyn = gy + g->gbeta * (T - g->t_ref);
+    real rho = R(0), jx = R(0), jy = R(0);
+    for (int q = 0; q < 9; ++q) {
+      const real v = g->f_src[q * n + i];
+      rho += v;
+      jx += (real)c9x[q] * v;
+      jy += (real)c9y[q] * v;
+    }
+    const real inv_rho = R(1) / rho;
+    g->rho[i] = rho;
+    g->ux[i] = (jx + R(0.5) * rho * gx) * inv_rho;
+    g->uy[i] = (jy + R(0.5) * rho * gyn) * inv_rho;
+    g->speed[i] = RSQRT(g->ux[i] * g->ux[i] + g->uy[i] * g->uy[i]);
+  }
+  g->step_count++;
+}
+
 void oracle_grid_step(oracle_grid* g, int collision, real gx, real gy, real gz, int use_les, real smag) {
+  if (g->thermal && g->dim == 2) {
+    thermal_step_2d(g, collision, gx, gy, g->step_count + 1);
+    return;
+  }
   int count = (int)g->N;
   real et = g->tau;
   uint64_t step = g->step_count + 1;

@@ -132,6 +132,13 @@ void oracle_grid_set_periodic(oracle_grid* g, int periodic);
 void oracle_grid_set_periodic_axes(oracle_grid* g, int per_y, int per_z);
 void oracle_grid_set_solid_box(oracle_grid* g, int x0, int y0, int z0, int x1, int y1, int z1);
 void oracle_grid_set_solid_sphere(oracle_grid* g, real cx, real cy, real cz, real r);
+/* EXTENSION: D2Q5 temperature coupled to the D2Q9 flow (Boussinesq). */
+void oracle_grid_enable_thermal(oracle_grid* g, real tau_g, real gbeta, real t_ref, real t_init);
+void oracle_grid_set_temperature_field(oracle_grid* g, const real* t);
+void oracle_grid_set_wall_temperature(oracle_grid* g, int x0, int y0, int x1, int y1, real t_wall);
+const real* oracle_grid_temperature(const oracle_grid* g);
+void oracle_grid_set_solid_ellipsoid(oracle_grid* g, real cx, real cy, real cz, real rx, real ry, real rz,
+                                     const real* axes);
 void oracle_grid_set_solid_circle(oracle_grid* g, real cx, real cy, real r);
 void oracle_grid_set_solid_cylinder_z(oracle_grid* g, real cx, real cy, real r);
 void oracle_grid_set_moving_wall_top(oracle_grid* g, real uwx, real uwz);

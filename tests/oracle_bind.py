@@ -41,6 +41,12 @@ def lib(bits=32):
     L.oracle_grid_set_periodic_axes.argtypes = [P, I, I]
     L.oracle_grid_set_solid_box.argtypes = [P, I, I, I, I, I, I]
     L.oracle_grid_set_solid_sphere.argtypes = [P, real, real, real, real]
+    L.oracle_grid_set_solid_ellipsoid.argtypes = [P, real, real, real, real, real, real, P]
+    L.oracle_grid_enable_thermal.argtypes = [P, real, real, real, real]
+    L.oracle_grid_set_temperature_field.argtypes = [P, P]
+    L.oracle_grid_set_wall_temperature.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, real]
+    L.oracle_grid_temperature.argtypes = [P]
+    L.oracle_grid_temperature.restype = P
     L.oracle_grid_set_solid_circle.argtypes = [P, real, real, real]
     L.oracle_grid_set_solid_cylinder_z.argtypes = [P, real, real, real]
     L.oracle_grid_set_moving_wall_top.argtypes = [P, real, real]
@@ -177,6 +183,28 @@ class OracleGrid:
 
     def set_solid_sphere(self, cx, cy, cz, r):
         self.L.oracle_grid_set_solid_sphere(self.h, cx, cy, cz, r)
+
+    def enable_thermal(self, tau_g, gbeta, t_ref, t_init):
+        self.L.oracle_grid_enable_thermal(self.h, tau_g, gbeta, t_ref, t_init)
+        self._thermal = True
+
+    def set_temperature_field(self, t):
+        a = np.ascontiguousarray(t, np.float64 if self.bits == 64 else np.float32)
+        self.L.oracle_grid_set_temperature_field(self.h, a.ctypes.data_as(ctypes.c_void_p))
+
+    def set_wall_temperature(self, x0, y0, x1, y1, t):
+        self.L.oracle_grid_set_wall_temperature(self.h, x0, y0, x1, y1, t)
+
+    def temperature(self):
+        return self._arr("temperature", self.N)
+
+    def set_solid_ellipsoid(self, cx, cy, cz, rx, ry, rz, axes=None):
+        a = None
+        if axes is not None:
+            a = np.ascontiguousarray(np.asarray(axes).reshape(9), np.float64 if self.bits == 64 else np.float32)
+            self._keep = a
+        self.L.oracle_grid_set_solid_ellipsoid(self.h, cx, cy, cz, rx, ry, rz,
+                                               None if a is None else a.ctypes.data_as(ctypes.c_void_p))
 
     def set_solid_circle(self, cx, cy, r):
         self.L.oracle_grid_set_solid_circle(self.h, cx, cy, r)

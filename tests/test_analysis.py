@@ -1,0 +1,106 @@
+"""CPU tests of the host-side force analysis (wake_amd.analysis) against the
+reference's known answers (tests/unit/test_forces_momentum_exchange.cpp:70-100)
+and an independent restatement of its radix-2 transform
+(src/forces/strouhal_analyzer.cpp:8-37)."""
+import cmath
+import math
+
+import numpy as np
+import pytest
+
+from wake_amd import analysis as an
+
+
+def _radix2_fft_c64(x):
+    """strouhal_analyzer.cpp:8-37 in complex64: bit reversal, then butterflies
+    whose twiddle is advanced by repeated multiplication."""
+    a = [np.complex64(v) for v in x]
+    n = len(a)
+    j = 0
+    for i in range(1, n):
+        bit = n >> 1
+        while j & bit:
+            j ^= bit
+            bit >>= 1
+        j ^= bit
+        if i < j:
+            a[i], a[j] = a[j], a[i]
+    length = 2
+    while length <= n:
+        ang = np.float32(-2 * math.pi) / np.float32(length)
+        wlen = np.complex64(complex(math.cos(ang), math.sin(ang)))
+        for i in range(0, n, length):
+            w = np.complex64(1)
+            for k in range(length // 2):
+                u = a[i + k]
+                v = np.complex64(a[i + k + length // 2] * w)
+                a[i + k] = np.complex64(u + v)
+                a[i + k + length // 2] = np.complex64(u - v)
+                w = np.complex64(w * wlen)
+        length <<= 1
+    return a
+
+
+def _reference_peak_bin(sig):
+    sig = np.asarray(sig, np.float32)
+    mean = np.float32(sig.sum(dtype=np.float32) / np.float32(sig.size))
+    n = 1
+    while n < sig.size:
+        n <<= 1
+    n = min(n, 4096)
+    m = min(n, sig.size)
+    x = np.zeros(n, np.float32)
+    for i in range(m):
+        w = np.float32(0.5) * (np.float32(1) - np.float32(math.cos(np.float32(2 * math.pi) * np.float32(i) / np.float32(m - 1))))
+        x[i] = (sig[i] - mean) * w
+    X = _radix2_fft_c64(x)
+    best, pk = np.float32(0), 1
+    for k in range(1, n // 2):
+        p = np.float32(X[k].real * X[k].real + X[k].imag * X[k].imag)
+        if p > best:
+            best, pk = p, k
+    return pk, n
+
+
+def test_reference_known_answer_sinusoid():
+    f_true, dt = 0.025, 1.0
+    t = np.arange(512, dtype=np.float32)
+    sig = (np.float32(0.35) * np.sin(np.float32(2 * np.pi * f_true) * t)
+           + np.float32(0.02) * np.cos(np.float32(2 * np.pi * 0.1) * t)).astype(np.float32)
+    f = an.analyze_frequency(sig, dt)
+    assert abs(f - f_true) <= 0.003  # the reference's bound
+    assert f == pytest.approx(13 / 512, rel=1e-6)  # the bin nearest 0.025 * 512 = 12.8
+    st = an.strouhal_number(f, 20.0, 0.1)
+    assert abs(st - 5.0) <= 0.6
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_peak_bin_matches_radix2_restatement(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(40, 200))
+    t = np.arange(n)
+    sig = (np.sin(2 * np.pi * rng.uniform(0.02, 0.3) * t) + 0.3 * rng.standard_normal(n)).astype(np.float32)
+    k, nfft = _reference_peak_bin(sig)
+    assert an.analyze_frequency(sig, 0.5) == pytest.approx(k / (nfft * 0.5), rel=1e-6)
+
+
+def test_only_first_4096_samples_are_analysed():
+    t = np.arange(6000, dtype=np.float64)
+    sig = np.where(t < 4096, np.sin(2 * np.pi * 0.05 * t), np.sin(2 * np.pi * 0.2 * t) * 10).astype(np.float32)
+    assert an.analyze_frequency(sig, 1.0) == pytest.approx(round(0.05 * 4096) / 4096, rel=1e-6)
+
+
+def test_short_or_bad_input_gives_zero():
+    assert an.analyze_frequency(np.ones(15, np.float32), 1.0) == 0.0
+    assert an.analyze_frequency(np.ones(64, np.float32), 0.0) == 0.0
+    assert an.strouhal_number(0.1, 10.0, 0.0) == 0.0
+
+
+def test_coefficients_closed_form():
+    c = an.aerodynamic_coefficients(0.02, -0.01, 0.0, 0.5, 1.0, 0.05, 8.0, 2.0)
+    q = 0.5 * 0.05 * 0.05
+    assert float(c["cd"]) == pytest.approx(0.02 / (q * 8.0), rel=1e-6)
+    assert float(c["cl"]) == pytest.approx(-0.01 / (q * 8.0), rel=1e-6)
+    assert float(c["cm"]) == pytest.approx(0.5 / (q * 8.0 * 2.0), rel=1e-6)
+    z = an.aerodynamic_coefficients(1.0, 1.0, 0.0, 0.0, 1.0, 1e-5, 1.0)
+    assert float(z["cd"]) == 0.0 and float(z["cl"]) == 0.0

@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 import oracle_bind as ob
+from parity import check_fp32_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -94,7 +95,7 @@ def taylor_green_2d(n, u0):
 
 
 # --------------------------------------------------------------------- 2D
-def test_config1_taylor_green_256_bgk_1000_steps(wa):
+def test_taylor_green_d2q9_256_bgk_1000_steps(wa):
     """D2Q9 BGK Taylor-Green, 256^2 fully periodic, tau 0.8, u0 0.04, feq +
     U(-1e-6, 1e-6) seed 1, 1000 steps: parity with the oracle and the
     analytic kinetic-energy decay exp(-4 nu k^2 t)."""
@@ -116,9 +117,25 @@ def test_config1_taylor_green_256_bgk_1000_steps(wa):
     assert abs(p.g.compute_total_mass() / p.o.total_mass() - 1) < 1e-7
 
 
+def _karman_oracle(o, nx, ny, D, u, init):
+    o.initialize_fields(*init)
+    o.set_inflow_west(u, 0.0)
+    o.set_outflow_east(1.0)
+    for x in range(nx):
+        o.set_cell_type(x, 0, 0, 1)
+        o.set_cell_type(x, ny - 1, 0, 1)
+    o.set_solid_circle(nx * 0.2, ny * 0.5, D / 2)
+
+
 def test_karman_trt_inflow_outflow_cylinder(wa):
     """karman_vortex.cpp:58-142 at reduced size: Zou-He inlet/outlet, walls,
-    circular cylinder, TRT, momentum exchange per step."""
+    circular cylinder, TRT, momentum exchange per step.
+
+    This flow amplifies rounding: Zou-He's rho_in = (...)/(1 - u_x) feeds the
+    inlet error back every step, and the FP32 oracle itself ends 1.4e-5 to
+    4.7e-5 (u, rel L2) away from the FP64 oracle over 200-1000 steps, above
+    the 1e-5 bar, so the velocity is held to the FP32-floor criterion of
+    parity.check_fp32_parity; density is held to the bar."""
     nx, ny, D, u = 160, 64, 10.0, 0.05
     nu = u * D / 100.0
     tau = 3 * nu + 0.5
@@ -129,21 +146,28 @@ def test_karman_trt_inflow_outflow_cylinder(wa):
     ux = np.full(nx * ny, u, np.float32)
     uy = (1e-3 * rng.standard_normal(nx * ny)).astype(np.float32)
     p.g.initialize_fields(r, ux, uy)
-    p.o.initialize_fields(r, ux, uy)
-    p.both("set_inflow_west", u, 0.0)
-    p.both("set_outflow_east", 1.0)
+    p.g.set_inflow_west(u, 0.0)
+    p.g.set_outflow_east(1.0)
     for x in range(nx):
-        p.cell(x, 0, 0, 1)
-        p.cell(x, ny - 1, 0, 1)
-    p.both("set_solid_circle", cx, cy, D / 2)
+        p.g.set_cell_type(x, 0, 1)
+        p.g.set_cell_type(x, ny - 1, 1)
+    p.g.set_solid_circle(cx, cy, D / 2)
+    _karman_oracle(p.o, nx, ny, D, u, (r, ux, uy))
+    o64 = ob.OracleGrid(2, nx, ny, 1, tau=tau, lattice=9, bits=64)
+    _karman_oracle(o64, nx, ny, D, u, (r, ux, uy))
     assert np.array_equal(p.g.cell_type_data(), p.o.cell_type())
     for _ in range(6):
         p.step(100, "TRT")
+        o64.run(100, "TRT")
         fg = p.g.momentum_exchange_2d(cx, cy, 1.5 * D)
         fo = p.o.momentum_exchange(cx, cy, 0.0, 1.5 * D)
         scale = abs(fo[0])
         assert abs(fg[0] - fo[0]) <= FORCE_TOL * scale and abs(fg[1] - fo[1]) <= FORCE_TOL * scale
-    p.check_fields()
+    fl = p.o.cell_type() == 0
+    assert rel_l2(p.g.rho_data()[fl], p.o.rho()[fl]) <= FIELD_TOL
+    vel = lambda a, b: np.concatenate([a[fl], b[fl]])
+    check_fp32_parity(vel(p.g.ux_data(), p.g.uy_data()), vel(p.o.ux(), p.o.uy()), vel(o64.ux(), o64.uy()),
+                      "karman u")
 
 
 def test_lid_driven_cavity_mrt_with_gravity(wa):
@@ -277,7 +301,7 @@ def test_config_change_mid_run_materializes_previous_state(wa):
     assert rel_l2(p.g.f_src_data(), p.o.f_src()) <= FIELD_TOL
 
 
-def test_config2_like_les_inlet_outlet_periodic_yz(wa):
+def test_sphere_les_inlet_outlet_periodic_yz_64x48x48(wa):
     """The sphere workload's boundary set at reduced size: MRT + coupled
     Smagorinsky LES (Cs 0.17, tau0 0.505), Zou-He velocity inlet with seeded
     1% turbulence, convective outflow, periodic y/z, BFL sphere."""

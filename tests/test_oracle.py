@@ -607,3 +607,27 @@ def test_oracle_fp32_fp64_agree_on_sphere_flow():
         out.append(np.stack([g.rho(), g.ux(), g.uy(), g.uz()]).astype(np.float64))
     err = np.linalg.norm(out[0] - out[1]) / np.linalg.norm(out[1])
     assert err < 1e-5
+
+
+def test_ellipsoid_voxelizer_axes_conventions(oracle32):
+    """primitives.cpp:73-100 restated; the rotated form (an extension) must
+    reduce to it: identity axes change nothing, swapping two body axes equals
+    swapping the semi-axes, and the voxel count follows 4/3 pi rx ry rz."""
+    def flags(rx, ry, rz, axes=None):
+        o = ob.OracleGrid(3, 48, 40, 36, bits=32)
+        o.set_solid_ellipsoid(24.3, 20.1, 17.7, rx, ry, rz, axes)
+        return o.cell_type().reshape(36, 40, 48)
+
+    base = flags(11.0, 7.0, 5.0)
+    assert np.array_equal(flags(11.0, 7.0, 5.0, np.eye(3)), base)
+    swap = np.array([[0, 1, 0], [1, 0, 0], [0, 0, 1]], float)
+    assert np.array_equal(flags(7.0, 11.0, 5.0, swap), base)
+    vol = (base == 1).sum()
+    assert abs(vol / (4.0 / 3.0 * np.pi * 11 * 7 * 5) - 1) < 0.03
+    # a 45 degree turn about z keeps the z extent and widens the y extent
+    c, s = np.cos(np.pi / 4), np.sin(np.pi / 4)
+    rot = flags(11.0, 7.0, 5.0, np.array([[c, s, 0], [-s, c, 0], [0, 0, 1]]))
+    assert abs((rot == 1).sum() / vol - 1) < 0.03
+    ys = lambda f: np.nonzero((f == 1).any(axis=(0, 2)))[0]
+    zs = lambda f: np.nonzero((f == 1).any(axis=(1, 2)))[0]
+    assert np.array_equal(zs(rot), zs(base)) and len(ys(rot)) > len(ys(base))

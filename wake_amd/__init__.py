@@ -6,6 +6,7 @@ kernel exports, `wake_amd.kernels`) and its grid API (`LatticeGrid2D`,
 the package fails if the library has not been built.
 """
 from . import _lib
+from .batch import Batch
 from .grid import (CS2, DEFAULT_SMAGORINSKY_CONSTANT, RHO_0, CellType, CollisionType, LatticeGrid2D,
                    LatticeGrid3D, Storage)
 
@@ -14,6 +15,6 @@ _lib.lib()  # fail loudly at import when the native library is missing
 kernels = _lib
 
 __all__ = [
-    "CellType", "CollisionType", "LatticeGrid2D", "LatticeGrid3D", "Storage", "RHO_0", "CS2",
+    "Batch", "CellType", "CollisionType", "LatticeGrid2D", "LatticeGrid3D", "Storage", "RHO_0", "CS2",
     "DEFAULT_SMAGORINSKY_CONSTANT", "kernels",
 ]

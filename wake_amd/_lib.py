@@ -67,6 +67,7 @@ DOMAIN_SIGS = {
     "wake_domain_set_periodic_axes": ([P, I, I], I),
     "wake_domain_set_solid_box": ([P, I, I, I, I, I, I], I),
     "wake_domain_set_solid_sphere": ([P, F, F, F, F], I),
+    "wake_domain_set_solid_ellipsoid": ([P, F, F, F, F, F, F, P], I),
     "wake_domain_set_solid_circle": ([P, F, F, F], I),
     "wake_domain_set_solid_cylinder_z": ([P, F, F, F], I),
     "wake_domain_set_moving_wall_top": ([P, F, F], I),
@@ -77,11 +78,15 @@ DOMAIN_SIGS = {
     "wake_domain_set_convective_outflow_east": ([P, F], I),
     "wake_domain_set_les_coupled": ([P, I], I),
     "wake_domain_set_relaxation_time": ([P, F], I),
+    "wake_domain_enable_thermal": ([P, F, F, F, F], I),
+    "wake_domain_set_temperature_field": ([P, P], I),
+    "wake_domain_set_wall_temperature": ([P, I, I, I, I, F], I),
     "wake_domain_build_curved_sphere_links": ([P, F, F, F, F, P], I),
     "wake_domain_get_links": ([P, P, P, P], I),
     "wake_domain_step": ([P, I, F, F, F, I, F], I),
     "wake_domain_run": ([P, I, I, F, F, F, I, F, P], I),
     "wake_domain_synchronize": ([P], I),
+    "wake_domain_kernel_time": ([P, P], I),
     "wake_domain_read": ([P, I, P], I),
     "wake_domain_read_cell_types": ([P, P], I),
     "wake_domain_update_macroscopic": ([P], I),
@@ -93,6 +98,20 @@ DOMAIN_SIGS = {
     "wake_hip_clear_error": ([], I),
     "wake_hip_device_count": ([], I),
     "wake_hip_d3q19_minv": ([P], None),
+}
+
+# Part 3: batches of domains
+BATCH_SIGS = {
+    "wake_batch_create": ([I, I, I, I, I, I, P], I),
+    "wake_batch_destroy": ([P], None),
+    "wake_batch_add": ([P, P, LL, F, F, F, F, I, F, F, F, F, P], I),
+    "wake_batch_remove": ([P, I], I),
+    "wake_batch_run": ([P, LL, I, P], I),
+    "wake_batch_remaining": ([P, I, P], I),
+    "wake_batch_forces": ([P, I, P, LL, P], I),
+    "wake_batch_tune": ([P, I, P, P], I),
+    "wake_batch_set_launch": ([P, I, I], I),
+    "wake_batch_info": ([P, P], I),
 }
 
 EXTRA_SIGS = {}
@@ -111,7 +130,7 @@ def lib():
         fn = getattr(L, name)
         fn.argtypes = [_CT[c] for c in sig]
         fn.restype = None
-    for name, (args, res) in {**DOMAIN_SIGS, **EXTRA_SIGS}.items():
+    for name, (args, res) in {**DOMAIN_SIGS, **BATCH_SIGS, **EXTRA_SIGS}.items():
         if not hasattr(L, name):
             continue
         fn = getattr(L, name)

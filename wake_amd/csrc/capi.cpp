@@ -5,12 +5,8 @@
 #include <string>
 
 #include "wake_hip.h"
-#include "domain.hpp"
+#include "capi_internal.hpp"
 #include "lattice.hpp"
-
-struct wake_domain {
-  wake_hip::Domain* d;
-};
 
 namespace {
 thread_local std::string g_error;
@@ -35,6 +31,12 @@ int guard(wake_domain* d, F&& f) {
 }  // namespace
 
 namespace wake_hip {
+int set_last_error(const std::string& msg) {
+  g_error = msg;
+  g_code = 1;
+  return 1;
+}
+
 // Shared with the kernel exports (ispc_compat.hip).
 void record_error(const std::string& msg) {
   g_error = msg;
@@ -116,6 +118,10 @@ int wake_domain_set_solid_box(wake_domain* d, int x0, int y0, int z0, int x1, in
 int wake_domain_set_solid_sphere(wake_domain* d, float cx, float cy, float cz, float r) {
   return guard(d, [&](wake_hip::Domain& g) { g.set_solid_sphere(cx, cy, cz, r); });
 }
+int wake_domain_set_solid_ellipsoid(wake_domain* d, float cx, float cy, float cz, float rx, float ry, float rz,
+                                    const float* axes) {
+  return guard(d, [&](wake_hip::Domain& g) { g.set_solid_ellipsoid(cx, cy, cz, rx, ry, rz, axes); });
+}
 int wake_domain_set_solid_circle(wake_domain* d, float cx, float cy, float r) {
   return guard(d, [&](wake_hip::Domain& g) { g.set_solid_circle(cx, cy, r); });
 }
@@ -142,6 +148,15 @@ int wake_domain_set_convective_outflow_east(wake_domain* d, float u_conv) {
 }
 int wake_domain_set_les_coupled(wake_domain* d, int coupled) {
   return guard(d, [&](wake_hip::Domain& g) { g.set_les_coupled(coupled != 0); });
+}
+int wake_domain_enable_thermal(wake_domain* d, float tau_g, float gbeta, float t_ref, float t_init) {
+  return guard(d, [&](wake_hip::Domain& g) { g.enable_thermal(tau_g, gbeta, t_ref, t_init); });
+}
+int wake_domain_set_temperature_field(wake_domain* d, const float* t) {
+  return guard(d, [&](wake_hip::Domain& g) { g.set_temperature_field(t); });
+}
+int wake_domain_set_wall_temperature(wake_domain* d, int x0, int y0, int x1, int y1, float t_wall) {
+  return guard(d, [&](wake_hip::Domain& g) { g.set_wall_temperature(x0, y0, x1, y1, t_wall); });
 }
 int wake_domain_set_relaxation_time(wake_domain* d, float tau) {
   return guard(d, [&](wake_hip::Domain& g) { g.set_tau(tau); });
@@ -171,6 +186,9 @@ int wake_domain_run(wake_domain* d, int steps, int collision, float gx, float gy
     if (ms) *ms = t;
   });
 }
+int wake_domain_kernel_time(wake_domain* d, float* avg_ms) {
+  return guard(d, [&](wake_hip::Domain& g) { *avg_ms = g.last_kernel_ms(); });
+}
 int wake_domain_synchronize(wake_domain* d) {
   return guard(d, [&](wake_hip::Domain& g) { g.synchronize(); });
 }
@@ -187,6 +205,7 @@ int wake_domain_read(wake_domain* d, int field, float* out) {
       case WAKE_FIELD_SPEED: src = g.speed(); break;
       case WAKE_FIELD_F_SRC: src = g.f_src(); n = N * g.lattice(); break;
       case WAKE_FIELD_F_DST: src = g.f_dst(); n = N * g.lattice(); break;
+      case WAKE_FIELD_TEMPERATURE: src = g.temperature(); break;
       default: throw wake_hip::HipError("unknown field");
     }
     std::memcpy(out, src, n * sizeof(float));

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 
 #include "lattice.hpp"
 
@@ -98,6 +99,7 @@ Domain::Domain(int dim, int lattice, int nx, int ny, int nz, float tau, int stor
 
 Domain::~Domain() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  for (hipEvent_t e : kev_) (void)hipEventDestroy(e);
   for (void* p : allocations_) (void)hipFree(p);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -110,12 +112,19 @@ void* Domain::alloc(size_t bytes) {
   return p;
 }
 
+void Domain::release(void* p) {
+  if (!p) return;
+  (void)hipFree(p);
+  allocations_.erase(std::remove(allocations_.begin(), allocations_.end(), p), allocations_.end());
+}
+
 const void* Domain::state_buffer() const { return buf_[s_buf_ >= 0 ? s_buf_ : a_buf_]; }
 
 // ------------------------------------------------------------ initialisation
 // lattice_grid_3d.cpp:66-93: feq of uniform fields in both buffers, macro
 // fields set, every cell type reset to Fluid.
 void Domain::initialize(float rho, float ux, float uy, float uz) {
+  ++version_;
   const LatTab L = tab(lattice_);
   if (dim_ == 2) uz = 0.0f;
   step_count_ = 0;
@@ -164,6 +173,7 @@ void Domain::initialize(float rho, float ux, float uy, float uz) {
 }
 
 void Domain::upload_state_from_host(const std::vector<float>& f_ref) {
+  ++version_;
   // f_ref: Q planes of N (reference layout) -> storage format, padded stride.
   const LatTab L = tab(lattice_);
   std::vector<char> host(static_cast<size_t>(Q_) * stride_ * elem_, 0);
@@ -251,6 +261,7 @@ void Domain::add_noise(float amplitude, uint64_t seed) {
 
 // ------------------------------------------------------------- cell types
 void Domain::before_config_change() {
+  ++version_;
   if (s_buf_ < 0 && a_buf_ >= 0) materialize();
 }
 
@@ -290,6 +301,34 @@ void Domain::set_solid_box(int x0, int y0, int z0, int x1, int y1, int z1) {
   for (int z = std::max(0, z0); z <= std::min(nz_ - 1, z1); ++z)
     for (int y = std::max(0, y0); y <= std::min(ny_ - 1, y1); ++y)
       for (int x = std::max(0, x0); x <= std::min(nx_ - 1, x1); ++x) set_cell_type(x, y, z, kSolid);
+}
+
+// primitives.cpp:73-100: FP32 ellipsoid test, bit-exact flags.  With axes
+// (rows = body axes in world coordinates) the point is first projected on
+// the body axes and the box is bounded by the largest semi-axis.
+void Domain::set_solid_ellipsoid(float cx, float cy, float cz, float rx, float ry, float rz, const float* axes) {
+  before_config_change();
+  is_periodic_ = false;
+  float ex = rx, ey = ry, ez = rz;
+  if (axes) ex = ey = ez = std::max(rx, std::max(ry, rz));
+  const int min_x = std::max(0, static_cast<int>(cx - ex - 1.0f)), max_x = std::min(nx_ - 1, static_cast<int>(cx + ex + 1.0f));
+  const int min_y = std::max(0, static_cast<int>(cy - ey - 1.0f)), max_y = std::min(ny_ - 1, static_cast<int>(cy + ey + 1.0f));
+  const int min_z = std::max(0, static_cast<int>(cz - ez - 1.0f)), max_z = std::min(nz_ - 1, static_cast<int>(cz + ez + 1.0f));
+  const float irx = 1.0f / (rx * rx), iry = 1.0f / (ry * ry), irz = 1.0f / (rz * rz);
+  for (int z = min_z; z <= max_z; ++z)
+    for (int y = min_y; y <= max_y; ++y)
+      for (int x = min_x; x <= max_x; ++x) {
+        float dx = static_cast<float>(x) - cx, dy = static_cast<float>(y) - cy, dz = static_cast<float>(z) - cz;
+        if (axes) {
+          const float b0 = axes[0] * dx + axes[1] * dy + axes[2] * dz;
+          const float b1 = axes[3] * dx + axes[4] * dy + axes[5] * dz;
+          const float b2 = axes[6] * dx + axes[7] * dy + axes[8] * dz;
+          dx = b0;
+          dy = b1;
+          dz = b2;
+        }
+        if (dx * dx * irx + dy * dy * iry + dz * dz * irz <= 1.0f) set_cell_type(x, y, z, kSolid);
+      }
 }
 
 // lattice_grid_3d.cpp:160-182 (FP32 ball test, bit-exact flags)
@@ -503,10 +542,19 @@ StepArgs Domain::args(bool commit) const {
   a.gx = g_last_[0];
   a.gy = g_last_[1];
   a.gz = g_last_[2];
+  a.tau_g = tau_g_;
+  a.gbeta = gbeta_;
+  a.t_ref = t_ref_;
+  a.t_wall = d_twall_;
   return a;
 }
 
 void Domain::sync_device_config() {
+  if (twall_dirty_) {
+    check(hipMemcpyAsync(d_twall_, h_twall_.data(), N_ * sizeof(float), hipMemcpyHostToDevice, stream_),
+          "upload wall temperatures");
+    twall_dirty_ = false;
+  }
   if (flags_dirty_) {
     check(hipMemcpyAsync(d_flags_, flags_.data(), N_, hipMemcpyHostToDevice, stream_), "upload flags");
     flags_dirty_ = false;
@@ -568,11 +616,13 @@ void Domain::materialize() {
   const StepArgs a = args(true);
   const int out = 1 - a_buf_;
   check(launch_materialize(lattice_, storage_, a, buf_[a_buf_], buf_[out], stream_), "materialize");
+  if (thermal_) check(launch_materialize_thermal(a, gbuf_[a_buf_], gbuf_[out], stream_), "materialize temperature");
   s_buf_ = out;
 }
 
 // ------------------------------------------------------------------ step
 void Domain::step(int collision, float gx, float gy, float gz, bool use_les, float smag) {
+  ++version_;
   sync_device_config();
   if (dim_ == 2) gz = 0.0f;
   StepArgs a = args(true);
@@ -582,7 +632,16 @@ void Domain::step(int collision, float gx, float gy, float gz, bool use_les, flo
   a.gz = gz;
   a.cs = smag;
   const bool les = use_les && les_coupled_;
-  if (s_buf_ >= 0) {
+  if (thermal_) {
+    // coupled flow + temperature: one kernel, open boundaries inline
+    const bool pull = s_buf_ < 0;
+    const int in = pull ? a_buf_ : s_buf_;
+    const int out = 1 - in;
+    check(launch_step_thermal(storage_, collision, pull, a, buf_[in], buf_[out], gbuf_[in], gbuf_[out], stream_),
+          "thermal step");
+    a_buf_ = out;
+    s_buf_ = -1;
+  } else if (s_buf_ >= 0) {
     const int out = 1 - s_buf_;
     check(launch_step(lattice_, storage_, collision, les, false, a, buf_[s_buf_], buf_[out], block_x_, stream_),
           "step");
@@ -590,12 +649,22 @@ void Domain::step(int collision, float gx, float gy, float gz, bool use_les, flo
     s_buf_ = -1;
   } else {
     const int out = 1 - a_buf_;
-    a.split_planes = (a.inflow || a.outflow) ? 1 : 0;
+    if (time_kernels_) {
+      if (kev_used_ + 2 > kev_.size()) {
+        for (int e = 0; e < 2; ++e) {
+          hipEvent_t ev;
+          check(hipEventCreate(&ev), "event");
+          kev_.push_back(ev);
+        }
+      }
+      check(hipEventRecord(kev_[kev_used_], stream_), "event");
+    }
     check(launch_step(lattice_, storage_, collision, les, true, a, buf_[a_buf_], buf_[out], block_x_, stream_),
           "step");
-    if (a.split_planes)
-      check(launch_step_planes(lattice_, storage_, collision, les, a, buf_[a_buf_], buf_[out], stream_),
-            "step planes");
+    if (time_kernels_) {
+      check(hipEventRecord(kev_[kev_used_ + 1], stream_), "event");
+      kev_used_ += 2;
+    }
     a_buf_ = out;
   }
   g_last_[0] = gx;
@@ -613,7 +682,15 @@ float Domain::run(int steps, int collision, float gx, float gy, float gz, bool u
     check(hipEventCreate(&e1), "event");
     check(hipEventRecord(e0, stream_), "event");
   }
-  for (int s = 0; s < steps; ++s) step(collision, gx, gy, gz, use_les, smag);
+  time_kernels_ = timed;
+  kev_used_ = 0;
+  try {
+    for (int s = 0; s < steps; ++s) step(collision, gx, gy, gz, use_les, smag);
+  } catch (...) {
+    time_kernels_ = false;
+    throw;
+  }
+  time_kernels_ = false;
   float ms = 0.0f;
   if (timed) {
     check(hipEventRecord(e1, stream_), "event");
@@ -621,6 +698,13 @@ float Domain::run(int steps, int collision, float gx, float gy, float gz, bool u
     check(hipEventElapsedTime(&ms, e0, e1), "event");
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    double sum = 0.0;
+    for (size_t k = 0; k + 1 < kev_used_; k += 2) {
+      float t = 0.0f;
+      check(hipEventElapsedTime(&t, kev_[k], kev_[k + 1]), "event");
+      sum += t;
+    }
+    last_kernel_ms_ = kev_used_ ? static_cast<float>(sum / (kev_used_ / 2)) : 0.0f;
   }
   return ms;
 }
@@ -638,7 +722,14 @@ void Domain::ensure_fields() {
   a.uz = d_uz_;
   a.speed = d_speed_;
   const bool from_s = s_buf_ >= 0;
-  check(launch_readback(lattice_, storage_, from_s, a, buf_[from_s ? s_buf_ : a_buf_], stream_), "readback");
+  const int b = from_s ? s_buf_ : a_buf_;
+  if (thermal_) {
+    a.temp = d_temp_;
+    check(launch_readback_thermal(storage_, from_s, a, buf_[b], gbuf_[b], stream_), "readback");
+    check(hipMemcpyAsync(h_temp_.data(), d_temp_, N_ * 4, hipMemcpyDeviceToHost, stream_), "d2h");
+  } else {
+    check(launch_readback(lattice_, storage_, from_s, a, buf_[b], stream_), "readback");
+  }
   float* d[5] = {d_rho_, d_ux_, d_uy_, d_uz_, d_speed_};
   std::vector<float>* h[5] = {&h_rho_, &h_ux_, &h_uy_, &h_uz_, &h_speed_};
   for (int k = 0; k < 5; ++k) check(hipMemcpyAsync(h[k]->data(), d[k], N_ * 4, hipMemcpyDeviceToHost, stream_), "d2h");
@@ -741,8 +832,12 @@ double Domain::max_speed() {
 
 // forces_momentum_exchange.ispc:6-138: links from Fluid nodes to in-range
 // Solid or MovingWall neighbours whose midpoint lies within r_max.
-void Domain::momentum_exchange(float ccx, float ccy, float ccz, float r_max, double out[3]) {
-  sync_device_config();
+// Fluid nodes x and directions q (1..Q-1) whose in-range neighbour x + c_q
+// is Solid or MovingWall, with the link midpoint within r_max of the centre
+// (forces_momentum_exchange.ispc:71-138); q-major order as the reference's
+// loops.  Cached per geometry version, centre and radius.
+const std::pair<std::vector<int>, std::vector<int>>& Domain::mem_link_list(float ccx, float ccy, float ccz,
+                                                                            float r_max) {
   const auto key = std::make_tuple(ccx, ccy, ccz, r_max, flags_version_ * 4 + (ext_axes_ ? (per_y_ ? 2 : 0) + (per_z_ ? 1 : 0) : 0));
   auto it = mem_links_.find(key);
   if (it == mem_links_.end()) {
@@ -778,8 +873,14 @@ void Domain::momentum_exchange(float ccx, float ccy, float ccz, float r_max, dou
     if (mem_links_.size() > 8) mem_links_.clear();
     it = mem_links_.emplace(key, std::make_pair(std::move(nodes), std::move(qs))).first;
   }
-  const auto& nodes = it->second.first;
-  const auto& qs = it->second.second;
+  return it->second;
+}
+
+void Domain::momentum_exchange(float ccx, float ccy, float ccz, float r_max, double out[3]) {
+  sync_device_config();
+  const auto& lists = mem_link_list(ccx, ccy, ccz, r_max);
+  const auto& nodes = lists.first;
+  const auto& qs = lists.second;
   const int n = static_cast<int>(nodes.size());
   if (n == 0) {
     out[0] = out[1] = out[2] = 0.0;
@@ -788,12 +889,8 @@ void Domain::momentum_exchange(float ccx, float ccy, float ccz, float r_max, dou
   if (n > mem_cap_) {
     if (d_mem_node_) {
       (void)hipStreamSynchronize(stream_);
-      (void)hipFree(d_mem_node_);
-      (void)hipFree(d_mem_q_);
-      allocations_.erase(std::remove(allocations_.begin(), allocations_.end(), static_cast<void*>(d_mem_node_)),
-                         allocations_.end());
-      allocations_.erase(std::remove(allocations_.begin(), allocations_.end(), static_cast<void*>(d_mem_q_)),
-                         allocations_.end());
+      release(d_mem_node_);
+      release(d_mem_q_);
     }
     mem_cap_ = n;
     d_mem_node_ = static_cast<int*>(alloc(n * sizeof(int)));
@@ -808,6 +905,119 @@ void Domain::momentum_exchange(float ccx, float ccy, float ccz, float r_max, dou
         "momentum exchange");
   check(hipMemcpyAsync(out, d_out_, 3 * sizeof(double), hipMemcpyDeviceToHost, stream_), "d2h");
   check(hipStreamSynchronize(stream_), "momentum exchange");
+}
+
+// ------------------------------------------------------------- temperature
+void Domain::enable_thermal(float tau_g, float gbeta, float t_ref, float t_init) {
+  if (dim_ != 2) throw HipError("the temperature coupling is 2D (D2Q9 + D2Q5) only");
+  if (!(tau_g > 0.5f)) throw HipError("temperature relaxation time must exceed 1/2");
+  before_config_change();
+  if (!thermal_) {
+    for (int k = 0; k < 2; ++k) gbuf_[k] = static_cast<float*>(alloc(5 * static_cast<size_t>(stride_) * sizeof(float)));
+    d_twall_ = static_cast<float*>(alloc(N_ * sizeof(float)));
+    d_temp_ = static_cast<float*>(alloc(N_ * sizeof(float)));
+    h_temp_.assign(N_, 0.0f);
+  }
+  h_twall_.assign(N_, std::numeric_limits<float>::quiet_NaN());
+  twall_dirty_ = true;
+  thermal_ = true;
+  tau_g_ = tau_g;
+  gbeta_ = gbeta;
+  t_ref_ = t_ref;
+  std::vector<float> t(N_, t_init);
+  set_temperature_field(t.data());
+}
+
+void Domain::set_temperature_field(const float* t) {
+  if (!thermal_) throw HipError("temperature is not enabled on this domain");
+  before_config_change();
+  std::vector<float> host(5 * static_cast<size_t>(stride_), 0.0f);
+  for (int q = 0; q < 5; ++q)
+    for (size_t i = 0; i < N_; ++i) host[q * static_cast<size_t>(stride_) + i] = D2Q5::w[q] * t[i];
+  for (int k = 0; k < 2; ++k)
+    check(hipMemcpyAsync(gbuf_[k], host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice, stream_),
+          "upload temperature");
+  check(hipStreamSynchronize(stream_), "upload temperature");
+  fields_valid_ = false;
+}
+
+void Domain::set_wall_temperature(int x0, int y0, int x1, int y1, float t_wall) {
+  if (!thermal_) throw HipError("temperature is not enabled on this domain");
+  before_config_change();
+  for (int y = std::max(0, y0); y <= std::min(ny_ - 1, y1); ++y)
+    for (int x = std::max(0, x0); x <= std::min(nx_ - 1, x1); ++x) h_twall_[static_cast<size_t>(y) * nx_ + x] = t_wall;
+  twall_dirty_ = true;
+}
+
+const float* Domain::temperature() {
+  if (!thermal_) throw HipError("temperature is not enabled on this domain");
+  ensure_fields();
+  return h_temp_.data();
+}
+
+// ------------------------------------------------------------ batched steps
+void Domain::batch_prepare(float gx, float gy, float gz, float smag, StepArgs& a, void* buf[2], float* gbuf[2],
+                           int& parity, bool& s_state) {
+  sync_device_config();
+  if (dim_ == 2) gz = 0.0f;
+  a = args(true);
+  a.gx = gx;
+  a.gy = gy;
+  a.gz = gz;
+  a.cs = smag;
+  buf[0] = buf_[0];
+  buf[1] = buf_[1];
+  gbuf[0] = gbuf_[0];
+  gbuf[1] = gbuf_[1];
+  s_state = s_buf_ >= 0;
+  parity = s_state ? s_buf_ : a_buf_;
+  // uploads were issued on the domain's stream; the batch runs on its own
+  check(hipStreamSynchronize(stream_), "batch prepare");
+}
+
+void Domain::batch_advance(uint64_t k, int parity_after, float gx, float gy, float gz) {
+  if (k == 0) return;
+  a_buf_ = parity_after;
+  s_buf_ = -1;
+  step_count_ += k;
+  g_last_[0] = gx;
+  g_last_[1] = gy;
+  g_last_[2] = dim_ == 2 ? 0.0f : gz;
+  invalidate_mirrors();
+}
+
+int Domain::batch_mem_links(float ccx, float ccy, float ccz, float r_max, const int** node, const int** q) {
+  sync_device_config();
+  const auto& lists = mem_link_list(ccx, ccy, ccz, r_max);
+  const int n = static_cast<int>(lists.first.size());
+  if (d_bmem_node_) {
+    check(hipStreamSynchronize(stream_), "batch links");
+    release(d_bmem_node_);
+    release(d_bmem_q_);
+    d_bmem_node_ = d_bmem_q_ = nullptr;
+  }
+  if (n > 0) {
+    d_bmem_node_ = static_cast<int*>(alloc(n * sizeof(int)));
+    d_bmem_q_ = static_cast<int*>(alloc(n * sizeof(int)));
+    check(hipMemcpyAsync(d_bmem_node_, lists.first.data(), n * sizeof(int), hipMemcpyHostToDevice, stream_), "links");
+    check(hipMemcpyAsync(d_bmem_q_, lists.second.data(), n * sizeof(int), hipMemcpyHostToDevice, stream_), "links");
+    check(hipStreamSynchronize(stream_), "batch links");
+  }
+  *node = d_bmem_node_;
+  *q = d_bmem_q_;
+  return n;
+}
+
+size_t Domain::bytes_needed(int dim, int lattice, int nx, int ny, int nz, int storage, long long plane_pad,
+                            bool thermal) {
+  const int Q = dim == 2 ? 9 : (lattice == 27 ? 27 : 19);
+  const size_t N = static_cast<size_t>(nx) * ny * (dim == 2 ? 1 : nz);
+  const long long base = static_cast<long long>((N + 255) / 256 * 256);
+  const long long stride = base + (plane_pad >= 0 ? plane_pad : (N >= (1u << 20) ? 1024 : 0));
+  const size_t yz = static_cast<size_t>(ny) * (dim == 2 ? 1 : nz);
+  const size_t t = thermal ? 2 * 5 * static_cast<size_t>(stride) * 4 + N * 8 : 0;
+  return 2 * static_cast<size_t>(Q) * stride * elem_size(storage) + N * (1 + 1 + 4 + 5 * 4) + Q * yz * 4 +
+         3 * 1024 * 8 + 64 + t;
 }
 
 }  // namespace wake_hip

@@ -60,6 +60,7 @@ class Domain {
   void set_periodic_axes(bool per_y, bool per_z);
   void set_solid_box(int x0, int y0, int z0, int x1, int y1, int z1);
   void set_solid_sphere(float cx, float cy, float cz, float r);
+  void set_solid_ellipsoid(float cx, float cy, float cz, float rx, float ry, float rz, const float* axes);
   void set_solid_circle(float cx, float cy, float r);
   void set_solid_cylinder_z(float cx, float cy, float r);
   void set_moving_wall_top(float uwx, float uwz);
@@ -68,7 +69,10 @@ class Domain {
   void set_inflow_turbulence(float amplitude, uint64_t seed);
   void set_outflow_east(float rho_out);
   void set_convective_outflow_east(float u_conv);
-  void set_les_coupled(bool coupled) { les_coupled_ = coupled; }
+  void set_les_coupled(bool coupled) {
+    les_coupled_ = coupled;
+    ++version_;
+  }
   bool les_coupled() const { return les_coupled_; }
   int build_curved_sphere_links(float cx, float cy, float cz, float r);
   void set_curved_links(const CurvedLinks& links);
@@ -80,6 +84,10 @@ class Domain {
   // domain's stream around the launches.
   float run(int steps, int collision, float gx, float gy, float gz, bool use_les, float smag, bool timed);
   void synchronize();
+  // Average device duration (ms) of the main step kernel over the launches
+  // of the last timed run() (events around each launch on the domain's
+  // stream; the open-boundary plane kernel is excluded).
+  float last_kernel_ms() const { return last_kernel_ms_; }
 
   // --- read-back (lazily synced host mirrors, reference layout)
   const float* rho();
@@ -110,13 +118,48 @@ class Domain {
   int storage() const { return storage_; }
   uint64_t current_step() const { return step_count_; }
   float tau() const { return tau_; }
-  void set_tau(float t) { tau_ = t; }
+  void set_tau(float t) {
+    tau_ = t;
+    ++version_;
+  }
+  // Changes with every change of state or configuration made through this
+  // class (a batch re-reads a domain whose version moved).
+  uint64_t version() const { return version_; }
   hipStream_t stream() const { return stream_; }
   void set_block_x(int bx) { block_x_ = bx; }
   size_t device_bytes() const { return device_bytes_; }
   // device pointer of the population buffer holding the current state
   const void* state_buffer() const;
   bool state_is_post_stream() const { return s_buf_ >= 0; }
+
+  // --- D2Q5 temperature coupled by a Boussinesq force (2D only)
+  // Temperature populations at rest equilibrium w_q t_init everywhere, wall
+  // temperatures cleared (adiabatic); a_y = gbeta (T - t_ref) on the flow.
+  void enable_thermal(float tau_g, float gbeta, float t_ref, float t_init);
+  bool thermal() const { return thermal_; }
+  // Temperature field (reference node order) as rest equilibrium.
+  void set_temperature_field(const float* t);
+  // Wall temperature of the solid nodes in the box (inclusive); NaN makes
+  // them adiabatic again.
+  void set_wall_temperature(int x0, int y0, int x1, int y1, float t_wall);
+  const float* temperature();
+  const float* thermal_buffer(int k) const { return gbuf_[k]; }
+
+  // --- batched stepping (batch_host.cpp)
+  // Arguments of the next step for a batch launch, after the configuration
+  // has reached the device.  parity: buffer holding the current state;
+  // s_state: that state is a materialised S (the next step collides only).
+  void batch_prepare(float gx, float gy, float gz, float smag, StepArgs& a, void* buf[2], float* gbuf[2],
+                     int& parity, bool& s_state);
+  // A batch took k >= 1 steps; the state is now post-collision in buffer
+  // `parity_after`.
+  void batch_advance(uint64_t k, int parity_after, float gx, float gy, float gz);
+  // Momentum-exchange links around (cx, cy, cz) within r_max, uploaded to
+  // device arrays the domain owns (valid until the next call).
+  int batch_mem_links(float cx, float cy, float cz, float r_max, const int** node, const int** q);
+  // Device bytes a domain of this shape allocates.
+  static size_t bytes_needed(int dim, int lattice, int nx, int ny, int nz, int storage, long long plane_pad = -1,
+                             bool thermal = false);
 
  private:
   StepArgs args(bool commit) const;
@@ -127,6 +170,8 @@ class Domain {
   void upload_state_from_host(const std::vector<float>& f_ref);
   void invalidate_mirrors() { fields_valid_ = fsrc_valid_ = fdst_valid_ = false; }
   void* alloc(size_t bytes);
+  void release(void* p);
+  const std::pair<std::vector<int>, std::vector<int>>& mem_link_list(float cx, float cy, float cz, float r_max);
 
   int dim_, lattice_, Q_, nx_, ny_, nz_;
   size_t N_;
@@ -135,7 +180,12 @@ class Domain {
   size_t elem_;
   float tau_;
   uint64_t step_count_ = 0;
+  uint64_t version_ = 0;
   int block_x_ = 0;
+  bool time_kernels_ = false;
+  size_t kev_used_ = 0;
+  std::vector<hipEvent_t> kev_;
+  float last_kernel_ms_ = 0.0f;
 
   // reference grid configuration
   bool is_periodic_;
@@ -171,11 +221,22 @@ class Domain {
   int* d_mem_node_ = nullptr;
   int* d_mem_q_ = nullptr;
   int mem_cap_ = 0;
+  int* d_bmem_node_ = nullptr;
+  int* d_bmem_q_ = nullptr;
   std::vector<void*> allocations_;
   size_t device_bytes_ = 0;
   bool flags_dirty_ = true, cls_dirty_ = true, bfl_dirty_ = true;
   uint64_t flags_version_ = 0;
   std::map<std::tuple<float, float, float, float, uint64_t>, std::pair<std::vector<int>, std::vector<int>>> mem_links_;
+
+  // temperature (D2Q5), ping-pong with buf_
+  bool thermal_ = false;
+  float tau_g_ = 0.8f, gbeta_ = 0.0f, t_ref_ = 0.0f;
+  float* gbuf_[2] = {nullptr, nullptr};
+  float* d_twall_ = nullptr;
+  float* d_temp_ = nullptr;
+  std::vector<float> h_twall_, h_temp_;
+  bool twall_dirty_ = false;
 
   // host mirrors
   std::vector<float> h_rho_, h_ux_, h_uy_, h_uz_, h_speed_, h_fsrc_, h_fdst_;

@@ -56,7 +56,9 @@ struct StepArgs {
   float rho_out, u_conv;
   int by0, by1, bz0, bz1;     // rows an open boundary acts on
   int commit_prev;            // convective: store S_k(outlet) as next step's f_prev
-  int split_planes;           // k_step leaves inlet/outlet nodes to k_step_planes
+  // D2Q5 temperature coupled by a Boussinesq force (2D only)
+  float tau_g, gbeta, t_ref;  // temperature relaxation, a_y = gbeta (T - t_ref)
+  const float* t_wall;        // per node: wall temperature of a solid node, NaN = adiabatic
   float* f_prev;              // convective outlet store: Q planes of ny*nz
   const uint8_t* flags;       // reference cell types
   const uint8_t* cls;         // ClassBits
@@ -69,6 +71,7 @@ struct StepArgs {
   float* uz;
   float* speed;
   float* s_out;               // optional: S_k planes, fp32, stride nx*ny*nz
+  float* temp;                // read-back temperature (thermal domains)
 };
 
 // Launchers (step_kernels.hip).  `lattice` is 9, 19 or 27.
@@ -78,11 +81,17 @@ hipError_t launch_mark_bfl(const int* nodes, int count, uint8_t* cls, int* aux, 
 // work-group width along x (64, 128 or 256).
 hipError_t launch_step(int lattice, int storage, int collision, bool les, bool pull, const StepArgs& a,
                        const void* a_in, void* a_out, int block_x, hipStream_t s);
-// The inlet (x = 0) and outlet (x = nx - 1) nodes of a pull step, when
-// a.split_planes is set: same gather, open boundaries and collision as
-// k_step, one lane per (y, z) of each open plane.
-hipError_t launch_step_planes(int lattice, int storage, int collision, bool les, const StepArgs& a,
-                              const void* a_in, void* a_out, hipStream_t s);
+// Thermal D2Q9 + D2Q5 domains: the same step with the temperature
+// populations g (FP32, 5 planes of a.stride) pulled, coupled and collided
+// beside f.  pull=false collides the materialised S state of both.
+hipError_t launch_step_thermal(int storage, int collision, bool pull, const StepArgs& a, const void* a_in,
+                               void* a_out, const float* g_in, float* g_out, hipStream_t s);
+// S_k of the temperature populations into g_s (the flow's S is materialised
+// by launch_materialize).
+hipError_t launch_materialize_thermal(const StepArgs& a, const float* g_in, float* g_s, hipStream_t s);
+// rho, u (with the node's buoyancy in the Guo shift), |u| and T of S_k.
+hipError_t launch_readback_thermal(int storage, bool from_s, const StepArgs& a, const void* buf,
+                                   const float* gbuf, hipStream_t s);
 // Materialise S_k (commit=true updates the convective outlet store) into
 // s_state (storage format, stride a.stride), or fields/planes for read-back.
 hipError_t launch_materialize(int lattice, int storage, const StepArgs& a, const void* a_in, void* s_state,

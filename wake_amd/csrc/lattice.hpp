@@ -35,6 +35,16 @@ struct D2Q9 {
                                  1.0f / 36.0f, 1.0f / 36.0f, 1.0f / 36.0f, 1.0f / 36.0f};
 };
 
+// D2Q5 temperature lattice (no reference counterpart): c = 0, +x, +y, -x, -y.
+// w_0 = 1 - 4 w_1 in FP32, so the weights sum to exactly one.
+struct D2Q5 {
+  static constexpr int Q = 5;
+  static constexpr int cx[Q] = {0, 1, 0, -1, 0};
+  static constexpr int cy[Q] = {0, 0, 1, 0, -1};
+  static constexpr int opp[Q] = {0, 3, 4, 1, 2};
+  static constexpr float w[Q] = {1.0f - 4.0f * (1.0f / 6.0f), 1.0f / 6.0f, 1.0f / 6.0f, 1.0f / 6.0f, 1.0f / 6.0f};
+};
+
 struct D3Q19 {
   static constexpr int Q = 19;
   static constexpr int D = 3;

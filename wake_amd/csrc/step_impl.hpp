@@ -195,7 +195,8 @@ __device__ __forceinline__ void gather(const StepArgs& a, const typename St::T* 
 
 // --------------------------------------------------------- open boundaries
 template <class L>
-__device__ __forceinline__ void apply_inlet(const StepArgs& a, float (&f)[L::Q], int y, int n) {
+__device__ __forceinline__ void apply_inlet(const StepArgs& a, float (&f)[L::Q], int y, int n,
+                                            unsigned long long s_step) {
   float ux = a.uin_x;
   if (a.inflow_parabolic) {
     const float s = static_cast<float>(y) - 0.5f;
@@ -204,7 +205,7 @@ __device__ __forceinline__ void apply_inlet(const StepArgs& a, float (&f)[L::Q],
   }
   if (a.turb_amp != 0.0f) {
     const unsigned long long N = static_cast<unsigned long long>(a.nx) * a.ny * a.nz;
-    ux = ux * (1.0f + a.turb_amp * rng_sym(a.turb_seed, a.s_step * N + static_cast<unsigned long long>(n)));
+    ux = ux * (1.0f + a.turb_amp * rng_sym(a.turb_seed, s_step * N + static_cast<unsigned long long>(n)));
   }
   const float uy = a.uin_y;
   if constexpr (L::Q == 9) {
@@ -295,14 +296,15 @@ __device__ __forceinline__ void apply_zh_outflow_d2q9(const StepArgs& a, float (
   f[7] = f7;
 }
 
-// Full post-stream state of a node: gather + open boundaries.  The convective
-// outlet needs the post-stream state of its upstream neighbour, which the
-// outlet lane gathers itself.
+// Full post-stream state S_k of a node (k = s_step): gather + open
+// boundaries.  The convective outlet needs the post-stream state of its
+// upstream neighbour, which the outlet lane gathers itself.
 template <class L, class St>
-__device__ __forceinline__ void form_s(const StepArgs& a, const typename St::T* __restrict__ A, int x, int y, int z,
-                                       int n, uint8_t c, bool commit, float (&f)[L::Q]) {
+__device__ __forceinline__ void form_s_at(const StepArgs& a, const typename St::T* __restrict__ A, int x, int y,
+                                          int z, int n, uint8_t c, bool commit, unsigned long long s_step,
+                                          float (&f)[L::Q]) {
   gather<L, St>(a, A, x, y, z, n, c, f);
-  if (c & kClsInlet) apply_inlet<L>(a, f, y, n);
+  if (c & kClsInlet) apply_inlet<L>(a, f, y, n, s_step);
   if (c & kClsOutlet) {
     if constexpr (L::Q == 9) {
       if (a.outflow == 1) apply_zh_outflow_d2q9(a, f);
@@ -311,7 +313,7 @@ __device__ __forceinline__ void form_s(const StepArgs& a, const typename St::T* 
       float nb[L::Q];
       const uint8_t cn = a.cls[n - 1];
       gather<L, St>(a, A, x - 1, y, z, n - 1, cn, nb);
-      if (cn & kClsInlet) apply_inlet<L>(a, nb, y, n - 1);
+      if (cn & kClsInlet) apply_inlet<L>(a, nb, y, n - 1, s_step);
       const float alpha = a.u_conv / (1.0f + a.u_conv);
       const float beta = 1.0f / (1.0f + a.u_conv);
       const long long plane = static_cast<long long>(a.ny) * a.nz;
@@ -324,6 +326,12 @@ __device__ __forceinline__ void form_s(const StepArgs& a, const typename St::T* 
       });
     }
   }
+}
+
+template <class L, class St>
+__device__ __forceinline__ void form_s(const StepArgs& a, const typename St::T* __restrict__ A, int x, int y, int z,
+                                       int n, uint8_t c, bool commit, float (&f)[L::Q]) {
+  form_s_at<L, St>(a, A, x, y, z, n, c, commit, a.s_step, f);
 }
 
 // ----------------------------------------------------------------- kernels
@@ -385,10 +393,7 @@ __global__ __launch_bounds__(256) WAKE_STEP_ATTR void k_step(const StepArgs a, c
     n = (z * a.ny + y) * a.nx + x;
     c = a.cls[n];
     if constexpr (PULL) {
-      // with split planes the open-boundary nodes are only gathered here
-      // (and skipped below); k_step_planes forms and commits them
-      const uint8_t cb = a.split_planes ? static_cast<uint8_t>(c & ~(kClsInlet | kClsOutlet)) : c;
-      form_s<L, St>(a, ain, x, y, z, n, cb, true, f);
+      form_s<L, St>(a, ain, x, y, z, n, c, true, f);
     } else {
       static_for<L::Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
@@ -412,8 +417,6 @@ __global__ __launch_bounds__(256) WAKE_STEP_ATTR void k_step(const StepArgs a, c
     const uint8_t c = cn;
     const int n = nn;
     if (k + 1 < WAKE_ZK && z + 1 < a.nz) fetch(z + 1, fn, cn, nn);
-    // open-boundary nodes belong to k_step_planes, so no bulk wave diverges
-    if (PULL && a.split_planes && (c & (kClsInlet | kClsOutlet))) continue;
     if (!(c & kClsSolid)) {
       if constexpr (LES) collide_les<L, COLL>(f, a.tau, a.cs, a.gx, a.gy, a.gz);
       else collide<L, COLL>(f, a.tau, a.gx, a.gy, a.gz);
@@ -423,31 +426,6 @@ __global__ __launch_bounds__(256) WAKE_STEP_ATTR void k_step(const StepArgs a, c
       St::template st<L, q>(aout, q * S + n, f[q]);
     });
   }
-}
-
-// Inlet and outlet planes of a split pull step: blockIdx.y = 0 for the
-// x = 0 plane, 1 for x = nx - 1; lanes run over (y, z) of the plane.
-template <class L, class St, int COLL, bool LES>
-__global__ __launch_bounds__(256) void k_step_planes(const StepArgs a, const typename St::T* __restrict__ ain,
-                                                     typename St::T* __restrict__ aout) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.ny * a.nz) return;
-  const int y = i % a.ny, z = i / a.ny;
-  if (blockIdx.y == 1 && a.nx == 1) return;  // one plane is both: formed once
-  const int x = blockIdx.y == 0 ? 0 : a.nx - 1;
-  const int n = (z * a.ny + y) * a.nx + x;
-  const uint8_t c = a.cls[n];
-  if (!(c & (kClsInlet | kClsOutlet))) return;
-  float f[L::Q];
-  form_s<L, St>(a, ain, x, y, z, n, c, true, f);
-  if (!(c & kClsSolid)) {
-    if constexpr (LES) collide_les<L, COLL>(f, a.tau, a.cs, a.gx, a.gy, a.gz);
-    else collide<L, COLL>(f, a.tau, a.gx, a.gy, a.gz);
-  }
-  static_for<L::Q>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    St::template st<L, q>(aout, q * a.stride + n, f[q]);
-  });
 }
 
 // S_k into the state buffer (storage format), committing the convective store.
@@ -504,14 +482,15 @@ __global__ __launch_bounds__(256) void k_readback(const StepArgs a, const typena
   }
 }
 
-// Momentum exchange on S_k over a link list, FP64, one work-group, fixed
-// reduction order (forces_momentum_exchange.ispc:6-138 restricted to links).
+// Momentum exchange contributions of this lane's links (every blockDim.x-th
+// from threadIdx.x) on S_k: F += 2 f_q c_q, and in 2D the torque about
+// (ccx, ccy) at the link midpoint (forces_momentum_exchange.ispc:6-138,
+// restricted to the precomputed links).
 template <class L, class St, bool FROM_S>
-__global__ __launch_bounds__(256) void k_mem(const StepArgs a, const typename St::T* __restrict__ buf,
-                                             const int* __restrict__ link_node, const int* __restrict__ link_q,
-                                             int nlinks, float ccx, float ccy, float ccz, double* __restrict__ out) {
-  __shared__ double red[3][256];
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+__device__ __forceinline__ void mem_links(const StepArgs& a, const typename St::T* __restrict__ buf,
+                                          unsigned long long s_step, const int* __restrict__ link_node,
+                                          const int* __restrict__ link_q, int nlinks, float ccx, float ccy,
+                                          double& s0, double& s1, double& s2) {
   for (int i = threadIdx.x; i < nlinks; i += blockDim.x) {
     const int n = link_node[i];
     const int qd = link_q[i];
@@ -525,7 +504,7 @@ __global__ __launch_bounds__(256) void k_mem(const StepArgs a, const typename St
         f[q] = St::template ld<L, q>(buf, q * a.stride + n);
       });
     } else {
-      form_s<L, St>(a, buf, x, y, z, n, a.cls[n], false, f);
+      form_s_at<L, St>(a, buf, x, y, z, n, a.cls[n], false, s_step, f);
     }
     float fv = 0.0f;
     int cx = 0, cy = 0, cz = 0;
@@ -552,6 +531,12 @@ __global__ __launch_bounds__(256) void k_mem(const StepArgs a, const typename St
       s2 += static_cast<double>(2.0f * fv * static_cast<float>(cz));
     }
   }
+}
+
+// Sum of three per-lane values over the work-group in a fixed tree order
+// (blockDim.x a power of two, at most 256); the result is valid in lane 0.
+__device__ __forceinline__ void block_sum3(double& s0, double& s1, double& s2) {
+  __shared__ double red[3][256];
   red[0][threadIdx.x] = s0;
   red[1][threadIdx.x] = s1;
   red[2][threadIdx.x] = s2;
@@ -561,8 +546,26 @@ __global__ __launch_bounds__(256) void k_mem(const StepArgs a, const typename St
       for (int k = 0; k < 3; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0)
-    for (int k = 0; k < 3; ++k) out[k] = red[k][0];
+  s0 = red[0][0];
+  s1 = red[1][0];
+  s2 = red[2][0];
+}
+
+// Momentum exchange on S_k over a link list, FP64, one work-group, fixed
+// reduction order.
+template <class L, class St, bool FROM_S>
+__global__ __launch_bounds__(256) void k_mem(const StepArgs a, const typename St::T* __restrict__ buf,
+                                             const int* __restrict__ link_node, const int* __restrict__ link_q,
+                                             int nlinks, float ccx, float ccy, float ccz, double* __restrict__ out) {
+  (void)ccz;  // the 3D force needs no moment arm
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  mem_links<L, St, FROM_S>(a, buf, a.s_step, link_node, link_q, nlinks, ccx, ccy, s0, s1, s2);
+  block_sum3(s0, s1, s2);
+  if (threadIdx.x == 0) {
+    out[0] = s0;
+    out[1] = s1;
+    out[2] = s2;
+  }
 }
 
 }  // namespace wake_hip

@@ -80,16 +80,6 @@ hipError_t launch_step(int lattice, int storage, int collision, bool les, bool p
   }
 }
 
-hipError_t launch_step_planes(int lattice, int storage, int collision, bool les, const StepArgs& a,
-                              const void* a_in, void* a_out, hipStream_t s) {
-  switch (lattice) {
-    case 9: return detail::step_planes_d2q9(storage, collision, les, a, a_in, a_out, s);
-    case 19: return detail::step_planes_d3q19(storage, collision, les, a, a_in, a_out, s);
-    case 27: return detail::step_planes_d3q27(storage, collision, les, a, a_in, a_out, s);
-    default: return hipErrorInvalidValue;
-  }
-}
-
 hipError_t launch_materialize(int lattice, int storage, const StepArgs& a, const void* a_in, void* s_state,
                               hipStream_t s) {
   switch (lattice) {

@@ -66,30 +66,6 @@ struct Launch {
     });
   }
 
-  static hipError_t step_planes(int storage, int collision, bool les, const StepArgs& a, const void* a_in,
-                                void* a_out, hipStream_t s) {
-    const dim3 g((a.ny * a.nz + 255) / 256, 2, 1);
-    return with_storage(storage, [&](auto st) {
-      using St = decltype(st);
-      using T = typename St::T;
-      const T* i = static_cast<const T*>(a_in);
-      T* o = static_cast<T*>(a_out);
-      auto go = [&](auto coll) {
-        constexpr int C = decltype(coll)::value;
-        if (les) k_step_planes<L, St, C, true><<<g, 256, 0, s>>>(a, i, o);
-        else k_step_planes<L, St, C, false><<<g, 256, 0, s>>>(a, i, o);
-        return hipGetLastError();
-      };
-      if (collision == kBGK) return go(std::integral_constant<int, kBGK>{});
-      if constexpr (L::Q == 27) {
-        return go(std::integral_constant<int, kTRT>{});
-      } else {
-        if (collision == kTRT) return go(std::integral_constant<int, kTRT>{});
-        return go(std::integral_constant<int, kMRT>{});
-      }
-    });
-  }
-
   static hipError_t materialize(int storage, const StepArgs& a, const void* a_in, void* s_state, hipStream_t s) {
     const int bx = pick_block(a.nx, 0);
     return with_storage(storage, [&](auto st) {
@@ -132,8 +108,6 @@ struct Launch {
   hipError_t build_class_##NAME(const StepArgs& a, uint8_t* cls, hipStream_t s);                               \
   hipError_t step_##NAME(int storage, int collision, bool les, bool pull, const StepArgs& a, const void* a_in, \
                          void* a_out, int block_x, hipStream_t s);                                             \
-  hipError_t step_planes_##NAME(int storage, int collision, bool les, const StepArgs& a, const void* a_in,      \
-                                void* a_out, hipStream_t s);                                                   \
   hipError_t materialize_##NAME(int storage, const StepArgs& a, const void* a_in, void* s_state, hipStream_t s); \
   hipError_t readback_##NAME(int storage, bool from_s, const StepArgs& a, const void* buf, hipStream_t s);      \
   hipError_t mem_##NAME(int storage, bool from_s, const StepArgs& a, const void* buf, const int* link_node,    \
@@ -146,10 +120,6 @@ struct Launch {
   hipError_t step_##NAME(int storage, int collision, bool les, bool pull, const StepArgs& a, const void* a_in,  \
                          void* a_out, int block_x, hipStream_t s) {                                             \
     return Launch<LAT>::step(storage, collision, les, pull, a, a_in, a_out, block_x, s);                        \
-  }                                                                                                             \
-  hipError_t step_planes_##NAME(int storage, int collision, bool les, const StepArgs& a, const void* a_in,      \
-                                void* a_out, hipStream_t s) {                                                  \
-    return Launch<LAT>::step_planes(storage, collision, les, a, a_in, a_out, s);                                \
   }                                                                                                             \
   hipError_t materialize_##NAME(int storage, const StepArgs& a, const void* a_in, void* s_state, hipStream_t s) { \
     return Launch<LAT>::materialize(storage, a, a_in, s_state, s);                                              \

@@ -150,6 +150,12 @@ class _Domain:
                                         ctypes.byref(ms) if timed else None), "run")
         return ms.value
 
+    def kernel_time(self):
+        """Average device ms of the fused step kernel per launch in the last timed run()."""
+        t = ctypes.c_float()
+        self._c(self._L.wake_domain_kernel_time(self._h, ctypes.byref(t)), "kernel_time")
+        return t.value
+
     def synchronize(self):
         self._c(self._L.wake_domain_synchronize(self._h), "synchronize")
 
@@ -173,6 +179,10 @@ class _Domain:
 
     def f_src_data(self):
         return self._read(5, self.Q * self.total_nodes()).reshape(self.Q, -1)
+
+    def temperature_data(self):
+        """Temperature of a thermal (D2Q9 + D2Q5) domain."""
+        return self._read(7, self.total_nodes())
 
     def cell_type_data(self):
         out = np.empty(self.total_nodes(), np.uint8)
@@ -241,6 +251,18 @@ class LatticeGrid2D(_Domain):
     def set_solid_box(self, x0, y0, x1, y1):
         self._c(self._L.wake_domain_set_solid_box(self._h, x0, y0, 0, x1, y1, 0), "set_solid_box")
 
+    def enable_thermal(self, tau_g, gbeta, t_ref=0.5, t_init=0.5):
+        """D2Q5 temperature coupled by a_y = gbeta (T - t_ref); alpha = (tau_g - 1/2) / 3."""
+        self._c(self._L.wake_domain_enable_thermal(self._h, tau_g, gbeta, t_ref, t_init), "enable_thermal")
+
+    def set_temperature_field(self, t):
+        a = np.ascontiguousarray(t, np.float32)
+        self._c(self._L.wake_domain_set_temperature_field(self._h, _lib.ptr(a)), "set_temperature_field")
+
+    def set_wall_temperature(self, x0, y0, x1, y1, t_wall):
+        """Anti-bounce-back temperature of the solid nodes in the box; NaN = adiabatic."""
+        self._c(self._L.wake_domain_set_wall_temperature(self._h, x0, y0, x1, y1, t_wall), "set_wall_temperature")
+
     def set_solid_circle(self, cx, cy, radius):
         self._c(self._L.wake_domain_set_solid_circle(self._h, cx, cy, radius), "set_solid_circle")
 
@@ -308,6 +330,13 @@ class LatticeGrid3D(_Domain):
 
     def set_solid_sphere(self, cx, cy, cz, radius):
         self._c(self._L.wake_domain_set_solid_sphere(self._h, cx, cy, cz, radius), "set_solid_sphere")
+
+    def set_solid_ellipsoid(self, cx, cy, cz, rx, ry, rz, axes=None):
+        """primitives.cpp:73-100; axes: optional 3x3 whose rows are the body
+        axes in world coordinates (rotated ellipsoid)."""
+        a = None if axes is None else np.ascontiguousarray(np.asarray(axes, np.float32).reshape(9))
+        self._c(self._L.wake_domain_set_solid_ellipsoid(self._h, cx, cy, cz, rx, ry, rz, _lib.ptr(a)),
+                "set_solid_ellipsoid")
 
     def set_solid_cylinder_z(self, cx, cy, radius):
         self._c(self._L.wake_domain_set_solid_cylinder_z(self._h, cx, cy, radius), "set_solid_cylinder_z")
