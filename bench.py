@@ -10,7 +10,10 @@ launch over the whole domain; MLUPS counts every node, as the reference does
 `--workload batched-wakes` runs the batched multi-domain path instead: 512
 D2Q9 MRT cylinder-wake domains of uneven size, one launch per step over all
 of them, and compares simulations per second with the plain
-one-domain-at-a-time path.
+one-domain-at-a-time path.  `--workload serve` replays a Poisson stream of
+mixed 3D bluff-body and 2D thermal-cavity requests through the
+continuous-batching serving loop and then through the plain sequential
+loop, and reports simulations per second and p50/p99 time-to-result.
 
 Prints ONE JSON line.  The roofline entry divides the algorithmic bytes of
 a step (one read and one write of every population: 152 B/node for D3Q19
@@ -35,6 +38,22 @@ METRIC = "D3Q19 MLUPS & %HBM peak; batched-serve vs plain sims/s + p99 at stated
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def measured_traffic(n, storage):
+    """HBM bytes per k_step launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (256^3 FP32 only), else None."""
+    if n != 256 or storage != 0:
+        return None
+    path = os.path.join(ROOT, "profiles", "k_step_d3q19_sphere256_pmc.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        return None
+    return None
 
 
 def sphere_workload(grid_cls, n=256, R=24.0, storage=0, lattice=19, plane_pad=-1):
@@ -197,10 +216,72 @@ def main_batched(args):
     print(json.dumps(out), flush=True)
 
 
+def main_serve(args):
+    """Serving stream: Poisson arrivals of mixed requests under an HBM
+    budget, served by the continuous-batching loop, then the same stream
+    through the plain one-request-at-a-time FP32 loop."""
+    import numpy as np
+
+    import wake_amd  # noqa: F401
+    from wake_amd import serve
+
+    reqs = serve.request_stream(args.requests, rate=args.rate, seed=5, step_scale=args.step_scale)
+    work = sum((r.n ** 3 if r.kind == "bluff3d" else r.n ** 2) * r.steps for r in reqs)
+    log(f"{len(reqs)} requests over {reqs[-1].arrival:.2f} s, {work / 1e9:.1f} G node-steps, "
+        f"{sum(r.kind == 'bluff3d' for r in reqs)} bluff3d / {sum(r.kind == 'cavity2d' for r in reqs)} cavity2d, "
+        f"{sum(r.tol >= 1e-3 for r in reqs)} with FP16 storage")
+    s = serve.Server(budget_bytes=args.budget_gb * 1e9, chunk=100)
+    t_srv = s.serve(reqs, realtime=True)
+    lat = serve.latency_stats(s.done)
+    log(f"served: {t_srv:.2f} s makespan, p50 {lat['p50_s']:.2f} s, p99 {lat['p99_s']:.2f} s, "
+        f"peak HBM {s.peak_used / 1e9:.1f} GB, {s.ticks} ticks")
+    plain, t_plain = serve.plain_loop(reqs, realtime=True)
+    lat_p = serve.latency_stats(plain)
+    log(f"plain:  {t_plain:.2f} s makespan, p50 {lat_p['p50_s']:.2f} s, p99 {lat_p['p99_s']:.2f} s")
+    # agreement of served (FP16 where allowed) and plain FP32 results
+    by = {r.rid: r for r in plain}
+    dev = []
+    for r in s.done:
+        for k, v in r.result.items():
+            ref = by[r.rid].result[k]
+            if k == "C_L" or not (math.isfinite(v) and math.isfinite(ref)):
+                continue
+            dev.append(abs(v / ref - 1) if ref != 0 else abs(v))
+    out = {
+        "metric": METRIC,
+        "value": round(len(reqs) / t_srv, 4),
+        "unit": "sims/s",
+        "n_gpus": 1,
+        "steps": int(sum(r.steps for r in reqs)),
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp16/fp32-storage, fp32 arithmetic",
+        "data": "synthetic",
+        "config": {
+            "workload": f"serving stream: {len(reqs)} Poisson arrivals at {args.rate}/s (seed 5), 60% D3Q19 MRT-LES "
+                        f"rotated-ellipsoid bluff bodies 64^3-192^3, 40% D2Q9+D2Q5 Boussinesq cavities 128^2-512^2, "
+                        f"steps U[2k,20k] x {args.step_scale}, HBM budget {args.budget_gb} GB",
+            "served_makespan_s": round(t_srv, 3), "plain_makespan_s": round(t_plain, 3),
+            "plain_sims_per_s": round(len(reqs) / t_plain, 4), "speedup_vs_plain": round(t_plain / t_srv, 3),
+            "p50_s": round(lat["p50_s"], 3), "p99_s": round(lat["p99_s"], 3),
+            "plain_p50_s": round(lat_p["p50_s"], 3), "plain_p99_s": round(lat_p["p99_s"], 3),
+            "aggregate_mlups": round(work / t_srv / 1e6, 1), "plain_mlups": round(work / t_plain / 1e6, 1),
+            "peak_hbm_gb": round(s.peak_used / 1e9, 2),
+            "result_rel_dev_vs_plain_fp32": {"median": float(np.median(dev)) if dev else None,
+                                             "max": float(np.max(dev)) if dev else None},
+            "parallelism": "single GPU",
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["sphere", "batched-wakes"], default="sphere")
+    ap.add_argument("--workload", choices=["sphere", "batched-wakes", "serve"], default="sphere")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=256)
@@ -211,6 +292,10 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--domains", type=int, default=512)
     ap.add_argument("--plain-probe", type=int, default=20)
+    ap.add_argument("--requests", type=int, default=40)
+    ap.add_argument("--rate", type=float, default=20.0)
+    ap.add_argument("--step-scale", type=float, default=1.0)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -218,6 +303,9 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "serve":
+        main_serve(args)
+        return
     if args.workload == "batched-wakes":
         if args.steps is None:
             args.steps = 20000
@@ -294,7 +382,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(n, storage),
+            "traffic_source": "profiles/k_step_d3q19_sphere256_pmc.md (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, "
+                              "copy-calibrated)",
             "bytes_per_node": bpn,
             "kernel": "k_step (fused pull + MRT/LES collide), HIP events per launch",
             "kernel_ms": round(kernel_ms, 5),

@@ -160,6 +160,10 @@ enum wake_field {
  * lattice: 9 (2D), 19 or 27 (3D).  storage: WAKE_STORE_F32, or FP16 / BF16
  * storage of f_q - w_q with FP32 arithmetic.  plane_pad < 0 picks the
  * default padding of the population plane stride. */
+/* Device bytes a domain of this shape allocates (with the temperature
+ * populations if thermal != 0); host-only, for admission control. */
+long long wake_domain_bytes_needed(int dim, int lattice, int nx, int ny, int nz, int storage, long long plane_pad,
+                                   int thermal);
 int wake_domain_create(int dim, int lattice, int nx, int ny, int nz, float tau, int storage, long long plane_pad,
                        wake_domain** out);
 void wake_domain_destroy(wake_domain* d);
@@ -278,6 +282,12 @@ int wake_batch_remove(wake_batch* b, int slot);
 /* `steps` launches (each member takes min(steps, its remaining steps));
  * *ms = device time.  use_graph = 0 launches each step directly. */
 int wake_batch_run(wake_batch* b, long long steps, int use_graph, float* ms);
+/* Asynchronous form: enqueue one chunk (min(steps, chunk) launches) on the
+ * batch's own stream and return; wake_batch_wait completes it (*ms = device
+ * time).  Chunks of different batches in flight together overlap on the
+ * GPU.  No other call on the batch or its members until the wait. */
+int wake_batch_launch(wake_batch* b, long long steps, int use_graph);
+int wake_batch_wait(wake_batch* b, float* ms);
 int wake_batch_remaining(wake_batch* b, int slot, long long* remaining);
 /* Force history since the member was added: *count steps, up to `capacity`
  * of them copied to out (3 doubles each). */

@@ -20,14 +20,15 @@ def build_oracle():
 
 
 def lib(bits=32):
-    """Load liboracle32.so (float) or liboracle64.so (double)."""
+    """Load liboracle32.so (float), liboracle64.so (double) or, for bits =
+    "32fma", the float build with fused multiply-adds."""
     if bits in _libs:
         return _libs[bits]
     path = os.path.join(ORACLE_DIR, f"liboracle{bits}.so")
     if not os.path.exists(path):
         build_oracle()
     L = ctypes.CDLL(path)
-    real = ctypes.c_float if bits == 32 else ctypes.c_double
+    real = ctypes.c_double if bits == 64 else ctypes.c_float
     P = ctypes.c_void_p
     I = ctypes.c_int
     L.oracle_grid_create.restype = P
@@ -77,7 +78,7 @@ def lib(bits=32):
     L.oracle_rng_u01.restype = ctypes.c_double
     L.oracle_rng_u01.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
     L.real = real
-    L.np_real = np.float32 if bits == 32 else np.float64
+    L.np_real = np.float64 if bits == 64 else np.float32
     _libs[bits] = L
     return L
 

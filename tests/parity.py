@@ -19,11 +19,22 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def check_fp32_parity(gpu, o32, o64, what="u"):
-    """Assert the bar, or the FP32-floor criterion where the floor exceeds it."""
+def check_fp32_parity(gpu, o32, o64, what="u", o32fma=None):
+    """Assert the bar, or the FP32-floor criterion where the floor exceeds it.
+
+    With o32fma (the FP32 oracle built with fused multiply-adds, as the
+    GPU compiler and the reference's fast-math ISPC build contract them) the
+    floor is the larger distance of the two FP32 builds from FP64, and the
+    GPU may sit as far from the FP32 oracle as 1.25x the spread between the
+    two FP32 builds: both are faithful FP32 evaluations of the same step."""
     floor = rel_l2(o32, o64)
+    spread = 0.0
+    if o32fma is not None:
+        floor = max(floor, rel_l2(o32fma, o64))
+        spread = rel_l2(o32fma, o32)
     e32, e64 = rel_l2(gpu, o32), rel_l2(gpu, o64)
-    print(f"{what}: GPU vs FP32 oracle {e32:.2e}, GPU vs FP64 {e64:.2e}, FP32 floor {floor:.2e}")
-    assert e32 <= FIELD_TOL or (e32 <= 0.5 * floor and e64 <= 1.25 * floor), \
-        f"{what}: GPU vs FP32 {e32:.2e}, vs FP64 {e64:.2e}, floor {floor:.2e}"
+    print(f"{what}: GPU vs FP32 oracle {e32:.2e}, GPU vs FP64 {e64:.2e}, FP32 floor {floor:.2e}, "
+          f"FMA spread {spread:.2e}")
+    assert e32 <= FIELD_TOL or (e32 <= max(0.5 * floor, 1.25 * spread) and e64 <= 1.25 * floor), \
+        f"{what}: GPU vs FP32 {e32:.2e}, vs FP64 {e64:.2e}, floor {floor:.2e}, spread {spread:.2e}"
     return e32, e64, floor

@@ -61,15 +61,19 @@ def test_thermal_cavity_parity_with_oracle(wa):
     g, alpha, H = cavity(wa, n)
     o, _, _ = cavity(None, n, oracle_bits=32)
     o64, _, _ = cavity(None, n, oracle_bits=64)
+    ofma, _, _ = cavity(None, n, oracle_bits="32fma")
     for _ in range(steps):
         g.step("MRT")
-    o.run(steps, "MRT")
-    o64.run(steps, "MRT")
+    for x in (o, o64, ofma):
+        x.run(steps, "MRT")
     fl = o.cell_type() == 0
     assert rel_l2(g.temperature_data()[fl], o.temperature()[fl]) <= FIELD_TOL
     assert rel_l2(g.rho_data()[fl], o.rho()[fl]) <= FIELD_TOL
     vel = lambda a, b: np.concatenate([a[fl], b[fl]])
-    check_fp32_parity(vel(g.ux_data(), g.uy_data()), vel(o.ux(), o.uy()), vel(o64.ux(), o64.uy()), "cavity u")
+    # the convection onset amplifies rounding: the two FP32 oracle builds
+    # (separate and fused multiply-adds) end 1.9e-5 apart in u at 2000 steps
+    check_fp32_parity(vel(g.ux_data(), g.uy_data()), vel(o.ux(), o.uy()), vel(o64.ux(), o64.uy()), "cavity u",
+                      o32fma=vel(ofma.ux(), ofma.uy()))
 
 
 def test_thermal_conduction_fixed_point_on_gpu(wa):

@@ -55,6 +55,7 @@ _CT = {"p": P, "i": I, "f": F}
 
 # Part 2: device-resident domains
 DOMAIN_SIGS = {
+    "wake_domain_bytes_needed": ([I, I, I, I, I, I, LL, I], LL),
     "wake_domain_create": ([I, I, I, I, I, F, I, LL, P], I),
     "wake_domain_destroy": ([P], None),
     "wake_domain_initialize": ([P, F, F, F, F], I),
@@ -107,6 +108,8 @@ BATCH_SIGS = {
     "wake_batch_add": ([P, P, LL, F, F, F, F, I, F, F, F, F, P], I),
     "wake_batch_remove": ([P, I], I),
     "wake_batch_run": ([P, LL, I, P], I),
+    "wake_batch_launch": ([P, LL, I], I),
+    "wake_batch_wait": ([P, P], I),
     "wake_batch_remaining": ([P, I, P], I),
     "wake_batch_forces": ([P, I, P, LL, P], I),
     "wake_batch_tune": ([P, I, P, P], I),

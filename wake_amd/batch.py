@@ -56,6 +56,16 @@ class Batch:
                    "wake_batch_run")
         return ms.value
 
+    def launch(self, steps, use_graph=True):
+        """Enqueue one chunk (min(steps, chunk) launches) and return at once."""
+        _lib.check(self._L.wake_batch_launch(self._h, int(steps), int(bool(use_graph))), "wake_batch_launch")
+
+    def wait(self):
+        """Complete the chunk in flight; returns its device time in ms."""
+        ms = ctypes.c_float()
+        _lib.check(self._L.wake_batch_wait(self._h, ctypes.byref(ms)), "wake_batch_wait")
+        return ms.value
+
     def remaining(self, slot):
         r = ctypes.c_longlong()
         _lib.check(self._L.wake_batch_remaining(self._h, int(slot), ctypes.byref(r)), "wake_batch_remaining")

@@ -47,6 +47,13 @@ class Batch {
 
   ~Batch() {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    if (pending_) {
+      // complete the bookkeeping so member domains read as stepped
+      try {
+        wait();
+      } catch (...) {
+      }
+    }
     drop_graph();
     if (d_slots_) (void)hipFree(d_slots_);
     if (d_tiles_) (void)hipFree(d_tiles_);
@@ -59,6 +66,7 @@ class Batch {
 
   int add(Domain* d, uint64_t steps, float gx, float gy, float gz, float smag, bool forces, float cx, float cy,
           float cz, float r_max) {
+    if (pending_) throw HipError("a chunk of this batch is in flight: wait first");
     if (!d) throw HipError("null domain");
     if (d->lattice() != lattice_ || d->storage() != storage_)
       throw HipError("domain lattice or storage differs from the batch's");
@@ -87,6 +95,7 @@ class Batch {
   }
 
   void remove(int id) {
+    if (pending_) throw HipError("a chunk of this batch is in flight: wait first");
     auto it = find(id);
     members_.erase(it);
     layout_dirty_ = true;
@@ -118,6 +127,7 @@ class Batch {
   // `steps` launches (each domain takes min(steps, its remaining steps));
   // returns the device time in ms between the first and the last launch.
   float run(uint64_t steps, bool use_graph = true) {
+    if (pending_) throw HipError("a chunk of this batch is in flight: wait first");
     prepare();
     if (members_.empty() || steps == 0) return 0.0f;
     check(hipEventRecord(e0_, stream_), "event");
@@ -141,9 +151,44 @@ class Batch {
     return ms;
   }
 
+  // One chunk (min(steps, chunk) launches) enqueued on the batch's stream;
+  // wait() completes it.  Batches on their own streams overlap on the GPU.
+  void launch(uint64_t steps, bool use_graph = true) {
+    if (pending_) throw HipError("a chunk of this batch is already in flight");
+    prepare();
+    pending_k_ = 0;
+    if (members_.empty() || steps == 0 || !any_active()) return;
+    const int k = static_cast<int>(std::min<uint64_t>(steps, static_cast<uint64_t>(chunk_)));
+    check(hipEventRecord(e0_, stream_), "event");
+    if (use_graph && k == chunk_) {
+      if (!graph_exec_) capture();
+      check(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch");
+    } else {
+      enqueue_chunk(k);
+    }
+    enqueue_hist_copy();
+    check(hipEventRecord(e1_, stream_), "event");
+    pending_ = true;
+    pending_k_ = k;
+  }
+
+  float wait() {
+    if (!pending_) return 0.0f;
+    pending_ = false;
+    check(hipEventSynchronize(e1_), "event");
+    float ms = 0.0f;
+    check(hipEventElapsedTime(&ms, e0_, e1_), "event");
+    host_collect(pending_k_);
+    finish_run();
+    return ms;
+  }
+
+  bool pending() const { return pending_; }
+
   // Times `probe` steps under each candidate shape and keeps the fastest;
   // the choice is cached per (kind, size class) for later batches.
   LaunchShape tune(int probe) {
+    if (pending_) throw HipError("a chunk of this batch is in flight: wait first");
     prepare();
     const auto key = std::make_tuple(lattice_ + (thermal_ ? 100 : 0), storage_, collision_, les_,
                                      static_cast<int>(std::log2(static_cast<double>(std::max<uint64_t>(nodes(), 1)))),
@@ -326,14 +371,22 @@ class Batch {
 
   // Host bookkeeping of a chunk of k launches (mirrors k_advance_batch) and
   // the force history.
+  void enqueue_hist_copy() {
+    if (!any_forces_) return;
+    check(hipMemcpyAsync(h_hist_, d_hist_, members_.size() * chunk_ * 3 * sizeof(double), hipMemcpyDeviceToHost,
+                         stream_),
+          "forces");
+  }
+
   void collect(int k) {
+    enqueue_hist_copy();
+    if (any_forces_) check(hipStreamSynchronize(stream_), "forces");
+    host_collect(k);
+  }
+
+  // after the chunk (and its force copy) completed
+  void host_collect(int k) {
     const int n = static_cast<int>(members_.size());
-    if (any_forces_) {
-      check(hipMemcpyAsync(h_hist_, d_hist_, static_cast<size_t>(n) * chunk_ * 3 * sizeof(double),
-                           hipMemcpyDeviceToHost, stream_),
-            "forces");
-      check(hipStreamSynchronize(stream_), "forces");
-    }
     for (int s = 0; s < n; ++s) {
       Member& m = members_[s];
       const uint64_t left = m.target > m.step ? m.target - m.step : 0;
@@ -369,6 +422,8 @@ class Batch {
   LaunchShape shape_;
   bool layout_dirty_ = true;
   bool any_forces_ = false;
+  bool pending_ = false;
+  int pending_k_ = 0;
   BatchSlot* d_slots_ = nullptr;
   int slot_cap_ = 0;
   int* d_tiles_ = nullptr;
@@ -438,6 +493,20 @@ int wake_batch_run(wake_batch* b, long long steps, int use_graph, float* ms) {
   return bguard(b, [&](wake_hip::Batch& B) {
     if (steps < 0) throw wake_hip::HipError("negative step count");
     const float t = B.run(static_cast<uint64_t>(steps), use_graph != 0);
+    if (ms) *ms = t;
+  });
+}
+
+int wake_batch_launch(wake_batch* b, long long steps, int use_graph) {
+  return bguard(b, [&](wake_hip::Batch& B) {
+    if (steps < 0) throw wake_hip::HipError("negative step count");
+    B.launch(static_cast<uint64_t>(steps), use_graph != 0);
+  });
+}
+
+int wake_batch_wait(wake_batch* b, float* ms) {
+  return bguard(b, [&](wake_hip::Batch& B) {
+    const float t = B.wait();
     if (ms) *ms = t;
   });
 }

@@ -67,6 +67,12 @@ void wake_hip_d3q19_minv(float out[361]) {
     for (int k = 0; k < 19; ++k) out[q * 19 + k] = wake_hip::kD3Q19Minv.minv[q][k];
 }
 
+long long wake_domain_bytes_needed(int dim, int lattice, int nx, int ny, int nz, int storage, long long plane_pad,
+                                   int thermal) {
+  return static_cast<long long>(
+      wake_hip::Domain::bytes_needed(dim, lattice, nx, ny, nz, storage, plane_pad, thermal != 0));
+}
+
 int wake_domain_create(int dim, int lattice, int nx, int ny, int nz, float tau, int storage, long long plane_pad,
                        wake_domain** out) {
   if (!out) return fail("wake: null output handle");

@@ -1,0 +1,283 @@
+"""Device-resident serving loop: continuous batching of simulation requests.
+
+Requests (3D bluff-body runs, 2D thermal cavities) arrive over time.  Each
+admitted request becomes a Domain whose populations stay in HBM until its
+last step; requests of one kind (lattice, population storage, collision,
+LES, temperature coupling) share a Batch that advances all of them with one
+launch per step.  Every scheduling tick launches one chunk on every
+non-empty batch (each on its own stream, so the kinds overlap on the GPU),
+waits, retires the requests that reached their step count, frees their
+memory, and admits queued requests in arrival order while the HBM budget
+allows (first fit: a request that does not fit waits; later, smaller ones
+may still enter).  A request therefore joins the running mix at the next
+chunk boundary after memory frees up.
+
+Results do not depend on the mix: a domain's arithmetic is the same alone
+and batched (tests/test_gpu_serve.py checks that).
+
+`plain_loop` is the reference-style baseline: one request at a time, in
+arrival order, FP32 populations, the single-domain step.
+"""
+import math
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .analysis import aerodynamic_coefficients
+from .batch import Batch
+from .grid import LatticeGrid2D, LatticeGrid3D, Storage
+
+
+@dataclass
+class SimRequest:
+    rid: int
+    kind: str                 # "bluff3d" or "cavity2d"
+    n: int                    # grid edge: n^3 (3D) or n^2 (2D)
+    steps: int
+    re: float = 0.0           # bluff3d: Reynolds number on U and 2 x the largest semi-axis
+    ra: float = 0.0           # cavity2d: Rayleigh number (Pr 0.71)
+    semi_axes: tuple = (8.0, 8.0, 8.0)
+    axes: np.ndarray = None   # rows = body axes (bluff3d)
+    tol: float = 1e-5         # allowed rel L2 error vs FP64: >= 1e-3 admits FP16 storage
+    arrival: float = 0.0      # seconds after the start of the stream
+    # filled in by the server
+    admitted: float = None
+    finished: float = None
+    result: dict = field(default_factory=dict)
+
+    @property
+    def storage(self):
+        return Storage.F16 if self.tol >= 1e-3 else Storage.F32
+
+
+U_BLUFF = 0.05
+CS_LES = 0.17
+PR = 0.71
+U_FREEFALL = 0.1
+
+
+def bluff_geometry(req):
+    n = req.n
+    c = (n / 3.0, n / 2.0, n / 2.0)
+    rmax = max(req.semi_axes)
+    nu = U_BLUFF * 2.0 * rmax / req.re
+    return c, rmax, 3.0 * nu + 0.5
+
+
+def make_domain(req, storage=None):
+    """Grid for a request (storage: None = the request's own choice)."""
+    st = req.storage if storage is None else storage
+    if req.kind == "bluff3d":
+        c, rmax, tau = bluff_geometry(req)
+        n = req.n
+        g = LatticeGrid3D(n, n, n, tau, storage=st)
+        g.set_periodic_axes(True, True)
+        g.initialize(1.0, U_BLUFF, 0.0, 0.0)
+        g.set_les_coupled(True)
+        g.set_inflow_west(U_BLUFF, 0.0)
+        g.set_convective_outflow_east(U_BLUFF)
+        g.set_solid_ellipsoid(*c, *req.semi_axes, req.axes)
+        return g
+    n = req.n
+    H = n - 2
+    nu = U_FREEFALL * H * math.sqrt(PR / req.ra)
+    alpha = nu / PR
+    g = LatticeGrid2D(n, n, 3.0 * nu + 0.5, storage=st)
+    g.initialize(1.0, 0.0, 0.0)
+    g.set_solid_box(0, 0, n - 1, 0)
+    g.set_solid_box(0, n - 1, n - 1, n - 1)
+    g.set_solid_box(0, 0, 0, n - 1)
+    g.set_solid_box(n - 1, 0, n - 1, n - 1)
+    g.enable_thermal(3.0 * alpha + 0.5, U_FREEFALL ** 2 / H, 0.5, 0.5)
+    g.set_wall_temperature(0, 0, 0, n - 1, 1.0)
+    g.set_wall_temperature(n - 1, 0, n - 1, n - 1, 0.0)
+    return g
+
+
+def batch_key(req, storage=None):
+    st = int(req.storage if storage is None else storage)
+    if req.kind == "bluff3d":
+        return (19, st, "MRT", True, False)
+    return (9, st, "MRT", False, True)
+
+
+def step_kwargs(req):
+    return {"use_les": True, "smag": CS_LES} if req.kind == "bluff3d" else {}
+
+
+def nusselt(T, ux, n, alpha):
+    T = T.reshape(n, n).astype(np.float64)
+    ux = ux.reshape(n, n).astype(np.float64)
+    H = n - 2
+    rows = slice(1, n - 1)
+    q = [(0.5 * (ux[rows, x] * T[rows, x] + ux[rows, x + 1] * T[rows, x + 1])
+          - alpha * (T[rows, x + 1] - T[rows, x])).sum() / H for x in range(1, n - 2)]
+    return float(np.mean(q)) * H / alpha
+
+
+def result_of(req, g, force=None):
+    """bluff3d: C_D from the momentum-exchange force on the final state
+    (equivalent-sphere area); cavity2d: mean Nusselt number."""
+    if req.kind == "bluff3d":
+        c, rmax, _ = bluff_geometry(req)
+        F = force if force is not None else g.momentum_exchange(*c, 1.5 * rmax)
+        r_eq = float(np.prod(req.semi_axes)) ** (1.0 / 3.0)
+        co = aerodynamic_coefficients(F[0], F[1], F[2], 0.0, 1.0, U_BLUFF, math.pi * r_eq * r_eq)
+        return {"C_D": float(co["cd"]), "C_L": float(co["cl"])}
+    n = req.n
+    nu = U_FREEFALL * (n - 2) * math.sqrt(PR / req.ra)
+    return {"Nu": nusselt(g.temperature_data(), g.ux_data(), n, nu / PR)}
+
+
+def bytes_needed(req, storage=None):
+    """Device bytes of the request's domain (wake_domain_bytes_needed)."""
+    from . import _lib
+
+    st = int(req.storage if storage is None else storage)
+    if req.kind == "bluff3d":
+        return _lib.lib().wake_domain_bytes_needed(3, 19, req.n, req.n, req.n, st, -1, 0)
+    return _lib.lib().wake_domain_bytes_needed(2, 9, req.n, req.n, 1, st, -1, 1)
+
+
+class Server:
+    """Continuous-batching request loop under an HBM admission budget."""
+
+    def __init__(self, budget_bytes=200e9, chunk=100, use_graph=True, tune=True):
+        self.budget = budget_bytes
+        self.chunk = chunk
+        self.use_graph = use_graph
+        self.tune = tune
+        self.batches = {}
+        self.active = {}        # rid -> (req, grid, key, slot)
+        self.queue = deque()
+        self.used = 0
+        self.done = []
+        self.peak_used = 0
+        self.ticks = 0
+        self.node_steps = 0
+
+    def submit(self, req):
+        self.queue.append(req)
+
+    def _admit(self, now):
+        kept = deque()
+        while self.queue:
+            req = self.queue.popleft()
+            need = bytes_needed(req)
+            if self.used + need > self.budget:
+                kept.append(req)
+                continue
+            g = make_domain(req)
+            key = batch_key(req)
+            b = self.batches.get(key)
+            if b is None:
+                lat, st, coll, les, thermal = key
+                b = Batch(lattice=lat, storage=st, collision=coll, use_les=les, thermal=thermal, chunk=self.chunk)
+                self.batches[key] = b
+            slot = b.add(g, req.steps, **({"smag": CS_LES} if req.kind == "bluff3d" else {}))
+            self.active[req.rid] = (req, g, key, slot)
+            self.used += need
+            self.peak_used = max(self.peak_used, self.used)
+            req.admitted = now
+        self.queue = kept
+
+    def tick(self, now_fn):
+        """Admit, run one chunk on every batch (overlapped), retire."""
+        self._admit(now_fn())
+        live = [b for b in self.batches.values() if b.info()["members"] > 0]
+        if not live:
+            return False
+        for b in live:
+            b.launch(self.chunk, self.use_graph)
+        for b in live:
+            b.wait()
+        self.ticks += 1
+        t = now_fn()
+        for rid in list(self.active):
+            req, g, key, slot = self.active[rid]
+            b = self.batches[key]
+            if b.remaining(slot) == 0:
+                req.result = result_of(req, g)
+                req.finished = t
+                b.remove(slot)
+                self.used -= bytes_needed(req)
+                self.node_steps += g.total_nodes() * req.steps
+                del self.active[rid]
+                self.done.append(req)
+        return True
+
+    def serve(self, requests, realtime=True):
+        """Replay `requests` (sorted by arrival) against the wall clock."""
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0
+        while pending or self.queue or self.active:
+            while pending and (not realtime or pending[0].arrival <= now()):
+                self.submit(pending.popleft())
+            if not self.tick(now) and pending:
+                wait = pending[0].arrival - now()
+                if wait > 0:
+                    time.sleep(wait)
+        return now()
+
+
+def plain_loop(requests, realtime=True):
+    """One request at a time in arrival order, FP32, single-domain step."""
+    t0 = time.perf_counter()
+    now = lambda: time.perf_counter() - t0
+    out = []
+    for req in sorted(requests, key=lambda r: r.arrival):
+        if realtime and req.arrival > now():
+            time.sleep(req.arrival - now())
+        r = SimRequest(**{k: getattr(req, k) for k in ("rid", "kind", "n", "steps", "re", "ra", "semi_axes", "axes",
+                                                         "tol", "arrival")})
+        r.admitted = now()
+        g = make_domain(r, storage=Storage.F32)
+        g.run(r.steps, "MRT", **step_kwargs(r))
+        r.result = result_of(r, g)
+        r.finished = now()
+        out.append(r)
+        del g
+    return out, now()
+
+
+def random_rotation(rng):
+    q, r = np.linalg.qr(rng.standard_normal((3, 3)))
+    q = q * np.sign(np.diag(r))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q.astype(np.float32)
+
+
+def request_stream(count, rate=20.0, seed=5, step_scale=1.0, n3=(64, 96, 128, 160, 192),
+                   n2=(128, 192, 256, 320, 384, 448, 512), p_fp16=0.7):
+    """Poisson arrivals (mean `rate` per second): 60% D3Q19 MRT-LES bluff
+    bodies (rotated ellipsoid, semi-axes U[8, min(32, n/6)], n^3 grid,
+    Re U[100, 1000]), 40% D2Q9 + D2Q5 cavities (n^2, Ra log-U[1e4, 1e6]);
+    steps U[2000, 20000] x step_scale; error budget 1e-3 (FP16 storage) with
+    probability p_fp16, else 1e-5."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    reqs = []
+    for i in range(count):
+        t += rng.exponential(1.0 / rate)
+        steps = max(10, int(rng.integers(2000, 20001) * step_scale))
+        tol = 1e-3 if rng.random() < p_fp16 else 1e-5
+        if rng.random() < 0.6:
+            n = int(rng.choice(n3))
+            hi = min(32.0, n / 6.0)
+            ax = tuple(float(v) for v in rng.uniform(8.0, hi, 3))
+            reqs.append(SimRequest(i, "bluff3d", n, steps, re=float(rng.uniform(100, 1000)), semi_axes=ax,
+                                   axes=random_rotation(rng), tol=tol, arrival=t))
+        else:
+            n = int(rng.choice(n2))
+            reqs.append(SimRequest(i, "cavity2d", n, steps, ra=float(10 ** rng.uniform(4, 6)), tol=tol, arrival=t))
+    return reqs
+
+
+def latency_stats(done):
+    ttr = np.array([r.finished - r.arrival for r in done])
+    return {"p50_s": float(np.percentile(ttr, 50)), "p99_s": float(np.percentile(ttr, 99)),
+            "mean_s": float(ttr.mean()), "max_s": float(ttr.max())}
