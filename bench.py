@@ -14,6 +14,9 @@ one-domain-at-a-time path.  `--workload serve` replays a Poisson stream of
 mixed 3D bluff-body and 2D thermal-cavity requests through the
 continuous-batching serving loop and then through the plain sequential
 loop, and reports simulations per second and p50/p99 time-to-result.
+`--workload wing` times the D3Q27 NACA-0012 wing channel (384x192x192)
+with FP32/FP16/BF16 storage and states each storage's error against the
+FP64 oracle on the same flow at reduced size.
 
 Prints ONE JSON line.  The roofline entry divides the algorithmic bytes of
 a step (one read and one write of every population: 152 B/node for D3Q19
@@ -278,10 +281,106 @@ def main_serve(args):
     print(json.dumps(out), flush=True)
 
 
+def wing_setup(g, nx, ny, chord, u=0.04, aoa=8.0):
+    """D3Q27 TRT + coupled LES channel: walls at y = 0, ny - 1 (halfway
+    bounce-back at the domain edges), periodic z, Zou-He inlet u, convective
+    outlet, NACA 0012 extruded over the full span, leading edge at
+    (nx/4, ny/2), angle of attack `aoa`."""
+    g.set_periodic_axes(False, True)
+    g.initialize(1.0, u, 0.0, 0.0)
+    g.set_les_coupled(True)
+    g.set_inflow_west(u, 0.0)
+    g.set_convective_outflow_east(u)
+    g.set_solid_naca(12, chord, aoa, nx / 4.0, ny / 2.0)
+    return g
+
+
+def main_wing(args):
+    """NACA 0012 wing at 8 deg, D3Q27 TRT + LES, Re 2000 on the chord: the
+    full 384x192x192 domain timed with FP32, FP16 and BF16 population
+    storage, and the same flow at 1/scale size run with each storage and with
+    the FP64 oracle to state each storage's error (velocity field, C_D, C_L,
+    Strouhal number of the lift history)."""
+    import numpy as np
+
+    import wake_amd
+    from wake_amd import analysis
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+
+    u, re = 0.04, 2000.0
+    names = {0: "fp32", 1: "fp16", 2: "bf16"}
+    perf = {}
+    nx, ny, nz, chord = 384, 192, 192, 96.0
+    tau = 3.0 * u * chord / re + 0.5
+    for st in (0, 1, 2):
+        g = wing_setup(wake_amd.LatticeGrid3D(nx, ny, nz, tau, lattice=27, storage=st), nx, ny, chord)
+        g.run(10, "TRT", use_les=True, smag=0.17)
+        ms = g.run(args.steps, "TRT", use_les=True, smag=0.17, timed=True)
+        perf[names[st]] = {"MLUPS": round(nx * ny * nz * args.steps / (ms * 1e-3) / 1e6, 1),
+                           "kernel_us": round(g.kernel_time() * 1e3, 1)}
+        log(f"full {nx}x{ny}x{nz} {names[st]}: {perf[names[st]]}")
+        del g
+    # error study at reduced size
+    k = args.wing_scale
+    sx, sy, sz, sc = nx // k, ny // k, nz // k, chord / k
+    stau = 3.0 * u * sc / re + 0.5
+    steps, every = args.wing_steps, 5
+    area = 0.5 * u * u * sc * sz
+
+    def coeffs(F):
+        return F[:, 0] / area, F[:, 1] / area
+
+    res = {}
+    fl = None
+    for st in (0, 1, 2):
+        g = wing_setup(wake_amd.LatticeGrid3D(sx, sy, sz, stau, lattice=27, storage=st), sx, sy, sc)
+        b = wake_amd.Batch(lattice=27, storage=st, collision="TRT", use_les=True, chunk=100)
+        b.add(g, steps, smag=0.17, forces=(0.0, 0.0, 0.0, 0.0))
+        b.run(steps)
+        F = b.forces(0)[every - 1::every]
+        fl = g.cell_type_data() == 0
+        res[names[st]] = (np.concatenate([g.ux_data()[fl], g.uy_data()[fl], g.uz_data()[fl]]), *coeffs(F))
+        del b, g
+    log(f"reduced {sx}x{sy}x{sz}: GPU runs done; FP64 oracle {steps} steps on the host")
+    t0 = time.perf_counter()
+    o = wing_setup(ob.OracleGrid(3, sx, sy, sz, tau=stau, lattice=27, bits=64), sx, sy, sc)
+    hist = []
+    for i in range(steps // every):
+        o.run(every, "TRT", use_les=True, smag=0.17)
+        hist.append(o.momentum_exchange(0.0, 0.0, 0.0, 0.0))
+    log(f"FP64 oracle done in {time.perf_counter() - t0:.0f} s")
+    ref_u = np.concatenate([o.ux()[fl], o.uy()[fl], o.uz()[fl]])
+    ref_cd, ref_cl = coeffs(np.array(hist))
+    half = len(ref_cd) // 2
+    st_of = lambda cl: analysis.strouhal_number(
+        analysis.analyze_frequency(cl[half:].astype(np.float32), float(every)), sc, u)
+    ref = {"C_D": float(np.mean(ref_cd[half:])), "C_L": float(np.mean(ref_cl[half:])), "St": st_of(ref_cl)}
+    errors = {}
+    for nm, (uvec, cd, cl) in res.items():
+        v = {"C_D": float(np.mean(cd[half:])), "C_L": float(np.mean(cl[half:])), "St": st_of(cl)}
+        errors[nm] = {"velocity_rel_l2": float(np.linalg.norm(uvec - ref_u) / np.linalg.norm(ref_u)),
+                      **{f"{q}_rel": (abs(v[q] / ref[q] - 1) if ref[q] else None) for q in v}, **v}
+        log(f"{nm} vs FP64: {errors[nm]}")
+    out = {
+        "metric": METRIC, "value": perf["fp32"]["MLUPS"], "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": 10, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 (and fp16/bf16 storage)", "data": "synthetic",
+        "config": {"workload": f"D3Q27 TRT(Lambda=1/4)+LES channel, NACA 0012 extruded at 8 deg, chord {chord:g}, "
+                               f"{nx}x{ny}x{nz}, inlet u=0.04, Re=2000",
+                   "storage_perf": perf,
+                   "error_study": {"size": [sx, sy, sz], "chord": sc, "steps": steps, "fp64_reference": ref,
+                                   "errors": errors},
+                   "parallelism": "single GPU"},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["sphere", "batched-wakes", "serve"], default="sphere")
+    ap.add_argument("--workload", choices=["sphere", "batched-wakes", "serve", "wing"], default="sphere")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=256)
@@ -296,6 +395,8 @@ def main():
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--step-scale", type=float, default=1.0)
     ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--wing-scale", type=int, default=4)
+    ap.add_argument("--wing-steps", type=int, default=5000)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -303,6 +404,11 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "wing":
+        if args.steps is None:
+            args.steps = 500
+        main_wing(args)
+        return
     if args.workload == "serve":
         main_serve(args)
         return

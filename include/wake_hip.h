@@ -193,6 +193,13 @@ int wake_domain_set_solid_sphere(wake_domain* d, float cx, float cy, float cz, f
  * counterpart in the reference). */
 int wake_domain_set_solid_ellipsoid(wake_domain* d, float cx, float cy, float cz, float rx, float ry, float rz,
                                     const float* axes);
+/* src/geometry/naca_airfoil.cpp:113-190: NACA 4-digit airfoil (digits as an
+ * integer, e.g. 12 for "0012"; camber, camber position and thickness as in
+ * NacaAirfoil::configure, :23-41), chord and angle of attack in degrees,
+ * leading edge at (lead_x, lead_y); rasterize (2D) or voxelize_extruded over
+ * z0..z1 (3D). */
+int wake_domain_set_solid_naca(wake_domain* d, int digits, float chord, float aoa_deg, float lead_x, float lead_y,
+                               int z0, int z1);
 int wake_domain_set_solid_circle(wake_domain* d, float cx, float cy, float r);
 int wake_domain_set_solid_cylinder_z(wake_domain* d, float cx, float cy, float r);
 int wake_domain_set_moving_wall_top(wake_domain* d, float u_wall_x, float u_wall_z);

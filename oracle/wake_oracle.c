@@ -23,8 +23,14 @@
 
 #if defined(ORACLE_DOUBLE)
 #define RSQRT(x) sqrt(x)
+#define RCOS(x) cos(x)
+#define RSIN(x) sin(x)
+#define RABS(x) fabs(x)
 #else
 #define RSQRT(x) sqrtf(x)
+#define RCOS(x) cosf(x)
+#define RSIN(x) sinf(x)
+#define RABS(x) fabsf(x)
 #endif
 
 #define R(x) ((real)(x))
@@ -1371,6 +1377,52 @@ void oracle_grid_set_solid_ellipsoid(oracle_grid* g, real cx, real cy, real cz, 
         }
         if (dx * dx * irx + dy * dy * iry + dz * dz * irz <= R(1)) oracle_grid_set_cell_type(g, x, y, z, CELL_SOLID);
       }
+}
+
+/* naca_airfoil.cpp:23-41 (digits) and :113-150 (contains_point). */
+static int naca_contains(real camber, real camber_pos, real thickness, real chord, real aoa_deg, real x, real y,
+                         real lead_x, real lead_y) {
+  const real pi = R(3.14159265358979323846);
+  real dx = x - lead_x;
+  real dy = y - lead_y;
+  real alpha_rad = -aoa_deg * (pi / R(180));
+  real cos_a = RCOS(-alpha_rad);
+  real sin_a = RSIN(-alpha_rad);
+  real x_chord = dx * cos_a - dy * sin_a;
+  real y_chord = dx * sin_a + dy * cos_a;
+  if (x_chord < R(0) || x_chord > chord) return 0;
+  real xc = x_chord / chord;
+  real yt = R(5) * thickness *
+            (R(0.2969) * RSQRT(xc) - R(0.1260) * xc - R(0.3516) * xc * xc + R(0.2843) * xc * xc * xc -
+             R(0.1036) * xc * xc * xc * xc) *
+            chord;
+  real yc = R(0);
+  if (camber_pos > R(1e-6)) {
+    if (xc < camber_pos) {
+      yc = (camber / (camber_pos * camber_pos)) * (R(2) * camber_pos * xc - xc * xc) * chord;
+    } else {
+      real one_minus_p = R(1) - camber_pos;
+      yc = (camber / (one_minus_p * one_minus_p)) * ((R(1) - R(2) * camber_pos) + R(2) * camber_pos * xc - xc * xc) *
+           chord;
+    }
+  }
+  return RABS(y_chord - yc) <= yt;
+}
+
+/* naca_airfoil.cpp:152-190: rasterize (2D) / voxelize_extruded over z0..z1. */
+void oracle_grid_set_solid_naca(oracle_grid* g, int digits, real chord, real aoa_deg, real lead_x, real lead_y,
+                                int z0, int z1) {
+  real camber = (real)(digits / 1000) / R(100), camber_pos = (real)((digits / 100) % 10) / R(10);
+  real thickness = (real)(digits % 100) / R(100);
+  real rad_bound = chord * R(1.5);
+  int min_x = imax(0, (int)(lead_x - rad_bound)), max_x = imin(g->nx - 1, (int)(lead_x + rad_bound));
+  int min_y = imax(0, (int)(lead_y - rad_bound)), max_y = imin(g->ny - 1, (int)(lead_y + rad_bound));
+  int za = g->dim == 2 ? 0 : imax(0, z0), zb = g->dim == 2 ? 0 : imin(g->nz - 1, z1);
+  g->is_periodic = 0;
+  for (int y = min_y; y <= max_y; ++y)
+    for (int x = min_x; x <= max_x; ++x)
+      if (naca_contains(camber, camber_pos, thickness, chord, aoa_deg, (real)x, (real)y, lead_x, lead_y))
+        for (int z = za; z <= zb; ++z) oracle_grid_set_cell_type(g, x, y, z, CELL_SOLID);
 }
 
 /* lattice_grid_2d.cpp:143-159 */

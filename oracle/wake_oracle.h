@@ -132,6 +132,8 @@ void oracle_grid_set_periodic(oracle_grid* g, int periodic);
 void oracle_grid_set_periodic_axes(oracle_grid* g, int per_y, int per_z);
 void oracle_grid_set_solid_box(oracle_grid* g, int x0, int y0, int z0, int x1, int y1, int z1);
 void oracle_grid_set_solid_sphere(oracle_grid* g, real cx, real cy, real cz, real r);
+void oracle_grid_set_solid_naca(oracle_grid* g, int digits, real chord, real aoa_deg, real lead_x, real lead_y,
+                                int z0, int z1);
 /* EXTENSION: D2Q5 temperature coupled to the D2Q9 flow (Boussinesq). */
 void oracle_grid_enable_thermal(oracle_grid* g, real tau_g, real gbeta, real t_ref, real t_init);
 void oracle_grid_set_temperature_field(oracle_grid* g, const real* t);

@@ -43,6 +43,7 @@ def lib(bits=32):
     L.oracle_grid_set_solid_box.argtypes = [P, I, I, I, I, I, I]
     L.oracle_grid_set_solid_sphere.argtypes = [P, real, real, real, real]
     L.oracle_grid_set_solid_ellipsoid.argtypes = [P, real, real, real, real, real, real, P]
+    L.oracle_grid_set_solid_naca.argtypes = [P, I, real, real, real, real, I, I]
     L.oracle_grid_enable_thermal.argtypes = [P, real, real, real, real]
     L.oracle_grid_set_temperature_field.argtypes = [P, P]
     L.oracle_grid_set_wall_temperature.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, real]
@@ -198,6 +199,10 @@ class OracleGrid:
 
     def temperature(self):
         return self._arr("temperature", self.N)
+
+    def set_solid_naca(self, code, chord, aoa_deg, lead_x, lead_y, z_start=0, z_end=None):
+        self.L.oracle_grid_set_solid_naca(self.h, int(code), chord, aoa_deg, lead_x, lead_y, z_start,
+                                          self.nz - 1 if z_end is None else z_end)
 
     def set_solid_ellipsoid(self, cx, cy, cz, rx, ry, rz, axes=None):
         a = None

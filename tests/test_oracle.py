@@ -631,3 +631,31 @@ def test_ellipsoid_voxelizer_axes_conventions(oracle32):
     ys = lambda f: np.nonzero((f == 1).any(axis=(0, 2)))[0]
     zs = lambda f: np.nonzero((f == 1).any(axis=(1, 2)))[0]
     assert np.array_equal(zs(rot), zs(base)) and len(ys(rot)) > len(ys(base))
+
+
+def test_naca_voxelizer_shape(oracle32):
+    """naca_airfoil.cpp:113-190 restated: NACA 0012 at 0 deg is symmetric
+    about its chord line, its section area is 0.685 t c^2 (the 4-digit
+    thickness polynomial integrates to 0.6851 t), its thickest section spans
+    0.12 c, and a positive angle of attack puts the trailing edge below the
+    leading edge."""
+    c, lx, ly = 96.0, 40.0, 64.0
+    o = ob.OracleGrid(2, 192, 128, bits=32)
+    o.set_solid_naca(12, c, 0.0, lx, ly)
+    f = o.cell_type().reshape(128, 192)
+    assert np.array_equal(f[64 + 1:64 + 20], f[64 - 1:64 - 20:-1])
+    area = (f == 1).sum()
+    assert abs(area / (0.6851 * 0.12 * c * c) - 1) < 0.03
+    col = int(lx + 0.3 * c)
+    assert abs((f[:, col] == 1).sum() - 0.12 * c) <= 2
+    o8 = ob.OracleGrid(2, 192, 128, bits=32)
+    o8.set_solid_naca(12, c, 8.0, lx, ly)
+    g = o8.cell_type().reshape(128, 192)
+    assert abs((g == 1).sum() / area - 1) < 0.03
+    ys = lambda x: np.nonzero(g[:, x] == 1)[0].mean()
+    assert ys(int(lx + 0.9 * c)) < ys(int(lx + 0.1 * c)) - 0.5 * 0.8 * c * np.sin(np.radians(8))
+    # extruded: every z slab equals the 2D section
+    o3 = ob.OracleGrid(3, 192, 128, 6, bits=32)
+    o3.set_solid_naca(12, c, 8.0, lx, ly)
+    h = o3.cell_type().reshape(6, 128, 192)
+    assert all(np.array_equal(h[z], g) for z in range(6))

@@ -64,8 +64,8 @@ __global__ __launch_bounds__(256) void k_mem_batch(const BatchSlot* __restrict__
   const int par = ((sl.parity + i) & 1) ^ 1;
   const auto* buf = static_cast<const typename St::T*>(sl.buf[par]);
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  mem_links<L, St, false>(sl.a, buf, cur + 1, sl.mem_node, sl.mem_q, sl.mem_count, sl.mem_cx, sl.mem_cy, s0, s1,
-                          s2);
+  mem_links<L, St, false>(sl.a, buf, cur + 1, sl.mem_node, sl.mem_q, sl.mem_count, threadIdx.x, blockDim.x,
+                          sl.mem_cx, sl.mem_cy, s0, s1, s2);
   block_sum3(s0, s1, s2);
   if (threadIdx.x == 0) {
     double* o = hist + (static_cast<long long>(blockIdx.x) * chunk + i) * 3;

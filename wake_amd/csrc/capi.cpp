@@ -128,6 +128,10 @@ int wake_domain_set_solid_ellipsoid(wake_domain* d, float cx, float cy, float cz
                                     const float* axes) {
   return guard(d, [&](wake_hip::Domain& g) { g.set_solid_ellipsoid(cx, cy, cz, rx, ry, rz, axes); });
 }
+int wake_domain_set_solid_naca(wake_domain* d, int digits, float chord, float aoa_deg, float lead_x, float lead_y,
+                               int z0, int z1) {
+  return guard(d, [&](wake_hip::Domain& g) { g.set_solid_naca(digits, chord, aoa_deg, lead_x, lead_y, z0, z1); });
+}
 int wake_domain_set_solid_circle(wake_domain* d, float cx, float cy, float r) {
   return guard(d, [&](wake_hip::Domain& g) { g.set_solid_circle(cx, cy, r); });
 }

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <limits>
 
 #include "lattice.hpp"
@@ -80,7 +81,7 @@ Domain::Domain(int dim, int lattice, int nx, int ny, int nz, float tau, int stor
   d_flags_ = static_cast<uint8_t*>(alloc(N_));
   d_cls_ = static_cast<uint8_t*>(alloc(N_));
   d_aux_ = static_cast<int*>(alloc(N_ * sizeof(int)));
-  d_prev_ = static_cast<float*>(alloc(static_cast<size_t>(Q_) * ny_ * nz_ * sizeof(float)));
+  d_prev_ = static_cast<float*>(alloc(static_cast<size_t>(prev_stride(Q_)) * ny_ * nz_ * sizeof(float)));
   d_rho_ = static_cast<float*>(alloc(N_ * 4));
   d_ux_ = static_cast<float*>(alloc(N_ * 4));
   d_uy_ = static_cast<float*>(alloc(N_ * 4));
@@ -112,6 +113,19 @@ void* Domain::alloc(size_t bytes) {
   return p;
 }
 
+// The convective outlet store: one record of prev_stride(Q) floats per
+// (y, z) row, value(row, q) for q < Q, zero padding.
+void Domain::upload_prev(const std::function<float(size_t, int)>& value) {
+  const int QP = prev_stride(Q_);
+  const size_t rows = static_cast<size_t>(ny_) * nz_;
+  h_prev_.assign(rows * QP, 0.0f);
+  for (size_t r = 0; r < rows; ++r)
+    for (int q = 0; q < Q_; ++q) h_prev_[r * QP + q] = value(r, q);
+  check(hipMemcpyAsync(d_prev_, h_prev_.data(), h_prev_.size() * sizeof(float), hipMemcpyHostToDevice, stream_),
+        "upload outlet store");
+  check(hipStreamSynchronize(stream_), "upload outlet store");
+}
+
 void Domain::release(void* p) {
   if (!p) return;
   (void)hipFree(p);
@@ -129,6 +143,7 @@ void Domain::initialize(float rho, float ux, float uy, float uz) {
   if (dim_ == 2) uz = 0.0f;
   step_count_ = 0;
   const float u_sq = dim_ == 2 ? ux * ux + uy * uy : ux * ux + uy * uy + uz * uz;
+  std::vector<float> fe_q(Q_);
   for (int q = 0; q < Q_; ++q) {
     const float ci_u = dim_ == 2 ? static_cast<float>(L.cx[q]) * ux + static_cast<float>(L.cy[q]) * uy
                                  : static_cast<float>(L.cx[q]) * ux + static_cast<float>(L.cy[q]) * uy +
@@ -143,12 +158,9 @@ void Domain::initialize(float rho, float ux, float uy, float uz) {
       const uint16_t bits = storage_ == kStoreF16 ? to_half_bits(fe - L.w[q]) : to_bf16_bits(fe - L.w[q]);
       check(hipMemsetD16Async(static_cast<hipDeviceptr_t>(plane), bits, stride_, stream_), "fill");
     }
-    uint32_t pb;
-    std::memcpy(&pb, &fe, 4);
-    check(hipMemsetD32Async(static_cast<hipDeviceptr_t>(d_prev_ + static_cast<size_t>(q) * ny_ * nz_), pb,
-                            static_cast<size_t>(ny_) * nz_, stream_),
-          "fill");
+    fe_q[q] = fe;
   }
+  upload_prev([&](size_t, int q) { return fe_q[q]; });
   s_buf_ = 0;
   a_buf_ = -1;
   std::fill(flags_.begin(), flags_.end(), static_cast<uint8_t>(kFluid));
@@ -190,13 +202,7 @@ void Domain::upload_state_from_host(const std::vector<float>& f_ref) {
   }
   check(hipMemcpyAsync(buf_[0], host.data(), host.size(), hipMemcpyHostToDevice, stream_), "upload");
   // convective outlet store = S_0 on the x = nx - 1 plane
-  std::vector<float> prev(static_cast<size_t>(Q_) * ny_ * nz_);
-  for (int q = 0; q < Q_; ++q)
-    for (int z = 0; z < nz_; ++z)
-      for (int y = 0; y < ny_; ++y)
-        prev[(static_cast<size_t>(q) * nz_ + z) * ny_ + y] =
-            f_ref[static_cast<size_t>(q) * N_ + (static_cast<size_t>(z) * ny_ + y) * nx_ + (nx_ - 1)];
-  check(hipMemcpyAsync(d_prev_, prev.data(), prev.size() * 4, hipMemcpyHostToDevice, stream_), "upload");
+  upload_prev([&](size_t row, int q) { return f_ref[static_cast<size_t>(q) * N_ + row * nx_ + (nx_ - 1)]; });
   check(hipStreamSynchronize(stream_), "upload");
   s_buf_ = 0;
   a_buf_ = -1;
@@ -301,6 +307,63 @@ void Domain::set_solid_box(int x0, int y0, int z0, int x1, int y1, int z1) {
   for (int z = std::max(0, z0); z <= std::min(nz_ - 1, z1); ++z)
     for (int y = std::max(0, y0); y <= std::min(ny_ - 1, y1); ++y)
       for (int x = std::max(0, x0); x <= std::min(nx_ - 1, x1); ++x) set_cell_type(x, y, z, kSolid);
+}
+
+// naca_airfoil.cpp:23-41 (digits -> camber, camber position, thickness) and
+// :113-150 (contains_point), statement for statement in FP32 so the flags are
+// bit-exact: the query point is turned into the unrotated chord frame and
+// tested against the 4-digit thickness and camber lines.
+namespace {
+struct Naca {
+  float camber, camber_pos, thickness, chord, aoa_deg;
+  bool contains(float x, float y, float lead_x, float lead_y) const {
+    const float kPi = 3.14159265358979323846f;
+    const float dx = x - lead_x;
+    const float dy = y - lead_y;
+    const float alpha_rad = -aoa_deg * (kPi / 180.0f);
+    const float cos_a = std::cos(-alpha_rad);
+    const float sin_a = std::sin(-alpha_rad);
+    const float x_chord = dx * cos_a - dy * sin_a;
+    const float y_chord = dx * sin_a + dy * cos_a;
+    if (x_chord < 0.0f || x_chord > chord) return false;
+    const float xc = x_chord / chord;
+    const float yt = 5.0f * thickness *
+                     (0.2969f * std::sqrt(xc) - 0.1260f * xc - 0.3516f * xc * xc + 0.2843f * xc * xc * xc -
+                      0.1036f * xc * xc * xc * xc) *
+                     chord;
+    float yc = 0.0f;
+    if (camber_pos > 1e-6f) {
+      if (xc < camber_pos) {
+        yc = (camber / (camber_pos * camber_pos)) * (2.0f * camber_pos * xc - xc * xc) * chord;
+      } else {
+        const float one_minus_p = 1.0f - camber_pos;
+        yc = (camber / (one_minus_p * one_minus_p)) * ((1.0f - 2.0f * camber_pos) + 2.0f * camber_pos * xc - xc * xc) *
+             chord;
+      }
+    }
+    return std::abs(y_chord - yc) <= yt;
+  }
+};
+}  // namespace
+
+// naca_airfoil.cpp:152-190: rasterize (2D) / voxelize_extruded over z0..z1 (3D).
+void Domain::set_solid_naca(int digits, float chord, float aoa_deg, float lead_x, float lead_y, int z0, int z1) {
+  if (digits < 0 || digits > 9999) throw HipError("NACA designation must be 4 digits");
+  before_config_change();
+  is_periodic_ = false;
+  const int m_digit = digits / 1000, p_digit = (digits / 100) % 10, t_digits = digits % 100;
+  Naca a{static_cast<float>(m_digit) / 100.0f, static_cast<float>(p_digit) / 10.0f,
+         static_cast<float>(t_digits) / 100.0f, chord, aoa_deg};
+  const float rad_bound = chord * 1.5f;
+  const int min_x = std::max(0, static_cast<int>(lead_x - rad_bound));
+  const int max_x = std::min(nx_ - 1, static_cast<int>(lead_x + rad_bound));
+  const int min_y = std::max(0, static_cast<int>(lead_y - rad_bound));
+  const int max_y = std::min(ny_ - 1, static_cast<int>(lead_y + rad_bound));
+  const int za = dim_ == 2 ? 0 : std::max(0, z0), zb = dim_ == 2 ? 0 : std::min(nz_ - 1, z1);
+  for (int y = min_y; y <= max_y; ++y)
+    for (int x = min_x; x <= max_x; ++x)
+      if (a.contains(static_cast<float>(x), static_cast<float>(y), lead_x, lead_y))
+        for (int z = za; z <= zb; ++z) set_cell_type(x, y, z, kSolid);
 }
 
 // primitives.cpp:73-100: FP32 ellipsoid test, bit-exact flags.  With axes
@@ -429,15 +492,9 @@ void Domain::set_convective_outflow_east(float u_conv) {
   u_conv_ = u_conv;
   for (int z = 0; z < nz_; ++z)
     for (int y = 0; y < ny_; ++y) set_cell_type(nx_ - 1, y, z, kOutflow);
-  // the outlet store starts from the current post-stream state
-  const float* f = f_src();
-  std::vector<float> prev(static_cast<size_t>(Q_) * ny_ * nz_);
-  for (int q = 0; q < Q_; ++q)
-    for (int z = 0; z < nz_; ++z)
-      for (int y = 0; y < ny_; ++y)
-        prev[(static_cast<size_t>(q) * nz_ + z) * ny_ + y] =
-            f[static_cast<size_t>(q) * N_ + (static_cast<size_t>(z) * ny_ + y) * nx_ + (nx_ - 1)];
-  check(hipMemcpy(d_prev_, prev.data(), prev.size() * 4, hipMemcpyHostToDevice), "upload");
+  // the outlet store starts from the current post-stream state (S is
+  // materialised by before_config_change)
+  check(launch_prev_from_s(lattice_, storage_, args(false), buf_[s_buf_], stream_), "outlet store");
   cls_dirty_ = true;
 }
 
@@ -496,7 +553,6 @@ void Domain::set_curved_links(const CurvedLinks& links) {
 
 // ------------------------------------------------------------ device sync
 StepArgs Domain::args(bool commit) const {
-  const LatTab L = tab(lattice_);
   StepArgs a;
   std::memset(&a, 0, sizeof(a));
   a.nx = nx_;
@@ -509,13 +565,9 @@ StepArgs Domain::args(bool commit) const {
   a.bfl_per_y = a.per_y;
   a.bfl_per_z = a.per_z;
   a.tau = tau_;
-  const float uwx = has_moving_wall_ ? uwx_ : 0.0f, uwy = 0.0f, uwz = has_moving_wall_ ? uwz_ : 0.0f;
-  for (int q = 0; q < L.Q; ++q) {
-    const float ci_uw = dim_ == 2 ? static_cast<float>(L.cx[q]) * uwx + static_cast<float>(L.cy[q]) * uwy
-                                  : static_cast<float>(L.cx[q]) * uwx + static_cast<float>(L.cy[q]) * uwy +
-                                        static_cast<float>(L.cz[q]) * uwz;
-    a.mom[q] = 2.0f * L.w[q] * 1.0f * 3.0f * ci_uw;
-  }
+  a.uw[0] = has_moving_wall_ ? uwx_ : 0.0f;
+  a.uw[1] = 0.0f;
+  a.uw[2] = has_moving_wall_ && dim_ == 3 ? uwz_ : 0.0f;
   a.inflow = has_inflow_ ? 1 : 0;
   a.inflow_parabolic = inflow_parabolic_ ? 1 : 0;
   a.uin_x = uin_x_;
@@ -878,30 +930,11 @@ const std::pair<std::vector<int>, std::vector<int>>& Domain::mem_link_list(float
 
 void Domain::momentum_exchange(float ccx, float ccy, float ccz, float r_max, double out[3]) {
   sync_device_config();
-  const auto& lists = mem_link_list(ccx, ccy, ccz, r_max);
-  const auto& nodes = lists.first;
-  const auto& qs = lists.second;
-  const int n = static_cast<int>(nodes.size());
-  if (n == 0) {
-    out[0] = out[1] = out[2] = 0.0;
-    return;
-  }
-  if (n > mem_cap_) {
-    if (d_mem_node_) {
-      (void)hipStreamSynchronize(stream_);
-      release(d_mem_node_);
-      release(d_mem_q_);
-    }
-    mem_cap_ = n;
-    d_mem_node_ = static_cast<int*>(alloc(n * sizeof(int)));
-    d_mem_q_ = static_cast<int*>(alloc(n * sizeof(int)));
-  }
-  check(hipMemcpyAsync(d_mem_node_, nodes.data(), n * sizeof(int), hipMemcpyHostToDevice, stream_), "mem links");
-  check(hipMemcpyAsync(d_mem_q_, qs.data(), n * sizeof(int), hipMemcpyHostToDevice, stream_), "mem links");
   const StepArgs a = args(false);
   const bool from_s = s_buf_ >= 0;
-  check(launch_mem(lattice_, storage_, from_s, a, buf_[from_s ? s_buf_ : a_buf_], d_mem_node_, d_mem_q_, n, ccx,
-                   ccy, ccz, d_out_, stream_),
+  const int nblocks = static_cast<int>(std::min<size_t>(1024, (N_ + 255) / 256));
+  check(launch_mem_scan(lattice_, storage_, from_s, a, buf_[from_s ? s_buf_ : a_buf_], ccx, ccy, ccz, r_max,
+                        d_partial_, nblocks, d_out_, stream_),
         "momentum exchange");
   check(hipMemcpyAsync(out, d_out_, 3 * sizeof(double), hipMemcpyDeviceToHost, stream_), "d2h");
   check(hipStreamSynchronize(stream_), "momentum exchange");
@@ -1016,7 +1049,7 @@ size_t Domain::bytes_needed(int dim, int lattice, int nx, int ny, int nz, int st
   const long long stride = base + (plane_pad >= 0 ? plane_pad : (N >= (1u << 20) ? 1024 : 0));
   const size_t yz = static_cast<size_t>(ny) * (dim == 2 ? 1 : nz);
   const size_t t = thermal ? 2 * 5 * static_cast<size_t>(stride) * 4 + N * 8 : 0;
-  return 2 * static_cast<size_t>(Q) * stride * elem_size(storage) + N * (1 + 1 + 4 + 5 * 4) + Q * yz * 4 +
+  return 2 * static_cast<size_t>(Q) * stride * elem_size(storage) + N * (1 + 1 + 4 + 5 * 4) + prev_stride(Q) * yz * 4 +
          3 * 1024 * 8 + 64 + t;
 }
 

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -61,6 +62,9 @@ class Domain {
   void set_solid_box(int x0, int y0, int z0, int x1, int y1, int z1);
   void set_solid_sphere(float cx, float cy, float cz, float r);
   void set_solid_ellipsoid(float cx, float cy, float cz, float rx, float ry, float rz, const float* axes);
+  // NACA 4-digit airfoil (digits: e.g. 12 for 0012), leading edge at
+  // (lead_x, lead_y), extruded over z0..z1 in 3D.
+  void set_solid_naca(int digits, float chord, float aoa_deg, float lead_x, float lead_y, int z0, int z1);
   void set_solid_circle(float cx, float cy, float r);
   void set_solid_cylinder_z(float cx, float cy, float r);
   void set_moving_wall_top(float uwx, float uwz);
@@ -171,6 +175,7 @@ class Domain {
   void invalidate_mirrors() { fields_valid_ = fsrc_valid_ = fdst_valid_ = false; }
   void* alloc(size_t bytes);
   void release(void* p);
+  void upload_prev(const std::function<float(size_t, int)>& value);
   const std::pair<std::vector<int>, std::vector<int>>& mem_link_list(float cx, float cy, float cz, float r_max);
 
   int dim_, lattice_, Q_, nx_, ny_, nz_;
@@ -240,6 +245,7 @@ class Domain {
 
   // host mirrors
   std::vector<float> h_rho_, h_ux_, h_uy_, h_uz_, h_speed_, h_fsrc_, h_fdst_;
+  std::vector<float> h_prev_;  // staging of the outlet store (kept until the upload completes)
   bool fields_valid_ = false, fsrc_valid_ = false, fdst_valid_ = false;
 };
 

@@ -46,7 +46,7 @@ struct StepArgs {
   int bfl_per_y, bfl_per_z;  // wrap of the BFL upstream read x_f - c_q
   float tau, gx, gy, gz;     // collision: shear tau and Guo body force
   float cs;                  // Smagorinsky constant (LES-coupled collision)
-  float mom[27];             // moving-wall term 2 w_q rho0 3 c_q.u_w per pulled q
+  float uw[3];               // moving-wall velocity (the pull adds 2 w_q rho0 3 c_q.u_w)
   int inflow;                // 1: Zou-He velocity inlet at x = 0
   int inflow_parabolic;      // profile 4 umax s (H - s) / H^2, s = y - 1/2, H = ny - 2
   float uin_x, uin_y, umax, turb_amp;
@@ -59,7 +59,7 @@ struct StepArgs {
   // D2Q5 temperature coupled by a Boussinesq force (2D only)
   float tau_g, gbeta, t_ref;  // temperature relaxation, a_y = gbeta (T - t_ref)
   const float* t_wall;        // per node: wall temperature of a solid node, NaN = adiabatic
-  float* f_prev;              // convective outlet store: Q planes of ny*nz
+  float* f_prev;              // convective outlet store: ny*nz records of prev_stride(Q) floats
   const uint8_t* flags;       // reference cell types
   const uint8_t* cls;         // ClassBits
   const int* aux;             // per node: index into bfl[] (valid where kClsBfl)
@@ -96,13 +96,23 @@ hipError_t launch_readback_thermal(int storage, bool from_s, const StepArgs& a, 
 // s_state (storage format, stride a.stride), or fields/planes for read-back.
 hipError_t launch_materialize(int lattice, int storage, const StepArgs& a, const void* a_in, void* s_state,
                               hipStream_t s);
+// Convective outlet store <- the x = nx - 1 plane of a materialised S state.
+hipError_t launch_prev_from_s(int lattice, int storage, const StepArgs& a, const void* s_state, hipStream_t s);
 hipError_t launch_readback(int lattice, int storage, bool from_s, const StepArgs& a, const void* buf,
                            hipStream_t s);
 // Momentum exchange over a link list (node, q) on S_k, accumulated in FP64
-// in a fixed order: out[0..2] = F (3D) or (Fx, Fy, torque) (2D).
+// in a fixed order (up to max_blocks work-group partials of 3 doubles, then
+// their sum in block order): out[0..2] = F (3D) or (Fx, Fy, torque about
+// (cx, cy)) (2D).
 hipError_t launch_mem(int lattice, int storage, bool from_s, const StepArgs& a, const void* buf,
-                      const int* link_node, const int* link_q, int nlinks, float cx, float cy, float cz,
-                      double* out, hipStream_t s);
+                      const int* link_node, const int* link_q, int nlinks, float cx, float cy, double* partial,
+                      int max_blocks, double* out, hipStream_t s);
+// The same forces without a link list: one pass over the class bytes, S
+// formed only at nodes next to a wall; nblocks work-group partials summed in
+// block order.  r_max <= 0 takes every wall link.
+hipError_t launch_mem_scan(int lattice, int storage, bool from_s, const StepArgs& a, const void* buf, float cx,
+                           float cy, float cz, float r_max, double* partial, int nblocks, double* out,
+                           hipStream_t s);
 // FP64 sums over Fluid nodes of rho and rho |u|^2 / 2, and max |u|.
 hipError_t launch_diagnostics(const uint8_t* flags, const float* rho, const float* speed, long long n,
                               double* partial, int nblocks, double* out3, hipStream_t s);

@@ -23,6 +23,10 @@ enum Cell : uint8_t { kFluid = 0, kSolid = 1, kInflow = 2, kOutflow = 3, kPeriod
                       kMovingWall = 6 };
 enum Collision : int { kBGK = 0, kTRT = 1, kMRT = 2 };
 
+// Floats per (y, z) row of the convective outlet store: Q rounded up to a
+// multiple of four, so each record is 16-byte aligned.
+constexpr int prev_stride(int q) { return (q + 3) / 4 * 4; }
+
 struct D2Q9 {
   static constexpr int Q = 9;
   static constexpr int D = 2;

@@ -151,7 +151,13 @@ __device__ __forceinline__ void gather(const StepArgs& a, const typename St::T* 
         float v = f[q];
         if (bounce) {
           v = St::template ld<L, o>(A, o * S + n);
-          if (t == kMovingWall) v += a.mom[q];
+          if (t == kMovingWall) {
+            // 2 w_q rho0 3 (c_q . u_w); the products with c in {-1, 0, 1} are
+            // exact, so this rounds as the host's left-to-right expression
+            const float ci_uw = static_cast<float>(cx) * a.uw[0] + static_cast<float>(cy) * a.uw[1] +
+                                static_cast<float>(cz) * a.uw[2];
+            v += 2.0f * L::w[q] * 1.0f * 3.0f * ci_uw;
+          }
         }
         f[q] = v;
       }
@@ -297,35 +303,78 @@ __device__ __forceinline__ void apply_zh_outflow_d2q9(const StepArgs& a, float (
 }
 
 // Full post-stream state S_k of a node (k = s_step): gather + open
-// boundaries.  The convective outlet needs the post-stream state of its
-// upstream neighbour, which the outlet lane gathers itself.
-template <class L, class St>
+// boundaries.  The convective outlet blends its previous post-stream state
+// (the outlet store, one 16-byte aligned record of prev_stride<Q> floats per
+// (y, z) row, read and written with vector accesses) with the post-stream
+// state of its upstream neighbour x - 1.  With ROW_LANES (lanes hold
+// consecutive nodes, as in every row- or tile-shaped kernel) that state is
+// taken from lane - 1 by a cross-lane shuffle of the values the neighbour
+// has just formed; otherwise (and for a lane 0 outlet) the outlet lane
+// gathers it itself.
+template <class L, class St, bool ROW_LANES = true>
 __device__ __forceinline__ void form_s_at(const StepArgs& a, const typename St::T* __restrict__ A, int x, int y,
                                           int z, int n, uint8_t c, bool commit, unsigned long long s_step,
                                           float (&f)[L::Q]) {
   gather<L, St>(a, A, x, y, z, n, c, f);
   if (c & kClsInlet) apply_inlet<L>(a, f, y, n, s_step);
-  if (c & kClsOutlet) {
-    if constexpr (L::Q == 9) {
-      if (a.outflow == 1) apply_zh_outflow_d2q9(a, f);
-    }
-    if (a.outflow == 2) {
-      float nb[L::Q];
-      const uint8_t cn = a.cls[n - 1];
-      gather<L, St>(a, A, x - 1, y, z, n - 1, cn, nb);
-      if (cn & kClsInlet) apply_inlet<L>(a, nb, y, n - 1, s_step);
-      const float alpha = a.u_conv / (1.0f + a.u_conv);
-      const float beta = 1.0f / (1.0f + a.u_conv);
-      const long long plane = static_cast<long long>(a.ny) * a.nz;
-      const long long row = static_cast<long long>(z) * a.ny + y;
+  if constexpr (L::Q == 9) {
+    if ((c & kClsOutlet) && a.outflow == 1) apply_zh_outflow_d2q9(a, f);
+  }
+  if (a.outflow != 2) return;
+  constexpr int QP = prev_stride(L::Q);
+  const float alpha = a.u_conv / (1.0f + a.u_conv);
+  const float beta = 1.0f / (1.0f + a.u_conv);
+  const bool is_out = (c & kClsOutlet) != 0;
+  float4* rec = reinterpret_cast<float4*>(a.f_prev + (static_cast<long long>(z) * a.ny + y) * QP);
+  auto load_prev = [&](float (&prev)[QP]) {
+    static_for<QP / 4>([&](auto vc) {
+      constexpr int v = decltype(vc)::value;
+      const float4 t = rec[v];
+      prev[4 * v] = t.x;
+      prev[4 * v + 1] = t.y;
+      prev[4 * v + 2] = t.z;
+      prev[4 * v + 3] = t.w;
+    });
+  };
+  auto store_prev = [&](const float (&prev)[QP]) {
+    static_for<QP / 4>([&](auto vc) {
+      constexpr int v = decltype(vc)::value;
+      rec[v] = make_float4(prev[4 * v], prev[4 * v + 1], prev[4 * v + 2], prev[4 * v + 3]);
+    });
+  };
+  bool done = false;
+  if constexpr (ROW_LANES) {
+    // wave-uniform: every lane loads its own row's record (in bounds for all
+    // lanes) and takes part in the shuffles; only outlet lanes keep results
+    if (__any(is_out)) {
+      const bool use = is_out && __lane_id() != 0;
+      float prev[QP];
+      load_prev(prev);
       static_for<L::Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        const float v = beta * a.f_prev[q * plane + row] + alpha * nb[q];
-        f[q] = v;
-        if (commit) a.f_prev[q * plane + row] = v;
+        const float v = beta * prev[q] + alpha * __shfl_up(f[q], 1);
+        if (use) f[q] = v;
+        prev[q] = v;
       });
+      if (use && commit) store_prev(prev);
+      done = use;
     }
   }
+  if (!is_out || done) return;
+  // the upstream neighbour is not in this wave: gather it
+  float nb[L::Q];
+  const uint8_t cn = a.cls[n - 1];
+  gather<L, St>(a, A, x - 1, y, z, n - 1, cn, nb);
+  if (cn & kClsInlet) apply_inlet<L>(a, nb, y, n - 1, s_step);
+  float prev[QP];
+  load_prev(prev);
+  static_for<L::Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    const float v = beta * prev[q] + alpha * nb[q];
+    f[q] = v;
+    prev[q] = v;
+  });
+  if (commit) store_prev(prev);
 }
 
 template <class L, class St>
@@ -428,6 +477,22 @@ __global__ __launch_bounds__(256) WAKE_STEP_ATTR void k_step(const StepArgs a, c
   }
 }
 
+// Convective outlet store from a materialised S state: record (y, z) = S at
+// x = nx - 1 of that row.  One lane per row.
+template <class L, class St>
+__global__ __launch_bounds__(256) void k_prev_from_s(const StepArgs a, const typename St::T* __restrict__ s_state) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= a.ny * a.nz) return;
+  constexpr int QP = prev_stride(L::Q);
+  const long long n = static_cast<long long>(row) * a.nx + (a.nx - 1);
+  float* rec = a.f_prev + static_cast<long long>(row) * QP;
+  static_for<QP>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    if constexpr (q < L::Q) rec[q] = St::template ld<L, q>(s_state, q * a.stride + n);
+    else rec[q] = 0.0f;
+  });
+}
+
 // S_k into the state buffer (storage format), committing the convective store.
 template <class L, class St>
 __global__ __launch_bounds__(256) void k_materialize(const StepArgs a, const typename St::T* __restrict__ ain,
@@ -482,16 +547,16 @@ __global__ __launch_bounds__(256) void k_readback(const StepArgs a, const typena
   }
 }
 
-// Momentum exchange contributions of this lane's links (every blockDim.x-th
-// from threadIdx.x) on S_k: F += 2 f_q c_q, and in 2D the torque about
-// (ccx, ccy) at the link midpoint (forces_momentum_exchange.ispc:6-138,
-// restricted to the precomputed links).
+// Momentum exchange contributions of this lane's links (i = first, first +
+// stride, ...) on S_k: F += 2 f_q c_q, and in 2D the torque about (ccx, ccy)
+// at the link midpoint (forces_momentum_exchange.ispc:6-138, restricted to
+// the precomputed links).
 template <class L, class St, bool FROM_S>
 __device__ __forceinline__ void mem_links(const StepArgs& a, const typename St::T* __restrict__ buf,
                                           unsigned long long s_step, const int* __restrict__ link_node,
-                                          const int* __restrict__ link_q, int nlinks, float ccx, float ccy,
-                                          double& s0, double& s1, double& s2) {
-  for (int i = threadIdx.x; i < nlinks; i += blockDim.x) {
+                                          const int* __restrict__ link_q, int nlinks, int first, int stride,
+                                          float ccx, float ccy, double& s0, double& s1, double& s2) {
+  for (int i = first; i < nlinks; i += stride) {
     const int n = link_node[i];
     const int qd = link_q[i];
     const int x = n % a.nx;
@@ -504,7 +569,7 @@ __device__ __forceinline__ void mem_links(const StepArgs& a, const typename St::
         f[q] = St::template ld<L, q>(buf, q * a.stride + n);
       });
     } else {
-      form_s_at<L, St>(a, buf, x, y, z, n, a.cls[n], false, s_step, f);
+      form_s_at<L, St, false>(a, buf, x, y, z, n, a.cls[n], false, s_step, f);
     }
     float fv = 0.0f;
     int cx = 0, cy = 0, cz = 0;
@@ -551,20 +616,88 @@ __device__ __forceinline__ void block_sum3(double& s0, double& s1, double& s2) {
   s2 = red[2][0];
 }
 
-// Momentum exchange on S_k over a link list, FP64, one work-group, fixed
-// reduction order.
+// Momentum exchange without a link list: every Fluid node whose pull is slow
+// (so it has a solid or moving-wall neighbour) forms S_k and adds 2 f_q c_q
+// for each q whose in-range neighbour x + c_q is Solid or MovingWall and
+// whose link midpoint lies within r_max of the centre (r_max <= 0: all),
+// forces_momentum_exchange.ispc:71-138.  The midpoint test rounds as the
+// host's link builder (no fused multiply-add).  Work-group partials as k_mem.
+template <class L, class St, bool FROM_S>
+__global__ __launch_bounds__(256) void k_mem_scan(const StepArgs a, const typename St::T* __restrict__ buf, float ccx,
+                                                  float ccy, float ccz, float r_max, double* __restrict__ partial) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  const long long N = static_cast<long long>(a.nx) * a.ny * a.nz;
+  const float r_max_sq = r_max * r_max;
+  for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < N;
+       i += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const int n = static_cast<int>(i);
+    const uint8_t c = a.cls[n];
+    if (!(c & kClsSlow) || a.flags[n] != kFluid) continue;
+    const int x = n % a.nx;
+    const int y = (n / a.nx) % a.ny;
+    const int z = n / (a.nx * a.ny);
+    float f[L::Q];
+    if constexpr (FROM_S) {
+      static_for<L::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        f[q] = St::template ld<L, q>(buf, q * a.stride + n);
+      });
+    } else {
+      form_s_at<L, St, false>(a, buf, x, y, z, n, c, false, a.s_step, f);
+    }
+    static_for<L::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      if constexpr (q > 0) {
+        constexpr int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
+        int ny_ = y + cy, nz_ = z + cz;
+        bool in = true;
+        if (a.per_y) ny_ = wrap(ny_, a.ny);
+        else in = in && ny_ >= 0 && ny_ < a.ny;
+        if (a.per_z) nz_ = wrap(nz_, a.nz);
+        else in = in && nz_ >= 0 && nz_ < a.nz;
+        const int nx_ = wrap(x + cx, a.nx);
+        const uint8_t t = in ? a.flags[(nz_ * a.ny + ny_) * a.nx + nx_] : static_cast<uint8_t>(kFluid);
+        if (t == kSolid || t == kMovingWall) {
+          const float rx = __fsub_rn(__fadd_rn(static_cast<float>(x), 0.5f * static_cast<float>(cx)), ccx);
+          const float ry = __fsub_rn(__fadd_rn(static_cast<float>(y), 0.5f * static_cast<float>(cy)), ccy);
+          const float rz = __fsub_rn(__fadd_rn(static_cast<float>(z), 0.5f * static_cast<float>(cz)), ccz);
+          float d2 = __fadd_rn(__fmul_rn(rx, rx), __fmul_rn(ry, ry));
+          if constexpr (L::D == 3) d2 = __fadd_rn(d2, __fmul_rn(rz, rz));
+          if (r_max <= 0.0f || d2 <= r_max_sq) {
+            const float dfx = 2.0f * f[q] * static_cast<float>(cx);
+            const float dfy = 2.0f * f[q] * static_cast<float>(cy);
+            s0 += dfx;
+            s1 += dfy;
+            if constexpr (L::D == 2) s2 += static_cast<double>(rx * dfy - ry * dfx);
+            else s2 += static_cast<double>(2.0f * f[q] * static_cast<float>(cz));
+          }
+        }
+      }
+    });
+  }
+  block_sum3(s0, s1, s2);
+  if (threadIdx.x == 0) {
+    partial[3 * blockIdx.x] = s0;
+    partial[3 * blockIdx.x + 1] = s1;
+    partial[3 * blockIdx.x + 2] = s2;
+  }
+}
+
+// Momentum exchange on S_k over a link list, FP64: each work-group sums a
+// grid-strided share of the links in a fixed tree order into partial[3 b];
+// k_sum3 adds the partials in block order, so the result is reproducible.
 template <class L, class St, bool FROM_S>
 __global__ __launch_bounds__(256) void k_mem(const StepArgs a, const typename St::T* __restrict__ buf,
                                              const int* __restrict__ link_node, const int* __restrict__ link_q,
-                                             int nlinks, float ccx, float ccy, float ccz, double* __restrict__ out) {
-  (void)ccz;  // the 3D force needs no moment arm
+                                             int nlinks, float ccx, float ccy, double* __restrict__ partial) {
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  mem_links<L, St, FROM_S>(a, buf, a.s_step, link_node, link_q, nlinks, ccx, ccy, s0, s1, s2);
+  mem_links<L, St, FROM_S>(a, buf, a.s_step, link_node, link_q, nlinks, blockIdx.x * blockDim.x + threadIdx.x,
+                           gridDim.x * blockDim.x, ccx, ccy, s0, s1, s2);
   block_sum3(s0, s1, s2);
   if (threadIdx.x == 0) {
-    out[0] = s0;
-    out[1] = s1;
-    out[2] = s2;
+    partial[3 * blockIdx.x] = s0;
+    partial[3 * blockIdx.x + 1] = s1;
+    partial[3 * blockIdx.x + 2] = s2;
   }
 }
 

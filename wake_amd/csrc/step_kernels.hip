@@ -1,5 +1,7 @@
 // step_kernels.hip -- lattice dispatch of the step launchers, BFL marking and
 // the FP64 diagnostics reductions.
+#include <algorithm>
+
 #include "step_launch.hpp"
 
 namespace wake_hip {
@@ -80,6 +82,44 @@ hipError_t launch_step(int lattice, int storage, int collision, bool les, bool p
   }
 }
 
+// Sum of nblocks (x, y, z) partials in block order.
+__global__ void k_sum3(const double* __restrict__ partial, int nblocks, double* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int b = 0; b < nblocks; ++b) {
+    s0 += partial[3 * b];
+    s1 += partial[3 * b + 1];
+    s2 += partial[3 * b + 2];
+  }
+  out[0] = s0;
+  out[1] = s1;
+  out[2] = s2;
+}
+
+hipError_t launch_mem_scan(int lattice, int storage, bool from_s, const StepArgs& a, const void* buf, float cx,
+                           float cy, float cz, float r_max, double* partial, int nblocks, double* out,
+                           hipStream_t s) {
+  hipError_t e;
+  switch (lattice) {
+    case 9: e = detail::mem_scan_d2q9(storage, from_s, a, buf, cx, cy, cz, r_max, partial, nblocks, s); break;
+    case 19: e = detail::mem_scan_d3q19(storage, from_s, a, buf, cx, cy, cz, r_max, partial, nblocks, s); break;
+    case 27: e = detail::mem_scan_d3q27(storage, from_s, a, buf, cx, cy, cz, r_max, partial, nblocks, s); break;
+    default: return hipErrorInvalidValue;
+  }
+  if (e != hipSuccess) return e;
+  k_sum3<<<1, 64, 0, s>>>(partial, nblocks, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_prev_from_s(int lattice, int storage, const StepArgs& a, const void* s_state, hipStream_t s) {
+  switch (lattice) {
+    case 9: return detail::prev_from_s_d2q9(storage, a, s_state, s);
+    case 19: return detail::prev_from_s_d3q19(storage, a, s_state, s);
+    case 27: return detail::prev_from_s_d3q27(storage, a, s_state, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 hipError_t launch_materialize(int lattice, int storage, const StepArgs& a, const void* a_in, void* s_state,
                               hipStream_t s) {
   switch (lattice) {
@@ -101,14 +141,19 @@ hipError_t launch_readback(int lattice, int storage, bool from_s, const StepArgs
 }
 
 hipError_t launch_mem(int lattice, int storage, bool from_s, const StepArgs& a, const void* buf,
-                      const int* link_node, const int* link_q, int nlinks, float cx, float cy, float cz,
-                      double* out, hipStream_t s) {
+                      const int* link_node, const int* link_q, int nlinks, float cx, float cy, double* partial,
+                      int max_blocks, double* out, hipStream_t s) {
+  const int nb = std::max(1, std::min(max_blocks, (nlinks + 255) / 256));
+  hipError_t e;
   switch (lattice) {
-    case 9: return detail::mem_d2q9(storage, from_s, a, buf, link_node, link_q, nlinks, cx, cy, cz, out, s);
-    case 19: return detail::mem_d3q19(storage, from_s, a, buf, link_node, link_q, nlinks, cx, cy, cz, out, s);
-    case 27: return detail::mem_d3q27(storage, from_s, a, buf, link_node, link_q, nlinks, cx, cy, cz, out, s);
+    case 9: e = detail::mem_d2q9(storage, from_s, a, buf, link_node, link_q, nlinks, cx, cy, partial, nb, s); break;
+    case 19: e = detail::mem_d3q19(storage, from_s, a, buf, link_node, link_q, nlinks, cx, cy, partial, nb, s); break;
+    case 27: e = detail::mem_d3q27(storage, from_s, a, buf, link_node, link_q, nlinks, cx, cy, partial, nb, s); break;
     default: return hipErrorInvalidValue;
   }
+  if (e != hipSuccess) return e;
+  k_sum3<<<1, 64, 0, s>>>(partial, nb, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_mark_bfl(const int* nodes, int count, uint8_t* cls, int* aux, hipStream_t s) {

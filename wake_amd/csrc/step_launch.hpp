@@ -66,6 +66,29 @@ struct Launch {
     });
   }
 
+  static hipError_t mem_scan(int storage, bool from_s, const StepArgs& a, const void* buf, float cx, float cy,
+                             float cz, float r_max, double* partial, int nblocks, hipStream_t s) {
+    return with_storage(storage, [&](auto st) {
+      using St = decltype(st);
+      using T = typename St::T;
+      if (from_s)
+        k_mem_scan<L, St, true><<<nblocks, 256, 0, s>>>(a, static_cast<const T*>(buf), cx, cy, cz, r_max, partial);
+      else
+        k_mem_scan<L, St, false><<<nblocks, 256, 0, s>>>(a, static_cast<const T*>(buf), cx, cy, cz, r_max, partial);
+      return hipGetLastError();
+    });
+  }
+
+  static hipError_t prev_from_s(int storage, const StepArgs& a, const void* s_state, hipStream_t s) {
+    const int rows = a.ny * a.nz;
+    return with_storage(storage, [&](auto st) {
+      using St = decltype(st);
+      using T = typename St::T;
+      k_prev_from_s<L, St><<<(rows + 255) / 256, 256, 0, s>>>(a, static_cast<const T*>(s_state));
+      return hipGetLastError();
+    });
+  }
+
   static hipError_t materialize(int storage, const StepArgs& a, const void* a_in, void* s_state, hipStream_t s) {
     const int bx = pick_block(a.nx, 0);
     return with_storage(storage, [&](auto st) {
@@ -88,16 +111,17 @@ struct Launch {
   }
 
   static hipError_t mem(int storage, bool from_s, const StepArgs& a, const void* buf, const int* link_node,
-                        const int* link_q, int nlinks, float cx, float cy, float cz, double* out, hipStream_t s) {
+                        const int* link_q, int nlinks, float cx, float cy, double* partial, int nblocks,
+                        hipStream_t s) {
     return with_storage(storage, [&](auto st) {
       using St = decltype(st);
       using T = typename St::T;
       if (from_s)
-        k_mem<L, St, true><<<1, 256, 0, s>>>(a, static_cast<const T*>(buf), link_node, link_q, nlinks, cx, cy, cz,
-                                             out);
+        k_mem<L, St, true><<<nblocks, 256, 0, s>>>(a, static_cast<const T*>(buf), link_node, link_q, nlinks, cx, cy,
+                                                   partial);
       else
-        k_mem<L, St, false><<<1, 256, 0, s>>>(a, static_cast<const T*>(buf), link_node, link_q, nlinks, cx, cy, cz,
-                                              out);
+        k_mem<L, St, false><<<nblocks, 256, 0, s>>>(a, static_cast<const T*>(buf), link_node, link_q, nlinks, cx,
+                                                    cy, partial);
       return hipGetLastError();
     });
   }
@@ -109,9 +133,13 @@ struct Launch {
   hipError_t step_##NAME(int storage, int collision, bool les, bool pull, const StepArgs& a, const void* a_in, \
                          void* a_out, int block_x, hipStream_t s);                                             \
   hipError_t materialize_##NAME(int storage, const StepArgs& a, const void* a_in, void* s_state, hipStream_t s); \
+  hipError_t prev_from_s_##NAME(int storage, const StepArgs& a, const void* s_state, hipStream_t s);             \
+  hipError_t mem_scan_##NAME(int storage, bool from_s, const StepArgs& a, const void* buf, float cx, float cy,    \
+                             float cz, float r_max, double* partial, int nblocks, hipStream_t s);               \
   hipError_t readback_##NAME(int storage, bool from_s, const StepArgs& a, const void* buf, hipStream_t s);      \
   hipError_t mem_##NAME(int storage, bool from_s, const StepArgs& a, const void* buf, const int* link_node,    \
-                        const int* link_q, int nlinks, float cx, float cy, float cz, double* out, hipStream_t s);
+                        const int* link_q, int nlinks, float cx, float cy, double* partial, int nblocks,        \
+                        hipStream_t s);
 
 #define WAKE_DEFINE_LATTICE(NAME, LAT)                                                                          \
   hipError_t build_class_##NAME(const StepArgs& a, uint8_t* cls, hipStream_t s) {                               \
@@ -124,12 +152,20 @@ struct Launch {
   hipError_t materialize_##NAME(int storage, const StepArgs& a, const void* a_in, void* s_state, hipStream_t s) { \
     return Launch<LAT>::materialize(storage, a, a_in, s_state, s);                                              \
   }                                                                                                             \
+  hipError_t prev_from_s_##NAME(int storage, const StepArgs& a, const void* s_state, hipStream_t s) {            \
+    return Launch<LAT>::prev_from_s(storage, a, s_state, s);                                                    \
+  }                                                                                                             \
+  hipError_t mem_scan_##NAME(int storage, bool from_s, const StepArgs& a, const void* buf, float cx, float cy,    \
+                             float cz, float r_max, double* partial, int nblocks, hipStream_t s) {              \
+    return Launch<LAT>::mem_scan(storage, from_s, a, buf, cx, cy, cz, r_max, partial, nblocks, s);              \
+  }                                                                                                             \
   hipError_t readback_##NAME(int storage, bool from_s, const StepArgs& a, const void* buf, hipStream_t s) {      \
     return Launch<LAT>::readback(storage, from_s, a, buf, s);                                                   \
   }                                                                                                             \
   hipError_t mem_##NAME(int storage, bool from_s, const StepArgs& a, const void* buf, const int* link_node,     \
-                        const int* link_q, int nlinks, float cx, float cy, float cz, double* out, hipStream_t s) { \
-    return Launch<LAT>::mem(storage, from_s, a, buf, link_node, link_q, nlinks, cx, cy, cz, out, s);            \
+                        const int* link_q, int nlinks, float cx, float cy, double* partial, int nblocks,         \
+                        hipStream_t s) {                                                                         \
+    return Launch<LAT>::mem(storage, from_s, a, buf, link_node, link_q, nlinks, cx, cy, partial, nblocks, s);    \
   }
 
 WAKE_DECLARE_LATTICE(d2q9)

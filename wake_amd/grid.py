@@ -263,6 +263,11 @@ class LatticeGrid2D(_Domain):
         """Anti-bounce-back temperature of the solid nodes in the box; NaN = adiabatic."""
         self._c(self._L.wake_domain_set_wall_temperature(self._h, x0, y0, x1, y1, t_wall), "set_wall_temperature")
 
+    def set_solid_naca(self, code, chord, aoa_deg, lead_x, lead_y):
+        """naca_airfoil.cpp:152-166 rasterize of NACA `code` (e.g. "0012")."""
+        self._c(self._L.wake_domain_set_solid_naca(self._h, int(code), chord, aoa_deg, lead_x, lead_y, 0, 0),
+                "set_solid_naca")
+
     def set_solid_circle(self, cx, cy, radius):
         self._c(self._L.wake_domain_set_solid_circle(self._h, cx, cy, radius), "set_solid_circle")
 
@@ -330,6 +335,12 @@ class LatticeGrid3D(_Domain):
 
     def set_solid_sphere(self, cx, cy, cz, radius):
         self._c(self._L.wake_domain_set_solid_sphere(self._h, cx, cy, cz, radius), "set_solid_sphere")
+
+    def set_solid_naca(self, code, chord, aoa_deg, lead_x, lead_y, z_start=0, z_end=None):
+        """naca_airfoil.cpp:152-190 voxelize_extruded of NACA `code` (e.g. "0012")."""
+        z1 = self.nz() - 1 if z_end is None else z_end
+        self._c(self._L.wake_domain_set_solid_naca(self._h, int(code), chord, aoa_deg, lead_x, lead_y, z_start, z1),
+                "set_solid_naca")
 
     def set_solid_ellipsoid(self, cx, cy, cz, rx, ry, rz, axes=None):
         """primitives.cpp:73-100; axes: optional 3x3 whose rows are the body
