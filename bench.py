@@ -332,35 +332,45 @@ def main_wing(args):
     def coeffs(F):
         return F[:, 0] / area, F[:, 1] / area
 
-    res = {}
+    # checkpoints: field errors while the runs are still correlated, and
+    # force statistics over the whole history
+    checks = sorted({c for c in (steps // 8, steps // 4, steps // 2, steps) if c > 0})
+    res, fields = {}, {}
     fl = None
     for st in (0, 1, 2):
         g = wing_setup(wake_amd.LatticeGrid3D(sx, sy, sz, stau, lattice=27, storage=st), sx, sy, sc)
         b = wake_amd.Batch(lattice=27, storage=st, collision="TRT", use_les=True, chunk=100)
         b.add(g, steps, smag=0.17, forces=(0.0, 0.0, 0.0, 0.0))
-        b.run(steps)
+        done = 0
+        for c in checks:
+            b.run(c - done)
+            done = c
+            fl = g.cell_type_data() == 0
+            fields[(names[st], c)] = np.concatenate([g.ux_data()[fl], g.uy_data()[fl], g.uz_data()[fl]])
         F = b.forces(0)[every - 1::every]
-        fl = g.cell_type_data() == 0
-        res[names[st]] = (np.concatenate([g.ux_data()[fl], g.uy_data()[fl], g.uz_data()[fl]]), *coeffs(F))
+        res[names[st]] = coeffs(F)
         del b, g
     log(f"reduced {sx}x{sy}x{sz}: GPU runs done; FP64 oracle {steps} steps on the host")
     t0 = time.perf_counter()
     o = wing_setup(ob.OracleGrid(3, sx, sy, sz, tau=stau, lattice=27, bits=64), sx, sy, sc)
-    hist = []
+    hist, ref_fields = [], {}
     for i in range(steps // every):
         o.run(every, "TRT", use_les=True, smag=0.17)
         hist.append(o.momentum_exchange(0.0, 0.0, 0.0, 0.0))
+        if (i + 1) * every in checks:
+            ref_fields[(i + 1) * every] = np.concatenate([o.ux()[fl], o.uy()[fl], o.uz()[fl]])
     log(f"FP64 oracle done in {time.perf_counter() - t0:.0f} s")
-    ref_u = np.concatenate([o.ux()[fl], o.uy()[fl], o.uz()[fl]])
     ref_cd, ref_cl = coeffs(np.array(hist))
     half = len(ref_cd) // 2
     st_of = lambda cl: analysis.strouhal_number(
         analysis.analyze_frequency(cl[half:].astype(np.float32), float(every)), sc, u)
     ref = {"C_D": float(np.mean(ref_cd[half:])), "C_L": float(np.mean(ref_cl[half:])), "St": st_of(ref_cl)}
     errors = {}
-    for nm, (uvec, cd, cl) in res.items():
+    for nm, (cd, cl) in res.items():
         v = {"C_D": float(np.mean(cd[half:])), "C_L": float(np.mean(cl[half:])), "St": st_of(cl)}
-        errors[nm] = {"velocity_rel_l2": float(np.linalg.norm(uvec - ref_u) / np.linalg.norm(ref_u)),
+        l2 = {str(c): float(np.linalg.norm(fields[(nm, c)] - ref_fields[c]) / np.linalg.norm(ref_fields[c]))
+              for c in checks}
+        errors[nm] = {"velocity_rel_l2_at_step": l2,
                       **{f"{q}_rel": (abs(v[q] / ref[q] - 1) if ref[q] else None) for q in v}, **v}
         log(f"{nm} vs FP64: {errors[nm]}")
     out = {

@@ -137,6 +137,13 @@ void d3q27_stream_periodic(const float f_src[], float f_dst[], int nx, int ny, i
 void d3q27_compute_macroscopic_moments(const float f[], float rho[], float ux[], float uy[], float uz[],
                                        int count);
 
+/* src/ispc/analytics_kernels.ispc:4 */
+void compute_vorticity_2d(const float ux[], const float uy[], float vorticity[], int nx, int ny);
+/* src/ispc/analytics_kernels.ispc:33 */
+void compute_vorticity_and_q_criterion_3d(const float ux[], const float uy[], const float uz[], float omega_x[],
+                                          float omega_y[], float omega_z[], float q_criterion[], int nx, int ny,
+                                          int nz);
+
 /* ----------------------------------------------------------------------
  * Part 2: device-resident domains (the LatticeGrid2D / LatticeGrid3D path)
  * -------------------------------------------------------------------- */
@@ -152,7 +159,13 @@ enum wake_field {
   WAKE_FIELD_SPEED = 4,
   WAKE_FIELD_F_SRC = 5, /* Q planes of nx*ny*nz: the post-stream populations */
   WAKE_FIELD_F_DST = 6, /* Q planes: the post-collision populations of the last step */
-  WAKE_FIELD_TEMPERATURE = 7 /* thermal domains: T of the post-stream temperature populations */
+  WAKE_FIELD_TEMPERATURE = 7, /* thermal domains: T of the post-stream temperature populations */
+  /* vorticity and Q-criterion of u (analytics_kernels.ispc:4-110, computed on
+   * the device); a 2D domain has WAKE_FIELD_OMEGA_Z only */
+  WAKE_FIELD_OMEGA_X = 8,
+  WAKE_FIELD_OMEGA_Y = 9,
+  WAKE_FIELD_OMEGA_Z = 10,
+  WAKE_FIELD_Q_CRITERION = 11
 };
 
 /* LatticeGrid3D(nx, ny, nz, tau) (lattice_grid_3d.cpp:46-64) for dim = 3,

@@ -1986,3 +1986,51 @@ void oracle_grid_momentum_exchange(const oracle_grid* g, real cx, real cy, real 
           cz, r_max, out3);
   }
 }
+
+/* analytics_kernels.ispc:4-29: omega_z = d uy/dx - d ux/dy, central
+ * differences, periodic wrap. */
+void oracle_compute_vorticity_2d(const real* ux, const real* uy, real* w, int nx, int ny) {
+  for (int y = 0; y < ny; ++y) {
+    int ym = (y - 1 + ny) % ny, yp = (y + 1) % ny;
+    for (int x = 0; x < nx; ++x) {
+      int xm = (x - 1 + nx) % nx, xp = (x + 1) % nx;
+      real duy_dx = R(0.5) * (uy[y * nx + xp] - uy[y * nx + xm]);
+      real dux_dy = R(0.5) * (ux[yp * nx + x] - ux[ym * nx + x]);
+      w[y * nx + x] = duy_dx - dux_dy;
+    }
+  }
+}
+
+/* analytics_kernels.ispc:33-110: curl and Q = (||Omega||^2 - ||S||^2) / 2. */
+void oracle_compute_vorticity_and_q_criterion_3d(const real* ux, const real* uy, const real* uz, real* ox, real* oy,
+                                                 real* oz, real* qc, int nx, int ny, int nz) {
+  for (int z = 0; z < nz; ++z) {
+    int zm = (z - 1 + nz) % nz, zp = (z + 1) % nz;
+    for (int y = 0; y < ny; ++y) {
+      int ym = (y - 1 + ny) % ny, yp = (y + 1) % ny;
+      size_t c = ((size_t)z * ny + y) * nx, cyp = ((size_t)z * ny + yp) * nx, cym = ((size_t)z * ny + ym) * nx;
+      size_t czp = ((size_t)zp * ny + y) * nx, czm = ((size_t)zm * ny + y) * nx;
+      for (int x = 0; x < nx; ++x) {
+        int xm = (x - 1 + nx) % nx, xp = (x + 1) % nx;
+        real dux_dx = R(0.5) * (ux[c + xp] - ux[c + xm]);
+        real dux_dy = R(0.5) * (ux[cyp + x] - ux[cym + x]);
+        real dux_dz = R(0.5) * (ux[czp + x] - ux[czm + x]);
+        real duy_dx = R(0.5) * (uy[c + xp] - uy[c + xm]);
+        real duy_dy = R(0.5) * (uy[cyp + x] - uy[cym + x]);
+        real duy_dz = R(0.5) * (uy[czp + x] - uy[czm + x]);
+        real duz_dx = R(0.5) * (uz[c + xp] - uz[c + xm]);
+        real duz_dy = R(0.5) * (uz[cyp + x] - uz[cym + x]);
+        real duz_dz = R(0.5) * (uz[czp + x] - uz[czm + x]);
+        ox[c + x] = duz_dy - duy_dz;
+        oy[c + x] = dux_dz - duz_dx;
+        oz[c + x] = duy_dx - dux_dy;
+        real s_xy = R(0.5) * (dux_dy + duy_dx), s_yz = R(0.5) * (duy_dz + duz_dy), s_xz = R(0.5) * (dux_dz + duz_dx);
+        real norm_s = (dux_dx * dux_dx + duy_dy * duy_dy + duz_dz * duz_dz) +
+                      R(2) * (s_xy * s_xy + s_yz * s_yz + s_xz * s_xz);
+        real o_xy = R(0.5) * (dux_dy - duy_dx), o_yz = R(0.5) * (duy_dz - duz_dy), o_xz = R(0.5) * (dux_dz - duz_dx);
+        real norm_o = R(2) * (o_xy * o_xy + o_yz * o_yz + o_xz * o_xz);
+        qc[c + x] = R(0.5) * (norm_o - norm_s);
+      }
+    }
+  }
+}

@@ -169,6 +169,11 @@ double oracle_grid_total_kinetic_energy(const oracle_grid* g);
 double oracle_grid_max_speed(const oracle_grid* g);
 void oracle_grid_momentum_exchange(const oracle_grid* g, real cx, real cy, real cz, real r_max, double* out3);
 
+/* analytics_kernels.ispc:4-110 */
+void oracle_compute_vorticity_2d(const real* ux, const real* uy, real* w, int nx, int ny);
+void oracle_compute_vorticity_and_q_criterion_3d(const real* ux, const real* uy, const real* uz, real* ox, real* oy,
+                                                 real* oz, real* qc, int nx, int ny, int nz);
+
 #ifdef __cplusplus
 }
 #endif

@@ -36,7 +36,8 @@ def test_library_exports_every_declared_function():
 
 
 def test_kernel_exports_match_reference_names():
-    # the 28 LBM exports of src/ispc that the hot path replaces
+    # the 28 LBM exports of src/ispc that the hot path replaces, and the two
+    # analytics exports that read its velocity field
     names = ["d2q9_compute_macroscopic_moments", "d2q9_compute_macroscopic_moments_forced",
              "d2q9_compute_velocity_magnitude", "d2q9_collide_bgk", "d2q9_collide_trt", "d2q9_collide_mrt",
              "d2q9_stream_periodic", "d2q9_stream_with_boundaries", "d2q9_apply_bounce_back",
@@ -46,7 +47,9 @@ def test_kernel_exports_match_reference_names():
              "d3q19_compute_macroscopic_moments_forced", "d3q19_compute_velocity_magnitude", "d3q19_collide_bgk",
              "d3q19_collide_trt", "d3q19_collide_mrt", "d3q19_stream_periodic", "d3q19_stream_with_boundaries",
              "d3q19_compute_smagorinsky_tau", "d3q19_apply_curved_bfl", "d3q19_compute_momentum_exchange_forces",
-             "d3q27_collide", "d3q27_stream_periodic", "d3q27_compute_macroscopic_moments"]
+             "d3q27_collide", "d3q27_stream_periodic", "d3q27_compute_macroscopic_moments",
+             # analytics_kernels.ispc (vorticity and Q-criterion)
+             "compute_vorticity_2d", "compute_vorticity_and_q_criterion_3d"]
     ex = exported_symbols()
     assert all(n in ex for n in names)
     import wake_amd._lib as wl

@@ -281,3 +281,49 @@ def test_momentum_exchange_kernels(K):
         ctypes.c_float(10.0), *[ctypes.byref(v) for v in b])
     for x, y in zip(a, b):
         assert abs(float(x[0]) - y.value) <= 1e-5 * max(1.0, abs(y.value))
+
+
+# ---------------------------------------------------------------- analytics
+def test_vorticity_and_q_criterion_exports(K):
+    """analytics_kernels.ispc:4-110 on random fields vs the oracle."""
+    rng = np.random.default_rng(21)
+    nx, ny, nz = 37, 21, 13
+    u2 = [(0.05 * rng.standard_normal(nx * ny)).astype(np.float32) for _ in range(2)]
+    w, wo = np.zeros(nx * ny, np.float32), np.zeros(nx * ny, np.float32)
+    K.kernel("compute_vorticity_2d", *u2, w, nx, ny)
+    call(32, "compute_vorticity_2d", *u2, wo, nx, ny)
+    close(w, wo)
+    n = nx * ny * nz
+    u3 = [(0.05 * rng.standard_normal(n)).astype(np.float32) for _ in range(3)]
+    g = [np.zeros(n, np.float32) for _ in range(4)]
+    o = [np.zeros(n, np.float32) for _ in range(4)]
+    K.kernel("compute_vorticity_and_q_criterion_3d", *u3, *g, nx, ny, nz)
+    call(32, "compute_vorticity_and_q_criterion_3d", *u3, *o, nx, ny, nz)
+    for a, b in zip(g, o):
+        close(a, b, 4e-6)
+
+
+def test_domain_vorticity_on_device():
+    """The grid's device-side vorticity / Q of its own velocity field equal
+    the oracle's analytics applied to the same (read-back) field."""
+    import wake_amd
+
+    g = wake_amd.LatticeGrid3D(40, 24, 20, 0.6)
+    g.set_periodic_axes(True, True)
+    g.initialize(1.0, 0.04, 0.0, 0.0)
+    g.set_solid_sphere(12.0, 12.0, 10.0, 4.0)
+    g.run(30, "MRT")
+    ox, oy, oz, q = g.vorticity_q_data()
+    u = [np.ascontiguousarray(a) for a in (g.ux_data(), g.uy_data(), g.uz_data())]
+    o = [np.zeros_like(u[0]) for _ in range(4)]
+    call(32, "compute_vorticity_and_q_criterion_3d", *u, *o, 40, 24, 20)
+    for a, b in zip((ox, oy, oz, q), o):
+        close(a, b, 4e-6)
+    g2 = wake_amd.LatticeGrid2D(64, 32, 0.6)
+    g2.initialize(1.0, 0.05, 0.0)
+    g2.set_solid_circle(16.0, 16.0, 4.0)
+    g2.run(40, "MRT")
+    w = g2.vorticity_data()
+    wo = np.zeros_like(w)
+    call(32, "compute_vorticity_2d", np.ascontiguousarray(g2.ux_data()), np.ascontiguousarray(g2.uy_data()), wo, 64, 32)
+    close(w, wo)

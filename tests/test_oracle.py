@@ -659,3 +659,30 @@ def test_naca_voxelizer_shape(oracle32):
     o3.set_solid_naca(12, c, 8.0, lx, ly)
     h = o3.cell_type().reshape(6, 128, 192)
     assert all(np.array_equal(h[z], g) for z in range(6))
+
+
+def test_vorticity_and_q_criterion_closed_forms():
+    """analytics_kernels.ispc:4-110 restated: on periodic sine fields the
+    central differences have closed forms (sin(k(y+1)) - sin(k(y-1)) =
+    2 sin k cos ky), so omega and Q are known exactly; pure shear has Q = 0."""
+    nx, ny, nz = 16, 12, 10
+    kx, ky = 2 * np.pi / nx, 2 * np.pi / ny
+    y2, x2 = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    ux = np.ascontiguousarray((-np.sin(ky * y2)).ravel())
+    uy = np.ascontiguousarray(np.sin(kx * x2).ravel())
+    w = np.zeros(nx * ny)
+    ob.call(64, "compute_vorticity_2d", ux, uy, w, nx, ny)
+    expect = np.sin(kx) * np.cos(kx * x2) + np.sin(ky) * np.cos(ky * y2)
+    assert np.max(np.abs(w - expect.ravel())) < 1e-12
+    z3, y3, x3 = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    u = [np.ascontiguousarray(a.ravel()) for a in (-np.sin(ky * y3), np.sin(kx * x3), np.zeros_like(x3, float))]
+    out = [np.zeros(nx * ny * nz) for _ in range(4)]
+    ob.call(64, "compute_vorticity_and_q_criterion_3d", *u, *out, nx, ny, nz)
+    dux_dy = -np.sin(ky) * np.cos(ky * y3)
+    duy_dx = np.sin(kx) * np.cos(kx * x3)
+    assert np.max(np.abs(out[2] - (duy_dx - dux_dy).ravel())) < 1e-12
+    assert np.max(np.abs(out[0])) == 0.0 and np.max(np.abs(out[1])) == 0.0
+    assert np.max(np.abs(out[3] - (-dux_dy * duy_dx).ravel())) < 1e-12
+    shear = [np.ascontiguousarray(np.sin(ky * y3).ravel()), np.zeros(nx * ny * nz), np.zeros(nx * ny * nz)]
+    ob.call(64, "compute_vorticity_and_q_criterion_3d", *shear, *out, nx, ny, nz)
+    assert np.max(np.abs(out[3])) == 0.0

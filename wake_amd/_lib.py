@@ -50,6 +50,8 @@ KERNEL_SIGS = {
     "d3q27_collide": "pppppppfffffi",
     "d3q27_stream_periodic": "ppiii",
     "d3q27_compute_macroscopic_moments": "pppppi",
+    "compute_vorticity_2d": "pppii",
+    "compute_vorticity_and_q_criterion_3d": "pppppppiii",
 }
 _CT = {"p": P, "i": I, "f": F}
 

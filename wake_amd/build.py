@@ -32,7 +32,7 @@ HOST_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-ffp-contract=off", f"--offload-arc
               "-Wno-unused-command-line-argument"]
 
 SOURCES = [
-    "step_d3q19.hip", "step_d3q27.hip", "step_d2q9.hip", "step_kernels.hip", "thermal.hip",
+    "step_d3q19.hip", "step_d3q27.hip", "step_d2q9.hip", "step_kernels.hip", "thermal.hip", "analytics.hip",
     "ispc_compat.hip", "batch.hip", "domain.cpp", "capi.cpp", "lattice_grid.cpp", "runner.cpp", "batch_host.cpp",
 ]
 

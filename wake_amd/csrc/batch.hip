@@ -3,8 +3,8 @@
 // The per-node work is the device-resident step's (form_s_at + collide, the
 // same device functions the single-domain kernels use), so a domain gives
 // the same populations stepped alone or in a batch; only the indexing
-// differs: a lane finds its node from the tile it runs and the slot that owns
-// the tile.
+// differs: a work-group finds its slot, row and x segment from the tile it
+// runs (all uniform), and a lane its x within the segment.
 #include "batch.hpp"
 #include "thermal_impl.hpp"
 
@@ -18,16 +18,17 @@ __global__ __launch_bounds__(256) void k_step_batch(const BatchSlot* __restrict_
     const BatchSlot& sl = slots[tile_slot[t]];
     const unsigned long long cur = sl.step + static_cast<unsigned long long>(i);
     if (cur >= sl.stop_at) continue;
-    const int n = (t - sl.tile0) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
-    if (n >= sl.nodes) continue;
     const StepArgs& a = sl.a;
+    const int local = t - sl.tile0;
+    const int row = local / sl.tiles_per_row;
+    const int x = (local - row * sl.tiles_per_row) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    if (x >= a.nx) continue;
+    const int y = row % a.ny;
+    const int z = row / a.ny;
+    const int n = row * a.nx + x;
     const int par = (sl.parity + i) & 1;
     const T* __restrict__ in = static_cast<const T*>(sl.buf[par]);
     T* __restrict__ out = static_cast<T*>(sl.buf[par ^ 1]);
-    const int x = n % a.nx;
-    const int yz = n / a.nx;
-    const int y = yz % a.ny;
-    const int z = yz / a.ny;
     if constexpr (THERMAL) {
       thermal_node<St, COLL>(a, in, out, sl.gbuf[par], sl.gbuf[par ^ 1], x, y, n, cur != sl.s_at, cur);
       continue;

@@ -2,9 +2,10 @@
 //
 // Every domain keeps its own buffers, cell types, classes and boundary
 // configuration (a Domain); a batch only adds a device table of slots and a
-// tile map.  A tile is `block` consecutive nodes of one domain in its own
-// node order (n = (z * ny + y) * nx + x), so a work-group never straddles two
-// domains and all per-domain arguments are work-group uniform (scalar loads).
+// tile map.  A tile is a segment of `block` consecutive x of one (y, z) row
+// of one domain, so a work-group never straddles two domains or two rows:
+// the per-domain arguments and the row coordinates are work-group uniform
+// (scalar loads and scalar arithmetic), as in the single-domain step.
 // The kernels derive everything that changes from step to step -- which
 // buffer is read, the step index of the inlet noise, whether the domain has
 // already reached its last step -- from the slot and the launch's position in
@@ -29,6 +30,7 @@ struct BatchSlot {
   int parity;                 // buffer holding the state at `step`
   int nodes;                  // nx * ny * nz
   int tile0, ntiles;          // tiles [tile0, tile0 + ntiles) belong to this slot
+  int tiles_per_row;          // ceil(nx / block)
   const int* mem_node;        // momentum-exchange links (node, q), or null
   const int* mem_q;
   int mem_count;

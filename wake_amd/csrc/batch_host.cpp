@@ -299,7 +299,8 @@ class Batch {
       b.parity = parity;
       b.nodes = static_cast<int>(m.d->total_nodes());
       b.tile0 = static_cast<int>(tile_slot.size());
-      b.ntiles = (b.nodes + shape_.block - 1) / shape_.block;
+      b.tiles_per_row = (m.d->nx() + shape_.block - 1) / shape_.block;
+      b.ntiles = m.d->ny() * m.d->nz() * b.tiles_per_row;
       tile_slot.insert(tile_slot.end(), b.ntiles, s);
       if (m.forces) {
         b.mem_count = m.d->batch_mem_links(m.c[0], m.c[1], m.c[2], m.r_max, &b.mem_node, &b.mem_q);

@@ -216,6 +216,10 @@ int wake_domain_read(wake_domain* d, int field, float* out) {
       case WAKE_FIELD_F_SRC: src = g.f_src(); n = N * g.lattice(); break;
       case WAKE_FIELD_F_DST: src = g.f_dst(); n = N * g.lattice(); break;
       case WAKE_FIELD_TEMPERATURE: src = g.temperature(); break;
+      case WAKE_FIELD_OMEGA_X: src = g.analytics(0); break;
+      case WAKE_FIELD_OMEGA_Y: src = g.analytics(1); break;
+      case WAKE_FIELD_OMEGA_Z: src = g.analytics(2); break;
+      case WAKE_FIELD_Q_CRITERION: src = g.analytics(3); break;
       default: throw wake_hip::HipError("unknown field");
     }
     std::memcpy(out, src, n * sizeof(float));

@@ -800,6 +800,30 @@ void Domain::update_macroscopic() {
   g_last_[2] = keep[2];
 }
 
+const float* Domain::analytics(int which) {
+  if (which < 0 || which > 3) throw HipError("unknown analytics field");
+  if (dim_ == 2 && which != 2) throw HipError("a 2D domain has only the z vorticity");
+  if (!an_valid_) {
+    ensure_fields();
+    const int k0 = dim_ == 2 ? 2 : 0, k1 = dim_ == 2 ? 2 : 3;
+    for (int k = k0; k <= k1; ++k)
+      if (!d_an_[k]) {
+        d_an_[k] = static_cast<float*>(alloc(N_ * sizeof(float)));
+        h_an_[k].resize(N_);
+      }
+    if (dim_ == 2)
+      check(launch_vorticity_2d(d_ux_, d_uy_, d_an_[2], nx_, ny_, stream_), "vorticity");
+    else
+      check(launch_vorticity_q_3d(d_ux_, d_uy_, d_uz_, d_an_[0], d_an_[1], d_an_[2], d_an_[3], nx_, ny_, nz_, stream_),
+            "vorticity");
+    for (int k = k0; k <= k1; ++k)
+      check(hipMemcpyAsync(h_an_[k].data(), d_an_[k], N_ * sizeof(float), hipMemcpyDeviceToHost, stream_), "d2h");
+    check(hipStreamSynchronize(stream_), "vorticity");
+    an_valid_ = true;
+  }
+  return h_an_[which].data();
+}
+
 const float* Domain::rho() { ensure_fields(); return h_rho_.data(); }
 const float* Domain::ux() { ensure_fields(); return h_ux_.data(); }
 const float* Domain::uy() { ensure_fields(); return h_uy_.data(); }

@@ -102,6 +102,10 @@ class Domain {
   const float* f_src();
   const float* f_dst();
   const uint8_t* cell_type() const { return flags_.data(); }
+  // Vorticity / Q-criterion of the current velocity field, on the device
+  // (analytics_kernels.ispc:4-110): which = 0 omega_x, 1 omega_y, 2 omega_z
+  // (the 2D vorticity), 3 Q-criterion (3D).
+  const float* analytics(int which);
   void update_macroscopic();
   // FP64 sums over Fluid nodes of rho and rho |u|^2 / 2, and max |u|.
   void diagnostics(double out[3]);
@@ -172,7 +176,7 @@ class Domain {
   void sync_device_config();
   void ensure_fields();
   void upload_state_from_host(const std::vector<float>& f_ref);
-  void invalidate_mirrors() { fields_valid_ = fsrc_valid_ = fdst_valid_ = false; }
+  void invalidate_mirrors() { fields_valid_ = fsrc_valid_ = fdst_valid_ = an_valid_ = false; }
   void* alloc(size_t bytes);
   void release(void* p);
   void upload_prev(const std::function<float(size_t, int)>& value);
@@ -240,6 +244,10 @@ class Domain {
   float* gbuf_[2] = {nullptr, nullptr};
   float* d_twall_ = nullptr;
   float* d_temp_ = nullptr;
+  // analytics outputs (allocated on first use)
+  float* d_an_[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<float> h_an_[4];
+  bool an_valid_ = false;
   std::vector<float> h_twall_, h_temp_;
   bool twall_dirty_ = false;
 

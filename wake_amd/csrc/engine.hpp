@@ -113,6 +113,12 @@ hipError_t launch_mem(int lattice, int storage, bool from_s, const StepArgs& a, 
 hipError_t launch_mem_scan(int lattice, int storage, bool from_s, const StepArgs& a, const void* buf, float cx,
                            float cy, float cz, float r_max, double* partial, int nblocks, double* out,
                            hipStream_t s);
+// Vorticity (2D: omega_z) and, in 3D, the curl and the Q-criterion of a
+// velocity field, central differences with periodic wrap (analytics.hip).
+// Null outputs are skipped.
+hipError_t launch_vorticity_2d(const float* ux, const float* uy, float* w, int nx, int ny, hipStream_t s);
+hipError_t launch_vorticity_q_3d(const float* ux, const float* uy, const float* uz, float* ox, float* oy, float* oz,
+                                 float* q, int nx, int ny, int nz, hipStream_t s);
 // FP64 sums over Fluid nodes of rho and rho |u|^2 / 2, and max |u|.
 hipError_t launch_diagnostics(const uint8_t* flags, const float* rho, const float* speed, long long n,
                               double* partial, int nblocks, double* out3, hipStream_t s);

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "wake_hip.h"
+#include "engine.hpp"
 #include "lattice.hpp"
 #include "node_ops.hpp"
 
@@ -514,6 +515,35 @@ void d2q9_compute_macroscopic_moments(const float f[], float rho[], float ux[], 
 void d2q9_compute_macroscopic_moments_forced(const float f[], float rho[], float ux[], float uy[], float gx,
                                              float gy, int count) {
   moments_call<D2Q9, true>(__func__, f, rho, ux, uy, nullptr, gx, gy, 0, count);
+}
+// analytics_kernels.ispc:4-29
+void compute_vorticity_2d(const float ux[], const float uy[], float vorticity[], int nx, int ny) {
+  Call c(__func__);
+  const size_t n = static_cast<size_t>(nx) * ny;
+  auto a = c.in(ux, n);
+  auto b = c.in(uy, n);
+  auto w = c.out(vorticity, n);
+  if (!c.begin()) return;
+  (void)launch_vorticity_2d(c.dev(a), c.dev(b), c.dev(w), nx, ny, nullptr);
+  c.end();
+}
+// analytics_kernels.ispc:33-110
+void compute_vorticity_and_q_criterion_3d(const float ux[], const float uy[], const float uz[], float omega_x[],
+                                          float omega_y[], float omega_z[], float q_criterion[], int nx, int ny,
+                                          int nz) {
+  Call c(__func__);
+  const size_t n = static_cast<size_t>(nx) * ny * nz;
+  auto a = c.in(ux, n);
+  auto b = c.in(uy, n);
+  auto d = c.in(uz, n);
+  auto ox = c.out(omega_x, n);
+  auto oy = c.out(omega_y, n);
+  auto oz = c.out(omega_z, n);
+  auto q = c.out(q_criterion, n);
+  if (!c.begin()) return;
+  (void)launch_vorticity_q_3d(c.dev(a), c.dev(b), c.dev(d), c.dev(ox), c.dev(oy), c.dev(oz), c.dev(q), nx, ny, nz,
+                              nullptr);
+  c.end();
 }
 void d2q9_compute_velocity_magnitude(const float ux[], const float uy[], float speed[], int count) {
   Call c(__func__);

@@ -117,6 +117,49 @@ __device__ __forceinline__ float smagorinsky_tau(const float (&f)[L::Q], float r
   return 0.5f * (tau0 + sqrtf(disc));
 }
 
+// The same relaxation time with the non-equilibrium stress taken from the
+// second moments, Pi_ab = sum c_a c_b f - (rho/3 delta_ab + rho u_a u_b), which
+// equals sum c_a c_b (f - feq) for the second-order equilibrium of D2Q9,
+// D3Q19 and D3Q27 (their weights are isotropic to fourth order).  Used by the
+// LES-coupled BGK/TRT step: no equilibrium is formed, so none is kept live
+// beside the collision's own.
+template <class L>
+__device__ __forceinline__ float smagorinsky_tau_moments(const float (&f)[L::Q], float rho, float ux, float uy,
+                                                         float uz, float tau0, float cs) {
+  float sxx = 0.0f, syy = 0.0f, szz = 0.0f, sxy = 0.0f, syz = 0.0f, sxz = 0.0f;
+  static_for<L::Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int a = L::cx[q], b = L::cy[q], c = L::cz[q];
+    const float v = f[q];
+    if constexpr (a != 0) sxx += v;
+    if constexpr (b != 0) syy += v;
+    if constexpr (c != 0) szz += v;
+    if constexpr (a * b == 1) sxy += v;
+    if constexpr (a * b == -1) sxy -= v;
+    if constexpr (b * c == 1) syz += v;
+    if constexpr (b * c == -1) syz -= v;
+    if constexpr (a * c == 1) sxz += v;
+    if constexpr (a * c == -1) sxz -= v;
+  });
+  const float r3 = rho * (1.0f / 3.0f);
+  const float pxx = sxx - (r3 + rho * ux * ux);
+  const float pyy = syy - (r3 + rho * uy * uy);
+  const float pxy = sxy - rho * ux * uy;
+  float tensor_sum;
+  if constexpr (L::D == 2) {
+    tensor_sum = pxx * pxx + 2.0f * pxy * pxy + pyy * pyy;
+  } else {
+    const float pzz = szz - (r3 + rho * uz * uz);
+    const float pyz = syz - rho * uy * uz;
+    const float pxz = sxz - rho * ux * uz;
+    tensor_sum = (pxx * pxx + pyy * pyy + pzz * pzz) + 2.0f * (pxy * pxy + pyz * pyz + pxz * pxz);
+  }
+  const float q_stress = sqrtf(2.0f * tensor_sum);
+  const float prefactor = 18.0f * 1.4142135623730951f * (cs * cs);
+  const float disc = tau0 * tau0 + (prefactor / rho) * q_stress;
+  return 0.5f * (tau0 + sqrtf(disc));
+}
+
 // TRT antisymmetric tau from the grid (lattice_grid_3d.cpp:328-330), Lambda = 1/4.
 __device__ __host__ __forceinline__ float trt_tau_a(float tau_s) {
   float diff = tau_s - 0.5f;
@@ -151,7 +194,11 @@ __device__ __forceinline__ Macro collide_bgk(float (&f)[L::Q], float tau, float 
 }
 
 // TRT with split Guo forcing, in place (d3q19_collision_trt.ispc:5-93,
-// d3q27_collision_mrt.ispc:5-91).  OPP selects the pairing table.
+// d3q27_collision_mrt.ispc:5-91).  OPP selects the pairing table.  Each
+// opposite pair is relaxed together from its two populations and their
+// equilibria formed on the fly, so no equilibrium or output array is held
+// (the D3Q27 kernel stays within 5 waves per SIMD); every output is the
+// same expression of the same operands as a per-population loop.
 template <class L, bool REF_PAIRS = false>
 __device__ __forceinline__ Macro collide_trt(float (&f)[L::Q], float tau_s, float tau_a, float gx, float gy,
                                              float gz) {
@@ -162,28 +209,28 @@ __device__ __forceinline__ Macro collide_trt(float (&f)[L::Q], float tau_s, floa
   guo_velocity(rho, jx, jy, jz, gx, gy, gz, ux, uy, uz);
   const float u_dot_g = ux * gx + uy * gy + uz * gz;
   const float usq = ux * ux + uy * uy + uz * uz;
-  float fe[L::Q];
-  static_for<L::Q>([&](auto qc) {
+  auto relax = [&](auto qc, float fq, float fo, float feq_q, float feo) {
     constexpr int q = decltype(qc)::value;
-    fe[q] = feq<L, q>(rho, ux, uy, uz, usq);
-  });
-  float out[L::Q];
-  static_for<L::Q>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    constexpr int o = REF_PAIRS ? L::opp_ref[q] : L::opp[q];
-    const float f_sym = 0.5f * (f[q] + f[o]);
-    const float f_asym = 0.5f * (f[q] - f[o]);
-    const float fe_sym = 0.5f * (fe[q] + fe[o]);
-    const float fe_asym = 0.5f * (fe[q] - fe[o]);
+    const float f_sym = 0.5f * (fq + fo);
+    const float f_asym = 0.5f * (fq - fo);
+    const float fe_sym = 0.5f * (feq_q + feo);
+    const float fe_asym = 0.5f * (feq_q - feo);
     const float cu = cdot<L, q>(ux, uy, uz);
     const float cg = cdot<L, q>(gx, gy, gz);
     const float F_asym = pre_a * L::w[q] * rho * (3.0f * cg);
     const float F_sym = pre_s * L::w[q] * rho * (-3.0f * u_dot_g + 9.0f * cu * cg);
-    out[q] = f[q] - omega_s * (f_sym - fe_sym) - omega_a * (f_asym - fe_asym) + (F_sym + F_asym);
-  });
+    return fq - omega_s * (f_sym - fe_sym) - omega_a * (f_asym - fe_asym) + (F_sym + F_asym);
+  };
   static_for<L::Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    f[q] = out[q];
+    constexpr int o = REF_PAIRS ? L::opp_ref[q] : L::opp[q];
+    if constexpr (q <= o) {
+      const float fq = f[q], fo = f[o];
+      const float feq_q = feq<L, q>(rho, ux, uy, uz, usq);
+      const float feo = feq<L, o>(rho, ux, uy, uz, usq);
+      f[q] = relax(qc, fq, fo, feq_q, feo);
+      if constexpr (q != o) f[o] = relax(std::integral_constant<int, o>{}, fo, fq, feo, feq_q);
+    }
   });
   return {rho, ux, uy, uz};
 }
@@ -412,7 +459,7 @@ __device__ __forceinline__ Macro collide_les(float (&f)[L::Q], float tau0, float
     float rho, jx, jy, jz, ux, uy, uz;
     density_momentum<L>(f, rho, jx, jy, jz);
     guo_velocity(rho, jx, jy, jz, gx, gy, gz, ux, uy, uz);
-    return collide<L, COLL>(f, smagorinsky_tau<L>(f, rho, ux, uy, uz, tau0, cs), gx, gy, gz);
+    return collide<L, COLL>(f, smagorinsky_tau_moments<L>(f, rho, ux, uy, uz, tau0, cs), gx, gy, gz);
   }
 }
 

@@ -10,11 +10,19 @@ namespace detail {
 
 inline dim3 grid_of(const StepArgs& a, int bx) { return dim3((a.nx + bx - 1) / bx, a.ny, a.nz); }
 
+// Work-group width along x: the one of 256 / 128 / 64 that leaves the
+// fewest idle lanes per row (the larger on a tie), e.g. 128 for nx = 384.
 inline int pick_block(int nx, int block_x) {
   if (block_x > 0) return block_x;
-  if (nx >= 256) return 256;
-  if (nx >= 128) return 128;
-  return 64;
+  int best = 256, best_pad = (nx + 255) / 256 * 256;
+  for (int b : {128, 64}) {
+    const int pad = (nx + b - 1) / b * b;
+    if (pad < best_pad) {
+      best = b;
+      best_pad = pad;
+    }
+  }
+  return best;
 }
 
 template <class F>

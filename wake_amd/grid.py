@@ -180,6 +180,14 @@ class _Domain:
     def f_src_data(self):
         return self._read(5, self.Q * self.total_nodes()).reshape(self.Q, -1)
 
+    def vorticity_data(self):
+        """z vorticity (2D) on the device (analytics_kernels.ispc:4-29)."""
+        return self._read(10, self.total_nodes())
+
+    def vorticity_q_data(self):
+        """3D: (omega_x, omega_y, omega_z, Q) (analytics_kernels.ispc:33-110)."""
+        return tuple(self._read(k, self.total_nodes()) for k in (8, 9, 10, 11))
+
     def temperature_data(self):
         """Temperature of a thermal (D2Q9 + D2Q5) domain."""
         return self._read(7, self.total_nodes())

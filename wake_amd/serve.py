@@ -175,6 +175,9 @@ class Server:
             if b is None:
                 lat, st, coll, les, thermal = key
                 b = Batch(lattice=lat, storage=st, collision=coll, use_les=les, thermal=thermal, chunk=self.chunk)
+                # row segments of 64 lanes: no idle lanes for the 64-multiple
+                # grid edges of the request mix
+                b.set_launch(64, 0)
                 self.batches[key] = b
             slot = b.add(g, req.steps, **({"smag": CS_LES} if req.kind == "bluff3d" else {}))
             self.active[req.rid] = (req, g, key, slot)
