@@ -686,3 +686,30 @@ def test_vorticity_and_q_criterion_closed_forms():
     shear = [np.ascontiguousarray(np.sin(ky * y3).ravel()), np.zeros(nx * ny * nz), np.zeros(nx * ny * nz)]
     ob.call(64, "compute_vorticity_and_q_criterion_3d", *shear, *out, nx, ny, nz)
     assert np.max(np.abs(out[3])) == 0.0
+
+
+def test_d2q9_apply_bounce_back_closed_form():
+    """d2q9_boundaries.ispc:5-39 restated: on Solid and MovingWall nodes the
+    populations are reflected (f_q <- f_opp(q) + 2 w_q rho0 3 c_q.u_w), so the
+    mass is kept, the momentum becomes -j + 2 u_w (sum w c c = I/3), applying
+    it twice with u_w = 0 is the identity, and Fluid nodes are untouched."""
+    n = 64
+    rng = np.random.default_rng(5)
+    f0 = (D2Q9_W[:, None] * (1 + 0.1 * rng.standard_normal((9, n)))).ravel()
+    ct = np.zeros(n, np.uint8)
+    ct[::3] = 1
+    ct[1::7] = 6
+    f = f0.copy()
+    ob.call(64, "d2q9_apply_bounce_back", f, ct, 0.0, 0.0, n)
+    ob.call(64, "d2q9_apply_bounce_back", f, ct, 0.0, 0.0, n)
+    assert np.array_equal(f, f0)
+    uw = (0.03, -0.02)
+    f = f0.copy()
+    ob.call(64, "d2q9_apply_bounce_back", f, ct, *uw, n)
+    a, b = f0.reshape(9, n), f.reshape(9, n)
+    wall = ct != 0
+    assert np.array_equal(a[:, ~wall], b[:, ~wall])
+    assert np.allclose(b[:, wall].sum(0), a[:, wall].sum(0), atol=1e-15)
+    cx, cy = D2Q9_C[:, 0][:, None], D2Q9_C[:, 1][:, None]
+    for c, u in ((cx, uw[0]), (cy, uw[1])):
+        assert np.allclose((c * b[:, wall]).sum(0), -(c * a[:, wall]).sum(0) + 2 * u, atol=1e-15)
