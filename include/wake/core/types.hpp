@@ -18,8 +18,31 @@ struct Vec3 {
   real_t x{0.0f}, y{0.0f}, z{0.0f};
   constexpr Vec3() = default;
   constexpr Vec3(real_t a, real_t b, real_t c) : x(a), y(b), z(c) {}
+  constexpr Vec3 operator+(const Vec3& o) const { return {x + o.x, y + o.y, z + o.z}; }
+  constexpr Vec3 operator-(const Vec3& o) const { return {x - o.x, y - o.y, z - o.z}; }
+  constexpr Vec3 operator*(real_t s) const { return {x * s, y * s, z * s}; }
+  constexpr Vec3 operator/(real_t s) const { return {x / s, y / s, z / s}; }
+  Vec3& operator+=(const Vec3& o) {
+    x += o.x;
+    y += o.y;
+    z += o.z;
+    return *this;
+  }
+  Vec3& operator-=(const Vec3& o) {
+    x -= o.x;
+    y -= o.y;
+    z -= o.z;
+    return *this;
+  }
+  Vec3& operator*=(real_t s) {
+    x *= s;
+    y *= s;
+    z *= s;
+    return *this;
+  }
   constexpr real_t dot(const Vec3& o) const { return x * o.x + y * o.y + z * o.z; }
   constexpr real_t norm_sq() const { return x * x + y * y + z * z; }
+  constexpr Vec3 cross(const Vec3& o) const { return {y * o.z - z * o.y, z * o.x - x * o.z, x * o.y - y * o.x}; }
 };
 
 enum class CellType : uint8_t { Fluid = 0, Solid = 1, Inflow = 2, Outflow = 3, Periodic = 4, Symmetry = 5, MovingWall = 6 };

@@ -213,6 +213,14 @@ int wake_domain_set_solid_ellipsoid(wake_domain* d, float cx, float cy, float cz
  * z0..z1 (3D). */
 int wake_domain_set_solid_naca(wake_domain* d, int digits, float chord, float aoa_deg, float lead_x, float lead_y,
                                int z0, int z1);
+/* src/geometry/obj_loader.cpp:208-400 (ObjLoader::voxelize_into_grid) on
+ * the device: tris holds ntri triangles as 9 floats each (v0, v1, v2 xyz,
+ * host memory, grid units).  Cells whose unit box [i, i+1]x[j, j+1]x[k, k+1]
+ * overlaps a triangle become Solid (the reference's separating-axis test,
+ * same FP32 expressions); with fill_interior, so does every cell that no
+ * 6-neighbour path through non-Solid cells joins to the outer faces.  The
+ * resulting cell types equal the CPU voxelizer's bit for bit. */
+int wake_domain_voxelize_mesh(wake_domain* d, const float* tris, long long ntri, int fill_interior);
 int wake_domain_set_solid_circle(wake_domain* d, float cx, float cy, float r);
 int wake_domain_set_solid_cylinder_z(wake_domain* d, float cx, float cy, float r);
 int wake_domain_set_moving_wall_top(wake_domain* d, float u_wall_x, float u_wall_z);
@@ -257,6 +265,9 @@ int wake_domain_diagnostics(wake_domain* d, double out[3]);
 int wake_domain_momentum_exchange(wake_domain* d, float cx, float cy, float cz, float r_max, double out[3]);
 /* out: nx, ny, nz, Q, plane stride, current step, storage, device bytes */
 int wake_domain_info(wake_domain* d, long long out[8]);
+/* lattice_grid_3d.hpp is_periodic(): 1 while the domain streams fully
+ * periodic (no non-Fluid cell type was ever set, lattice_grid_3d.cpp:133-140). */
+int wake_domain_is_periodic(wake_domain* d, int* out);
 /* D2Q5 temperature coupled to the D2Q9 flow by a Boussinesq acceleration
  * a_y = gbeta (T - t_ref) per node (2D only; no counterpart in the reference,
  * whose src/thermal/thermal_solver.cpp:83-123 couples only the mean

@@ -26,11 +26,15 @@
 #define RCOS(x) cos(x)
 #define RSIN(x) sin(x)
 #define RABS(x) fabs(x)
+#define RFLOOR(x) floor(x)
+#define RCEIL(x) ceil(x)
 #else
 #define RSQRT(x) sqrtf(x)
 #define RCOS(x) cosf(x)
 #define RSIN(x) sinf(x)
 #define RABS(x) fabsf(x)
+#define RFLOOR(x) floorf(x)
+#define RCEIL(x) ceilf(x)
 #endif
 
 #define R(x) ((real)(x))
@@ -2033,4 +2037,134 @@ void oracle_compute_vorticity_and_q_criterion_3d(const real* ux, const real* uy,
       }
     }
   }
+}
+
+/* ------------------------------------------------------------------------
+ * Triangle-mesh voxelizer.  src/geometry/obj_loader.cpp:208-400.
+ * ---------------------------------------------------------------------- */
+
+static real rmin3(real a, real b, real c) {
+  real m = a;
+  if (b < m) m = b;
+  if (c < m) m = c;
+  return m;
+}
+static real rmax3(real a, real b, real c) {
+  real m = a;
+  if (b > m) m = b;
+  if (c > m) m = c;
+  return m;
+}
+
+/* obj_loader.cpp:262-271: projections a*p - b*q of the three vertices on one
+ * candidate axis against the box radius fa*h1 + fb*h2. */
+static int sat_axis(real a, real b, real fa, real fb, real p0, real q0, real p1, real q1, real p2, real q2, real h1,
+                    real h2) {
+  real s0 = a * p0 - b * q0;
+  real s1 = a * p1 - b * q1;
+  real s2 = a * p2 - b * q2;
+  real rad = fa * h1 + fb * h2;
+  return rmin3(s0, s1, s2) <= rad && rmax3(s0, s1, s2) >= -rad;
+}
+
+/* obj_loader.cpp:209-306 (test_triangle_box_overlap).  The nine edge axes
+ * are called with the reference's arguments (:277-301). */
+int oracle_tri_box_overlap(const real* tri, const real* c, const real* h) {
+  real v0x = tri[0] - c[0], v0y = tri[1] - c[1], v0z = tri[2] - c[2];
+  real v1x = tri[3] - c[0], v1y = tri[4] - c[1], v1z = tri[5] - c[2];
+  real v2x = tri[6] - c[0], v2y = tri[7] - c[1], v2z = tri[8] - c[2];
+  real e[3][3] = {{v1x - v0x, v1y - v0y, v1z - v0z}, {v2x - v1x, v2y - v1y, v2z - v1z}, {v0x - v2x, v0y - v2y, v0z - v2z}};
+  if (rmin3(v0x, v1x, v2x) > h[0] || rmax3(v0x, v1x, v2x) < -h[0]) return 0;
+  if (rmin3(v0y, v1y, v2y) > h[1] || rmax3(v0y, v1y, v2y) < -h[1]) return 0;
+  if (rmin3(v0z, v1z, v2z) > h[2] || rmax3(v0z, v1z, v2z) < -h[2]) return 0;
+  /* plane of the triangle: normal e0 x e1 */
+  real nx = e[0][1] * e[1][2] - e[0][2] * e[1][1];
+  real ny = e[0][2] * e[1][0] - e[0][0] * e[1][2];
+  real nz = e[0][0] * e[1][1] - e[0][1] * e[1][0];
+  real d = nx * v0x + ny * v0y + nz * v0z;
+  real mnx = nx > R(0) ? -h[0] : h[0], mxx = nx > R(0) ? h[0] : -h[0];
+  real mny = ny > R(0) ? -h[1] : h[1], mxy = ny > R(0) ? h[1] : -h[1];
+  real mnz = nz > R(0) ? -h[2] : h[2], mxz = nz > R(0) ? h[2] : -h[2];
+  if (nx * mnx + ny * mny + nz * mnz > d || nx * mxx + ny * mxy + nz * mxz < d) return 0;
+  for (int k = 0; k < 3; ++k) {
+    real ex = e[k][0], ey = e[k][1], ez = e[k][2];
+    real fx = RABS(ex), fy = RABS(ey), fz = RABS(ez);
+    if (!sat_axis(ez, ey, fz, fy, v0y, v0z, v1y, v1z, v2y, v2z, h[1], h[2])) return 0;
+    if (!sat_axis(-ez, ex, fz, fx, v0x, v0z, v1x, v1z, v2x, v2z, h[0], h[2])) return 0;
+    if (!sat_axis(ey, -ex, fy, fx, v0x, v0y, v1x, v1y, v2x, v2y, h[0], h[1])) return 0;
+  }
+  return 1;
+}
+
+/* obj_loader.cpp:308-400 (voxelize_into_grid): surface cells by the overlap
+ * test over each triangle's floor/ceil cell box, then a breadth-first walk
+ * from the non-Solid cells of the six outer faces; unreached cells become
+ * Solid.  Returns how many Solid writes were made (set_cell_type clears
+ * the periodic mode on any of them). */
+long long oracle_voxelize_mesh(uint8_t* flags, int nx, int ny, int nz, const real* tris, long long ntri,
+                               int fill_interior) {
+  const real half[3] = {R(0.5), R(0.5), R(0.5)};
+  long long writes = 0;
+  for (long long t = 0; t < ntri; ++t) {
+    const real* v = tris + 9 * t;
+    int i_min = imax(0, (int)RFLOOR(rmin3(v[0], v[3], v[6]))), i_max = imin(nx - 1, (int)RCEIL(rmax3(v[0], v[3], v[6])));
+    int j_min = imax(0, (int)RFLOOR(rmin3(v[1], v[4], v[7]))), j_max = imin(ny - 1, (int)RCEIL(rmax3(v[1], v[4], v[7])));
+    int k_min = imax(0, (int)RFLOOR(rmin3(v[2], v[5], v[8]))), k_max = imin(nz - 1, (int)RCEIL(rmax3(v[2], v[5], v[8])));
+    for (int k = k_min; k <= k_max; ++k)
+      for (int j = j_min; j <= j_max; ++j)
+        for (int i = i_min; i <= i_max; ++i) {
+          real c[3] = {(real)i + R(0.5), (real)j + R(0.5), (real)k + R(0.5)};
+          if (oracle_tri_box_overlap(v, c, half)) {
+            flags[((size_t)k * ny + j) * nx + i] = CELL_SOLID;
+            ++writes;
+          }
+        }
+  }
+  if (!fill_interior) return writes;
+  size_t n = (size_t)nx * ny * nz;
+  uint8_t* seen = (uint8_t*)calloc(n ? n : 1, 1);
+  size_t* queue = (size_t*)malloc((n ? n : 1) * sizeof(size_t));
+  size_t head = 0, tail = 0;
+#define PUSH(x, y, z)                                            \
+  do {                                                           \
+    size_t id_ = ((size_t)(z) * ny + (y)) * nx + (x);            \
+    if (!seen[id_] && flags[id_] != CELL_SOLID) {                \
+      seen[id_] = 1;                                             \
+      queue[tail++] = id_;                                       \
+    }                                                            \
+  } while (0)
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j) {
+      PUSH(0, j, k);
+      PUSH(nx - 1, j, k);
+    }
+  for (int k = 0; k < nz; ++k)
+    for (int i = 0; i < nx; ++i) {
+      PUSH(i, 0, k);
+      PUSH(i, ny - 1, k);
+    }
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      PUSH(i, j, 0);
+      PUSH(i, j, nz - 1);
+    }
+  while (head < tail) {
+    size_t cur = queue[head++];
+    int x = (int)(cur % (size_t)nx), y = (int)((cur / (size_t)nx) % (size_t)ny), z = (int)(cur / ((size_t)nx * ny));
+    if (x + 1 < nx) PUSH(x + 1, y, z);
+    if (x > 0) PUSH(x - 1, y, z);
+    if (y + 1 < ny) PUSH(x, y + 1, z);
+    if (y > 0) PUSH(x, y - 1, z);
+    if (z + 1 < nz) PUSH(x, y, z + 1);
+    if (z > 0) PUSH(x, y, z - 1);
+  }
+#undef PUSH
+  for (size_t i = 0; i < n; ++i)
+    if (!seen[i]) {
+      flags[i] = CELL_SOLID;
+      ++writes;
+    }
+  free(queue);
+  free(seen);
+  return writes;
 }

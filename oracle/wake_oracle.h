@@ -169,6 +169,15 @@ double oracle_grid_total_kinetic_energy(const oracle_grid* g);
 double oracle_grid_max_speed(const oracle_grid* g);
 void oracle_grid_momentum_exchange(const oracle_grid* g, real cx, real cy, real cz, real r_max, double* out3);
 
+/* obj_loader.cpp:209-306: triangle (v0, v1, v2 as 9 coordinates) against
+ * the box of centre c and half extents h, separating-axis test. */
+int oracle_tri_box_overlap(const real* tri, const real* c, const real* h);
+/* obj_loader.cpp:308-400: voxelize ntri triangles into flags[(z*ny+y)*nx+x]
+ * (surface, then the enclosed cells with fill_interior); returns the number
+ * of Solid writes. */
+long long oracle_voxelize_mesh(uint8_t* flags, int nx, int ny, int nz, const real* tris, long long ntri,
+                               int fill_interior);
+
 /* analytics_kernels.ispc:4-110 */
 void oracle_compute_vorticity_2d(const real* ux, const real* uy, real* w, int nx, int ny);
 void oracle_compute_vorticity_and_q_criterion_3d(const real* ux, const real* uy, const real* uz, real* ox, real* oy,

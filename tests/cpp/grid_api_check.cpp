@@ -5,9 +5,16 @@
 #include <cmath>
 #include <cstdio>
 #include <memory>
+#include <string>
+#include <vector>
 
+#include "wake/analytics/field_analytics.hpp"
 #include "wake/core/error.hpp"
 #include "wake/forces/momentum_exchange.hpp"
+#include "wake/forces/strouhal_analyzer.hpp"
+#include "wake/geometry/naca_airfoil.hpp"
+#include "wake/geometry/obj_loader.hpp"
+#include "wake/geometry/primitives.hpp"
 #include "wake/grid/lattice_grid_2d.hpp"
 #include "wake/grid/lattice_grid_3d.hpp"
 #include "wake/runner/simulation_runner.hpp"
@@ -144,8 +151,163 @@ static void runner() {
   std::printf("runner: %d callbacks\n", calls);
 }
 
+// 2D primitives and NACA 0012, as tests/unit/test_geometry.cpp:10-101
+static void geometry() {
+  LatticeGrid2D g2(64, 64, 0.02f);
+  g2.initialize();
+  GeometryPrimitives::rasterize_circle(g2, 32.0f, 32.0f, 10.0f);
+  EXPECT(g2.get_cell_type(32, 32) == CellType::Solid, "circle centre");
+  EXPECT(g2.get_cell_type(0, 0) == CellType::Fluid, "circle far corner");
+  GeometryPrimitives::rasterize_rectangle(g2, 5.0f, 5.0f, 15.0f, 15.0f);
+  EXPECT(g2.get_cell_type(10, 10) == CellType::Solid, "rectangle");
+  GeometryPrimitives::rasterize_rotated_plate(g2, 45.0f, 45.0f, 20.0f, 4.0f, 45.0f);
+  EXPECT(g2.get_cell_type(45, 45) == CellType::Solid, "rotated plate");
+
+  NacaAirfoil naca("0012", 50.0f, 5.0f);
+  EXPECT(naca.max_thickness() == 0.12f && naca.max_camber() == 0.0f, "NACA 0012 parameters");
+  EXPECT(!naca.polygon().empty(), "NACA polygon");
+  LatticeGrid2D a2(128, 64, 0.02f);
+  a2.initialize();
+  naca.rasterize(a2, 30.0f, 32.0f);
+  EXPECT(a2.get_cell_type(42, 32) == CellType::Solid, "NACA quarter chord");
+  EXPECT(a2.get_cell_type(0, 0) == CellType::Fluid, "NACA far corner");
+  // the rasterized cells are exactly the ones contains_point accepts
+  int mism = 0, solid = 0;
+  for (int y = 0; y < 64; ++y)
+    for (int x = 0; x < 128; ++x) {
+      const bool in = naca.contains_point(static_cast<real_t>(x), static_cast<real_t>(y), 30.0f, 32.0f);
+      mism += in != (a2.get_cell_type(x, y) == CellType::Solid);
+      solid += in;
+    }
+  EXPECT(mism == 0 && solid > 50, "NACA raster vs contains_point: %d mismatches, %d solid", mism, solid);
+  LatticeGrid3D a3(64, 64, 32, 0.02f);
+  a3.initialize();
+  naca.voxelize_extruded(a3, 15.0f, 32.0f, 8, 24);
+  EXPECT(a3.get_cell_type(25, 32, 16) == CellType::Solid, "extruded wing");
+  EXPECT(a3.get_cell_type(25, 32, 0) == CellType::Fluid && a3.get_cell_type(25, 32, 31) == CellType::Fluid,
+         "extrusion span");
+
+  // axis-aligned ellipsoid: the FP32 test (x-cx)^2/rx^2 + ... <= 1 node by node
+  LatticeGrid3D e3(40, 32, 24, 0.6f);
+  e3.initialize();
+  GeometryPrimitives::voxelize_ellipsoid(e3, 20.0f, 16.0f, 12.0f, 9.0f, 6.0f, 4.0f);
+  int emism = 0;
+  for (int z = 0; z < 24; ++z)
+    for (int y = 0; y < 32; ++y)
+      for (int x = 0; x < 40; ++x) {
+        const double dx = (x - 20.0) / 9.0, dy = (y - 16.0) / 6.0, dz = (z - 12.0) / 4.0;
+        const double r = dx * dx + dy * dy + dz * dz;
+        if (std::fabs(r - 1.0) < 1e-5) continue;  // FP32 ties decide these
+        emism += (r <= 1.0) != (e3.get_cell_type(x, y, z) == CellType::Solid);
+      }
+  EXPECT(emism == 0, "ellipsoid: %d mismatches", emism);
+  std::printf("geometry: NACA 0012 %d solid cells\n", solid);
+}
+
+// OBJ cube parsed, fitted into [10, 22]^3 and voxelized with its interior,
+// as tests/unit/test_geometry.cpp:29-72; the cell types are then exactly
+// the cells 10..22 on every axis (floor/ceil cell boxes, touching overlaps).
+static void obj_mesh() {
+  const std::string cube =
+      "v -1.0 -1.0 -1.0\nv  1.0 -1.0 -1.0\nv  1.0  1.0 -1.0\nv -1.0  1.0 -1.0\n"
+      "v -1.0 -1.0  1.0\nv  1.0 -1.0  1.0\nv  1.0  1.0  1.0\nv -1.0  1.0  1.0\n"
+      "f 1 2 3 4\nf 5 8 7 6\nf 1 5 6 2\nf 2 6 7 3\nf 3 7 8 4\nf 5 1 4 8\n";
+  ObjLoader loader;
+  EXPECT(loader.load_from_string(cube), "OBJ parse");
+  EXPECT(loader.vertex_count() == 8 && loader.triangle_count() == 12, "OBJ counts %zu %zu", loader.vertex_count(),
+         loader.triangle_count());
+  LatticeGrid3D grid(32, 32, 32, 0.02f);
+  grid.initialize();
+  loader.fit_into_box(Vec3(10.0f, 10.0f, 10.0f), Vec3(22.0f, 22.0f, 22.0f));
+  Vec3 lo, hi;
+  loader.get_bounds(lo, hi);
+  EXPECT(lo.x == 10.0f && hi.z == 22.0f, "fit_into_box bounds");
+  loader.voxelize_into_grid(grid, true);
+  EXPECT(grid.get_cell_type(16, 16, 16) == CellType::Solid, "cube centre");
+  EXPECT(grid.get_cell_type(2, 2, 2) == CellType::Fluid && grid.get_cell_type(30, 30, 30) == CellType::Fluid,
+         "cube far corners");
+  int mism = 0;
+  auto in = [](int i) { return i >= 10 && i <= 22; };
+  for (int z = 0; z < 32; ++z)
+    for (int y = 0; y < 32; ++y)
+      for (int x = 0; x < 32; ++x)
+        mism += (in(x) && in(y) && in(z)) != (grid.get_cell_type(x, y, z) == CellType::Solid);
+  EXPECT(mism == 0, "voxelized cube: %d cells off the 10..22 block", mism);
+  EXPECT(!grid.is_periodic(), "a Solid write clears the periodic mode");
+  std::printf("obj_mesh: 12 triangles, %d mismatches\n", mism);
+}
+
+// rigid rotation curl = 2 omega; pure shear Q = 0 and w_z = -gamma, as
+// tests/unit/test_field_analytics.cpp:8-80
+static void analytics() {
+  const int NX = 16, NY = 16;
+  const real_t om = 0.05f;
+  LatticeGrid2D g2(NX, NY);
+  g2.initialize_custom([&](int x, int y, real_t& rho, real_t& ux, real_t& uy) {
+    rho = 1.0f;
+    ux = -om * (static_cast<real_t>(y) - 7.5f);
+    uy = om * (static_cast<real_t>(x) - 7.5f);
+  });
+  AlignedBuffer<real_t> w;
+  FieldAnalytics::compute_vorticity(g2, w);
+  EXPECT(w.size() == static_cast<std::size_t>(NX * NY), "vorticity size");
+  real_t worst = 0.0f;
+  for (int y = 2; y < NY - 2; ++y)
+    for (int x = 2; x < NX - 2; ++x) worst = std::fmax(worst, std::fabs(w[y * NX + x] - 2.0f * om));
+  EXPECT(worst <= 1e-5f, "rigid rotation curl off by %g", worst);
+  // the maximum includes the periodic seam, where the rotation field jumps
+  real_t wmax = 0.0f;
+  for (std::size_t i = 0; i < w.size(); ++i) wmax = std::fmax(wmax, std::fabs(w[i]));
+  EXPECT(FieldAnalytics::compute_max_vorticity_2d(w) == wmax && wmax >= 2.0f * om, "max vorticity %g vs %g",
+         FieldAnalytics::compute_max_vorticity_2d(w), wmax);
+
+  const int L = 16;
+  const real_t gam = 0.04f;
+  LatticeGrid3D g3(L, L, L);
+  g3.initialize_custom([&](int, int y, int, real_t& rho, real_t& ux, real_t& uy, real_t& uz) {
+    rho = 1.0f;
+    ux = gam * static_cast<real_t>(y);
+    uy = uz = 0.0f;
+  });
+  AlignedBuffer<real_t> wx, wy, wz, q;
+  FieldAnalytics::compute_3d_features(g3, wx, wy, wz, q);
+  real_t wq = 0.0f, ww = 0.0f;
+  for (int z = 2; z < L - 2; ++z)
+    for (int y = 2; y < L - 2; ++y)
+      for (int x = 2; x < L - 2; ++x) {
+        const int i = (z * L + y) * L + x;
+        wq = std::fmax(wq, std::fabs(q[i]));
+        ww = std::fmax(ww, std::fabs(wz[i] + gam) + std::fabs(wx[i]) + std::fabs(wy[i]));
+      }
+  EXPECT(wq <= 1e-5f && ww <= 1e-5f, "pure shear: |Q| %g, vorticity off by %g", wq, ww);
+  std::printf("analytics: curl error %g, shear Q %g\n", worst, wq);
+}
+
+// 0.025 Hz sinusoid plus a 0.1 Hz harmonic over 512 samples, as
+// tests/unit/test_forces_momentum_exchange.cpp:70-100
+static void strouhal() {
+  std::vector<real_t> sig(512);
+  for (std::size_t i = 0; i < sig.size(); ++i) {
+    const real_t t = static_cast<real_t>(i);
+    sig[i] = 0.35f * std::sin(TWO_PI * 0.025f * t) + 0.02f * std::cos(TWO_PI * 0.1f * t);
+  }
+  StrouhalAnalyzer an;
+  const real_t f = an.analyze_frequency(sig, 1.0f);
+  EXPECT(std::fabs(f - 0.025f) <= 0.003f, "Strouhal frequency %g", f);
+  EXPECT(std::fabs(f - 13.0f / 512.0f) < 1e-7f, "peak bin %g (13/512 expected)", f);
+  const real_t st = an.compute_strouhal_number(f, 20.0f, 0.1f);
+  EXPECT(std::fabs(st - f * 200.0f) < 1e-5f, "St %g", st);
+  EXPECT(an.frequencies().size() == 256 && an.power_spectrum().size() == 256, "spectrum size %zu",
+         an.frequencies().size());
+  std::printf("strouhal: f = %g, St = %g\n", f, st);
+}
+
 int main() {
   try {
+    geometry();
+    obj_mesh();
+    analytics();
+    strouhal();
     taylor_green_3d();
     poiseuille_2d();
     sphere_forces();

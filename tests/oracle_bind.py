@@ -78,6 +78,10 @@ def lib(bits=32):
     L.oracle_rng_splitmix64.argtypes = [ctypes.c_uint64]
     L.oracle_rng_u01.restype = ctypes.c_double
     L.oracle_rng_u01.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+    L.oracle_tri_box_overlap.argtypes = [P, P, P]
+    L.oracle_tri_box_overlap.restype = I
+    L.oracle_voxelize_mesh.argtypes = [P, I, I, I, P, ctypes.c_longlong, I]
+    L.oracle_voxelize_mesh.restype = ctypes.c_longlong
     L.real = real
     L.np_real = np.float64 if bits == 64 else np.float32
     _libs[bits] = L
@@ -104,6 +108,25 @@ def call(bits, name, *args):
         else:
             cargs.append(a)
     return fn(*cargs)
+
+
+def tri_box_overlap(tri, center, half, bits=32):
+    """oracle_tri_box_overlap: triangle (3x3 vertices) against a box."""
+    L = lib(bits)
+    t = np.ascontiguousarray(np.asarray(tri, L.np_real).reshape(9))
+    c = np.ascontiguousarray(np.asarray(center, L.np_real).reshape(3))
+    h = np.ascontiguousarray(np.asarray(half, L.np_real).reshape(3))
+    return bool(L.oracle_tri_box_overlap(ptr(t), ptr(c), ptr(h)))
+
+
+def voxelize_mesh(flags, tris, fill_interior=True, bits=32):
+    """oracle_voxelize_mesh into `flags` ((nz, ny, nx) uint8, in place);
+    returns the number of Solid writes."""
+    L = lib(bits)
+    assert flags.dtype == np.uint8 and flags.ndim == 3 and flags.flags.c_contiguous
+    t = np.ascontiguousarray(np.asarray(tris, L.np_real).reshape(-1, 9))
+    nz, ny, nx = flags.shape
+    return int(L.oracle_voxelize_mesh(ptr(flags), nx, ny, nz, ptr(t), t.shape[0], 1 if fill_interior else 0))
 
 
 COLL = {"BGK": 0, "TRT": 1, "MRT": 2}

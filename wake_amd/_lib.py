@@ -72,6 +72,8 @@ DOMAIN_SIGS = {
     "wake_domain_set_solid_sphere": ([P, F, F, F, F], I),
     "wake_domain_set_solid_ellipsoid": ([P, F, F, F, F, F, F, P], I),
     "wake_domain_set_solid_naca": ([P, I, F, F, F, F, I, I], I),
+    "wake_domain_voxelize_mesh": ([P, P, LL, I], I),
+    "wake_domain_is_periodic": ([P, P], I),
     "wake_domain_set_solid_circle": ([P, F, F, F], I),
     "wake_domain_set_solid_cylinder_z": ([P, F, F, F], I),
     "wake_domain_set_moving_wall_top": ([P, F, F], I),

@@ -34,7 +34,10 @@ HOST_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-ffp-contract=off", f"--offload-arc
 SOURCES = [
     "step_d3q19.hip", "step_d3q27.hip", "step_d2q9.hip", "step_kernels.hip", "thermal.hip", "analytics.hip",
     "ispc_compat.hip", "batch.hip", "domain.cpp", "capi.cpp", "lattice_grid.cpp", "runner.cpp", "batch_host.cpp",
+    "host_api.cpp", "geometry.hip",
 ]
+# The voxelizer's overlap test must round every product and sum on its own.
+EXTRA_FLAGS = {"geometry.hip": ["-ffp-contract=off"]}
 
 
 def _headers_mtime():
@@ -50,7 +53,7 @@ def _compile(src, force, hdr_mtime):
         om = os.path.getmtime(obj)
         if om >= os.path.getmtime(path) and om >= hdr_mtime:
             return obj, "up to date"
-    flags = DEVICE_FLAGS if src.endswith(".hip") else HOST_FLAGS
+    flags = (DEVICE_FLAGS if src.endswith(".hip") else HOST_FLAGS) + EXTRA_FLAGS.get(src, [])
     cmd = ["hipcc", *flags, "-I", os.path.join(ROOT, "include"), "-I", CSRC, "-c", path, "-o", obj]
     if src.endswith(".cpp"):
         cmd[1:1] = ["-x", "hip"]

@@ -132,6 +132,9 @@ int wake_domain_set_solid_naca(wake_domain* d, int digits, float chord, float ao
                                int z0, int z1) {
   return guard(d, [&](wake_hip::Domain& g) { g.set_solid_naca(digits, chord, aoa_deg, lead_x, lead_y, z0, z1); });
 }
+int wake_domain_voxelize_mesh(wake_domain* d, const float* tris, long long ntri, int fill_interior) {
+  return guard(d, [&](wake_hip::Domain& g) { g.voxelize_mesh(tris, ntri, fill_interior != 0); });
+}
 int wake_domain_set_solid_circle(wake_domain* d, float cx, float cy, float r) {
   return guard(d, [&](wake_hip::Domain& g) { g.set_solid_circle(cx, cy, r); });
 }
@@ -250,6 +253,9 @@ int wake_domain_info(wake_domain* d, long long out[8]) {
     out[6] = g.storage();
     out[7] = static_cast<long long>(g.device_bytes());
   });
+}
+int wake_domain_is_periodic(wake_domain* d, int* out) {
+  return guard(d, [&](wake_hip::Domain& g) { *out = g.is_periodic() ? 1 : 0; });
 }
 int wake_domain_set_block_x(wake_domain* d, int block_x) {
   return guard(d, [&](wake_hip::Domain& g) { g.set_block_x(block_x); });

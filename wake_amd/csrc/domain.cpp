@@ -394,6 +394,27 @@ void Domain::set_solid_ellipsoid(float cx, float cy, float cz, float rx, float r
       }
 }
 
+// obj_loader.cpp:208-400 on the device.  The host mirror of the cell types
+// is refreshed from the device afterwards; set_cell_type's rule applies: a
+// Solid write clears the all-periodic mode.
+void Domain::voxelize_mesh(const float* tris, long long ntri, bool fill_interior) {
+  if (ntri < 0 || (ntri > 0 && !tris)) throw HipError("voxelize_mesh: bad triangle array");
+  for (long long i = 0; i < 9 * ntri; ++i)
+    if (!std::isfinite(tris[i])) throw HipError("voxelize_mesh: mesh vertices must be finite");
+  before_config_change();
+  if (flags_dirty_) {
+    check(hipMemcpyAsync(d_flags_, flags_.data(), N_, hipMemcpyHostToDevice, stream_), "upload flags");
+    flags_dirty_ = false;
+  }
+  bool any = false;
+  check(launch_voxelize_mesh(tris, ntri, fill_interior, d_flags_, nx_, ny_, nz_, stream_, &any), "voxelize mesh");
+  check(hipMemcpyAsync(flags_.data(), d_flags_, N_, hipMemcpyDeviceToHost, stream_), "download flags");
+  check(hipStreamSynchronize(stream_), "voxelize mesh");
+  if (any) is_periodic_ = false;
+  cls_dirty_ = true;
+  ++flags_version_;
+}
+
 // lattice_grid_3d.cpp:160-182 (FP32 ball test, bit-exact flags)
 void Domain::set_solid_sphere(float cx, float cy, float cz, float radius) {
   before_config_change();

@@ -65,6 +65,9 @@ class Domain {
   // NACA 4-digit airfoil (digits: e.g. 12 for 0012), leading edge at
   // (lead_x, lead_y), extruded over z0..z1 in 3D.
   void set_solid_naca(int digits, float chord, float aoa_deg, float lead_x, float lead_y, int z0, int z1);
+  // Triangle mesh (ntri triangles of 3 xyz vertices) voxelized on the
+  // device: surface cells, plus the enclosed cells with fill_interior.
+  void voxelize_mesh(const float* tris, long long ntri, bool fill_interior);
   void set_solid_circle(float cx, float cy, float r);
   void set_solid_cylinder_z(float cx, float cy, float r);
   void set_moving_wall_top(float uwx, float uwz);

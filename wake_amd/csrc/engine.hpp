@@ -119,6 +119,12 @@ hipError_t launch_mem_scan(int lattice, int storage, bool from_s, const StepArgs
 hipError_t launch_vorticity_2d(const float* ux, const float* uy, float* w, int nx, int ny, hipStream_t s);
 hipError_t launch_vorticity_q_3d(const float* ux, const float* uy, const float* uz, float* ox, float* oy, float* oz,
                                  float* q, int nx, int ny, int nz, hipStream_t s);
+// Triangle mesh (ntri x 3 vertices x xyz, host memory) into the device
+// cell types: cells whose unit box overlaps a triangle become Solid, and
+// with fill_interior so do cells the outer faces cannot reach through
+// non-Solid cells (geometry.hip).  any_set: some cell was set Solid.
+hipError_t launch_voxelize_mesh(const float* tris, long long ntri, bool fill_interior, uint8_t* d_flags, int nx,
+                                int ny, int nz, hipStream_t s, bool* any_set);
 // FP64 sums over Fluid nodes of rho and rho |u|^2 / 2, and max |u|.
 hipError_t launch_diagnostics(const uint8_t* flags, const float* rho, const float* speed, long long n,
                               double* partial, int nblocks, double* out3, hipStream_t s);

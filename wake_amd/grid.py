@@ -89,6 +89,11 @@ class _Domain:
     def current_step(self):
         return self.info()[5]
 
+    def is_periodic(self):
+        out = ctypes.c_int(0)
+        self._c(self._L.wake_domain_is_periodic(self._h, ctypes.byref(out)), "is_periodic")
+        return bool(out.value)
+
     def plane_stride(self):
         return self.info()[4]
 
@@ -356,6 +361,13 @@ class LatticeGrid3D(_Domain):
         a = None if axes is None else np.ascontiguousarray(np.asarray(axes, np.float32).reshape(9))
         self._c(self._L.wake_domain_set_solid_ellipsoid(self._h, cx, cy, cz, rx, ry, rz, _lib.ptr(a)),
                 "set_solid_ellipsoid")
+
+    def voxelize_mesh(self, triangles, fill_interior=True):
+        """obj_loader.cpp:208-400 voxelize_into_grid on the device.
+        triangles: (ntri, 3, 3) vertex coordinates in grid units."""
+        t = np.ascontiguousarray(np.asarray(triangles, np.float32).reshape(-1, 9))
+        self._c(self._L.wake_domain_voxelize_mesh(self._h, _lib.ptr(t), t.shape[0], 1 if fill_interior else 0),
+                "voxelize_mesh")
 
     def set_solid_cylinder_z(self, cx, cy, radius):
         self._c(self._L.wake_domain_set_solid_cylinder_z(self._h, cx, cy, radius), "set_solid_cylinder_z")
