@@ -2,7 +2,8 @@
 oracle: the cell types after LatticeGrid3D.voxelize_mesh equal the oracle's
 bit for bit, for closed and open meshes, with and without the interior
 fill, over existing cell types, at 256^3, and the grid then steps like the
-oracle grid with the same cell types.
+oracle grid with the same cell types.  The device BFL link builder
+(lattice_grid_3d.cpp:217-290) gives the oracle's links bit for bit.
 """
 import time
 
@@ -135,3 +136,49 @@ def test_steps_like_oracle_after_voxelize(wa):
     vel = lambda a, b, c: np.concatenate([a[fl], b[fl], c[fl]])
     check_fp32_parity(vel(g.ux_data(), g.uy_data(), g.uz_data()), vel(o.ux(), o.uy(), o.uz()),
                       vel(o64.ux(), o64.uy(), o64.uz()), "mesh channel u")
+
+
+@pytest.mark.parametrize("lattice,shape,per", [(19, (72, 40, 36), (True, True)), (27, (48, 36, 30), (False, True)),
+                                               (19, (40, 44, 38), (False, False))])
+def test_device_link_builder_bit_exact(wa, lattice, shape, per):
+    """lattice_grid_3d.cpp:217-290 on the device: the same links, order and
+    wall fractions (bit for bit) as the oracle, with periodic axes and a
+    sphere cut by the domain edge."""
+    nx, ny, nz = shape
+    c, R = (20.3, ny - 6.6, nz / 2 - 0.4), 9.7
+    g = wa.LatticeGrid3D(nx, ny, nz, 0.6, lattice=lattice)
+    o = ob.OracleGrid(3, nx, ny, nz, tau=0.6, lattice=lattice, bits=32)
+    for t in (g, o):
+        t.set_periodic_axes(*per)
+        t.set_solid_sphere(*c, R)
+    assert g.build_curved_sphere_links(*c, R) == o.build_curved_sphere_links(*c, R) > 0
+    gi, gq, gd = g.curved_links()
+    oi, oq, od = o.links()
+    assert np.array_equal(gi, oi) and np.array_equal(gq, oq)
+    assert np.array_equal(gd.view(np.uint32), od.view(np.uint32))
+
+
+def test_device_link_builder_sphere_256(wa):
+    """The 256^3 sphere workload's links (R = 24 at (64, 128, 128), periodic
+    y/z): equal to the oracle's, built in milliseconds."""
+    n, c, R = 256, (64.0, 128.0, 128.0), 24.0
+    g = wa.LatticeGrid3D(n, n, n, 0.505)
+    g.set_periodic_axes(True, True)
+    g.set_solid_sphere(*c, R)
+    t0 = time.perf_counter()
+    ng = g.build_curved_sphere_links(*c, R)
+    dt = time.perf_counter() - t0
+    flags = g.cell_type_data().reshape(n, n, n)
+    o = ob.OracleGrid(3, n, n, n, tau=0.505, bits=32)
+    o.set_periodic_axes(True, True)
+    o.set_solid_sphere(*c, R)
+    assert np.array_equal(o.cell_type().reshape(n, n, n), flags)
+    t0 = time.perf_counter()
+    no = o.build_curved_sphere_links(*c, R)
+    dto = time.perf_counter() - t0
+    print(f"256^3 sphere links: {ng}; device {dt * 1e3:.1f} ms, oracle host scan {dto * 1e3:.0f} ms")
+    assert ng == no
+    gi, gq, gd = g.curved_links()
+    oi, oq, od = o.links()
+    assert np.array_equal(gi, oi) and np.array_equal(gq, oq)
+    assert np.array_equal(gd.view(np.uint32), od.view(np.uint32))

@@ -36,8 +36,9 @@ SOURCES = [
     "ispc_compat.hip", "batch.hip", "domain.cpp", "capi.cpp", "lattice_grid.cpp", "runner.cpp", "batch_host.cpp",
     "host_api.cpp", "geometry.hip",
 ]
-# The voxelizer's overlap test must round every product and sum on its own.
-EXTRA_FLAGS = {"geometry.hip": ["-ffp-contract=off"]}
+# The voxelizer's overlap test and the link builder's wall fractions must
+# round every product and sum on its own, with IEEE division and sqrt.
+EXTRA_FLAGS = {"geometry.hip": ["-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt"]}
 
 
 def _headers_mtime():

@@ -521,47 +521,23 @@ void Domain::set_convective_outflow_east(float u_conv) {
 
 // lattice_grid_3d.cpp:217-290: links in z -> y -> x -> q order from Fluid
 // nodes to Solid neighbours (x periodic, y/z in range or wrapped on a
-// periodic axis), FP32 ray-sphere quadratic, delta clamped to [0.01, 0.99].
+// periodic axis), FP32 ray-sphere quadratic, delta clamped to [0.01, 0.99];
+// built on the device from the device cell types (geometry.hip).
 int Domain::build_curved_sphere_links(float cx, float cy, float cz, float radius) {
   before_config_change();
   is_periodic_ = false;
-  const LatTab L = tab(lattice_);
-  const bool py = ext_axes_ && per_y_, pz = ext_axes_ && per_z_;
+  if (flags_dirty_) {
+    check(hipMemcpyAsync(d_flags_, flags_.data(), N_, hipMemcpyHostToDevice, stream_), "upload flags");
+    flags_dirty_ = false;
+    cls_dirty_ = true;
+  }
   CurvedLinks k;
-  for (int z = 0; z < nz_; ++z)
-    for (int y = 0; y < ny_; ++y)
-      for (int x = 0; x < nx_; ++x) {
-        if (get_cell_type(x, y, z) != kFluid) continue;
-        const int idx_f = (z * ny_ + y) * nx_ + x;
-        for (int q = 1; q < L.Q; ++q) {
-          const int nxx = (x + L.cx[q] + nx_) % nx_;
-          int nyy = y + L.cy[q], nzz = z + L.cz[q];
-          if (py) nyy = (nyy + ny_) % ny_;
-          if (pz) nzz = (nzz + nz_) % nz_;
-          if (nyy < 0 || nyy >= ny_ || nzz < 0 || nzz >= nz_) continue;
-          if (get_cell_type(nxx, nyy, nzz) != kSolid) continue;
-          const float p0x = static_cast<float>(x) - cx, p0y = static_cast<float>(y) - cy, p0z = static_cast<float>(z) - cz;
-          const float dx = static_cast<float>(L.cx[q]), dy = static_cast<float>(L.cy[q]), dz = static_cast<float>(L.cz[q]);
-          const float a = dx * dx + dy * dy + dz * dz;
-          const float b = 2.0f * (p0x * dx + p0y * dy + p0z * dz);
-          const float c = (p0x * p0x + p0y * p0y + p0z * p0z) - radius * radius;
-          const float disc = b * b - 4.0f * a * c;
-          float delta = 0.5f;
-          if (disc >= 0.0f) {
-            const float t1 = (-b - std::sqrt(disc)) / (2.0f * a);
-            const float t2 = (-b + std::sqrt(disc)) / (2.0f * a);
-            if (t1 >= 0.0f && t1 <= 1.0f) delta = t1;
-            else if (t2 >= 0.0f && t2 <= 1.0f) delta = t2;
-          }
-          delta = std::clamp(delta, 0.01f, 0.99f);
-          k.fluid_index.push_back(idx_f);
-          k.q_dir.push_back(q);
-          k.delta.push_back(delta);
-          k.uw_x.push_back(0.0f);
-          k.uw_y.push_back(0.0f);
-          k.uw_z.push_back(0.0f);
-        }
-      }
+  check(launch_build_sphere_links(lattice_, d_flags_, nx_, ny_, nz_, ext_axes_ && per_y_, ext_axes_ && per_z_, cx, cy,
+                                  cz, radius, stream_, k.fluid_index, k.q_dir, k.delta),
+        "build curved links");
+  k.uw_x.assign(k.fluid_index.size(), 0.0f);
+  k.uw_y.assign(k.fluid_index.size(), 0.0f);
+  k.uw_z.assign(k.fluid_index.size(), 0.0f);
   set_curved_links(k);
   return static_cast<int>(links_.fluid_index.size());
 }

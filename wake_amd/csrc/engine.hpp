@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace wake_hip {
 
@@ -125,6 +126,11 @@ hipError_t launch_vorticity_q_3d(const float* ux, const float* uy, const float* 
 // non-Solid cells (geometry.hip).  any_set: some cell was set Solid.
 hipError_t launch_voxelize_mesh(const float* tris, long long ntri, bool fill_interior, uint8_t* d_flags, int nx,
                                 int ny, int nz, hipStream_t s, bool* any_set);
+// BFL links to a sphere (centre, radius) from the device cell types, in
+// z -> y -> x -> q order, into host vectors (geometry.hip).
+hipError_t launch_build_sphere_links(int lattice, const uint8_t* d_flags, int nx, int ny, int nz, bool per_y,
+                                     bool per_z, float cx, float cy, float cz, float r, hipStream_t s,
+                                     std::vector<int>& node, std::vector<int>& qdir, std::vector<float>& delta);
 // FP64 sums over Fluid nodes of rho and rho |u|^2 / 2, and max |u|.
 hipError_t launch_diagnostics(const uint8_t* flags, const float* rho, const float* speed, long long n,
                               double* partial, int nblocks, double* out3, hipStream_t s);

@@ -1,4 +1,5 @@
-// geometry.hip -- triangle-mesh voxelizer on the device (gfx950).
+// geometry.hip -- triangle-mesh voxelizer and curved-wall link builder on
+// the device (gfx950).
 //
 // src/geometry/obj_loader.cpp:208-400: every cell whose unit box overlaps a
 // triangle (separating-axis test) becomes Solid; optionally every cell that
@@ -22,6 +23,7 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "lattice.hpp"
 
 namespace wake_hip {
 namespace {
@@ -282,6 +284,163 @@ hipError_t launch_voxelize_mesh(const float* tris, long long ntri, bool fill_int
   WAKE_TRY(hipStreamSynchronize(s));
   *any_set = any != 0;
   return hipSuccess;
+}
+
+// ------------------------------------------------------------ BFL links
+// lattice_grid_3d.cpp:217-290: a link from every Fluid node x along each
+// c_q (q >= 1) whose neighbour -- x periodic, y and z in range or wrapped on
+// a periodic axis -- is Solid; the wall fraction delta is the first root in
+// [0, 1] of |x - c + t c_q| = R (FP32, correctly rounded square root and
+// division, no contraction), else 1/2, clamped to [0.01, 0.99].  One
+// work-group per (y, z) row: a counting pass, a host prefix over rows, then
+// an emitting pass whose in-row prefix keeps the z -> y -> x -> q order.
+namespace {
+
+struct LinkArgs {
+  int nx, ny, nz, per_y, per_z;
+  float cx, cy, cz, r;
+};
+
+template <class L>
+__device__ inline bool links_to_solid(const uint8_t* __restrict__ flags, const LinkArgs& a, int x, int y, int z,
+                                      int q) {
+  const int xn = (x + L::cx[q] + a.nx) % a.nx;
+  int yn = y + L::cy[q], zn = z + L::cz[q];
+  if (a.per_y) yn = (yn + a.ny) % a.ny;
+  if (a.per_z) zn = (zn + a.nz) % a.nz;
+  if (yn < 0 || yn >= a.ny || zn < 0 || zn >= a.nz) return false;
+  return flags[(static_cast<size_t>(zn) * a.ny + yn) * a.nx + xn] == kCellSolid;
+}
+
+template <class L>
+__device__ inline int node_link_count(const uint8_t* __restrict__ flags, const LinkArgs& a, int x, int y, int z) {
+  if (flags[(static_cast<size_t>(z) * a.ny + y) * a.nx + x] != 0) return 0;  // Fluid only
+  int n = 0;
+  for (int q = 1; q < L::Q; ++q) n += links_to_solid<L>(flags, a, x, y, z, q);
+  return n;
+}
+
+template <class L>
+__device__ inline float wall_fraction(const LinkArgs& a, int x, int y, int z, int q) {
+  const float p0x = static_cast<float>(x) - a.cx, p0y = static_cast<float>(y) - a.cy,
+              p0z = static_cast<float>(z) - a.cz;
+  const float dx = static_cast<float>(L::cx[q]), dy = static_cast<float>(L::cy[q]), dz = static_cast<float>(L::cz[q]);
+  const float qa = dx * dx + dy * dy + dz * dz;
+  const float qb = 2.0f * (p0x * dx + p0y * dy + p0z * dz);
+  const float qc = (p0x * p0x + p0y * p0y + p0z * p0z) - a.r * a.r;
+  const float disc = qb * qb - 4.0f * qa * qc;
+  float delta = 0.5f;
+  if (disc >= 0.0f) {
+    // sqrtf and / are correctly rounded here (-fhip-fp32-correctly-rounded-
+    // divide-sqrt); __fsqrt_rn would be the native, not correctly rounded, root
+    const float root = sqrtf(disc);
+    const float t1 = (-qb - root) / (2.0f * qa);
+    const float t2 = (-qb + root) / (2.0f * qa);
+    if (t1 >= 0.0f && t1 <= 1.0f) delta = t1;
+    else if (t2 >= 0.0f && t2 <= 1.0f) delta = t2;
+  }
+  return delta < 0.01f ? 0.01f : (delta > 0.99f ? 0.99f : delta);
+}
+
+template <class L>
+__global__ __launch_bounds__(256) void k_link_rows(const uint8_t* __restrict__ flags, const LinkArgs a,
+                                                   int* __restrict__ row_count) {
+  __shared__ int part[256];
+  const int row = blockIdx.x, y = row % a.ny, z = row / a.ny;
+  int n = 0;
+  for (int x = threadIdx.x; x < a.nx; x += blockDim.x) n += node_link_count<L>(flags, a, x, y, z);
+  part[threadIdx.x] = n;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) part[threadIdx.x] += part[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) row_count[row] = part[0];
+}
+
+template <class L>
+__global__ __launch_bounds__(256) void k_link_emit(const uint8_t* __restrict__ flags, const LinkArgs a,
+                                                   const long long* __restrict__ row_start, int* __restrict__ node,
+                                                   int* __restrict__ qdir, float* __restrict__ delta) {
+  __shared__ int scan[256];
+  const int row = blockIdx.x, y = row % a.ny, z = row / a.ny;
+  long long base = row_start[row];
+  for (int x0 = 0; x0 < a.nx; x0 += blockDim.x) {
+    const int x = x0 + static_cast<int>(threadIdx.x);
+    const int n = x < a.nx ? node_link_count<L>(flags, a, x, y, z) : 0;
+    scan[threadIdx.x] = n;
+    __syncthreads();
+    for (int off = 1; off < static_cast<int>(blockDim.x); off <<= 1) {
+      const int v = static_cast<int>(threadIdx.x) >= off ? scan[threadIdx.x - off] : 0;
+      __syncthreads();
+      scan[threadIdx.x] += v;
+      __syncthreads();
+    }
+    const int total = scan[blockDim.x - 1];
+    if (n) {
+      long long o = base + scan[threadIdx.x] - n;
+      const int idx = (z * a.ny + y) * a.nx + x;
+      for (int q = 1; q < L::Q; ++q)
+        if (links_to_solid<L>(flags, a, x, y, z, q)) {
+          node[o] = idx;
+          qdir[o] = q;
+          delta[o] = wall_fraction<L>(a, x, y, z, q);
+          ++o;
+        }
+    }
+    base += total;
+    __syncthreads();
+  }
+}
+
+template <class L>
+hipError_t build_links(const uint8_t* d_flags, const LinkArgs& a, hipStream_t s, std::vector<int>& node,
+                       std::vector<int>& qdir, std::vector<float>& delta) {
+  const int rows = a.ny * a.nz;
+  DevBuf d_count, d_start, d_node, d_q, d_delta;
+  WAKE_TRY(d_count.alloc(rows * sizeof(int)));
+  k_link_rows<L><<<rows, 256, 0, s>>>(d_flags, a, static_cast<int*>(d_count.p));
+  WAKE_TRY(hipGetLastError());
+  std::vector<int> count(rows);
+  WAKE_TRY(hipMemcpyAsync(count.data(), d_count.p, rows * sizeof(int), hipMemcpyDeviceToHost, s));
+  WAKE_TRY(hipStreamSynchronize(s));
+  std::vector<long long> start(rows);
+  long long total = 0;
+  for (int r = 0; r < rows; ++r) {
+    start[r] = total;
+    total += count[r];
+  }
+  node.resize(total);
+  qdir.resize(total);
+  delta.resize(total);
+  if (total == 0) return hipSuccess;
+  WAKE_TRY(d_start.alloc(rows * sizeof(long long)));
+  WAKE_TRY(d_node.alloc(total * sizeof(int)));
+  WAKE_TRY(d_q.alloc(total * sizeof(int)));
+  WAKE_TRY(d_delta.alloc(total * sizeof(float)));
+  WAKE_TRY(hipMemcpyAsync(d_start.p, start.data(), rows * sizeof(long long), hipMemcpyHostToDevice, s));
+  k_link_emit<L><<<rows, 256, 0, s>>>(d_flags, a, static_cast<const long long*>(d_start.p),
+                                      static_cast<int*>(d_node.p), static_cast<int*>(d_q.p),
+                                      static_cast<float*>(d_delta.p));
+  WAKE_TRY(hipGetLastError());
+  WAKE_TRY(hipMemcpyAsync(node.data(), d_node.p, total * sizeof(int), hipMemcpyDeviceToHost, s));
+  WAKE_TRY(hipMemcpyAsync(qdir.data(), d_q.p, total * sizeof(int), hipMemcpyDeviceToHost, s));
+  WAKE_TRY(hipMemcpyAsync(delta.data(), d_delta.p, total * sizeof(float), hipMemcpyDeviceToHost, s));
+  return hipStreamSynchronize(s);
+}
+
+}  // namespace
+
+hipError_t launch_build_sphere_links(int lattice, const uint8_t* d_flags, int nx, int ny, int nz, bool per_y,
+                                     bool per_z, float cx, float cy, float cz, float r, hipStream_t s,
+                                     std::vector<int>& node, std::vector<int>& qdir, std::vector<float>& delta) {
+  const LinkArgs a{nx, ny, nz, per_y ? 1 : 0, per_z ? 1 : 0, cx, cy, cz, r};
+  switch (lattice) {
+    case 9: return build_links<D2Q9>(d_flags, a, s, node, qdir, delta);
+    case 19: return build_links<D3Q19>(d_flags, a, s, node, qdir, delta);
+    case 27: return build_links<D3Q27>(d_flags, a, s, node, qdir, delta);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 }  // namespace wake_hip
