@@ -238,6 +238,9 @@ def main_serve(args):
     lat = serve.latency_stats(s.done)
     log(f"served: {t_srv:.2f} s makespan, p50 {lat['p50_s']:.2f} s, p99 {lat['p99_s']:.2f} s, "
         f"peak HBM {s.peak_used / 1e9:.1f} GB, {s.ticks} ticks")
+    if args.results:
+        serve.write_results_jsonl(s.done, args.results)
+        log(f"per-request results written to {args.results}")
     plain, t_plain = serve.plain_loop(reqs, realtime=True)
     lat_p = serve.latency_stats(plain)
     log(f"plain:  {t_plain:.2f} s makespan, p50 {lat_p['p50_s']:.2f} s, p99 {lat_p['p99_s']:.2f} s")
@@ -405,6 +408,7 @@ def main():
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--step-scale", type=float, default=1.0)
     ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--results", default=None, help="serve: write per-request results (JSON lines) here")
     ap.add_argument("--wing-scale", type=int, default=4)
     ap.add_argument("--wing-steps", type=int, default=5000)
     args = ap.parse_args()

@@ -265,6 +265,12 @@ int wake_domain_diagnostics(wake_domain* d, double out[3]);
 int wake_domain_momentum_exchange(wake_domain* d, float cx, float cy, float cz, float r_max, double out[3]);
 /* out: nx, ny, nz, Q, plane stride, current step, storage, device bytes */
 int wake_domain_info(wake_domain* d, long long out[8]);
+/* src/export/vtk_exporter.cpp:8-191: the current fields as a VTK XML
+ * ImageData file: Velocity, Speed, Density, [with_analytics: 2D Vorticity;
+ * 3D Q_Criterion and Vorticity (3 components), computed on the device],
+ * CellType.  binary = 1: raw little-endian appended arrays with UInt64
+ * sizes; 0: the reference's ASCII layout. */
+int wake_domain_export_vti(wake_domain* d, const char* path, int binary, int with_analytics);
 /* lattice_grid_3d.hpp is_periodic(): 1 while the domain streams fully
  * periodic (no non-Fluid cell type was ever set, lattice_grid_3d.cpp:133-140). */
 int wake_domain_is_periodic(wake_domain* d, int* out);

@@ -4,12 +4,19 @@
 // every check passed.
 #include <cmath>
 #include <cstdio>
+#include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <sstream>
 #include <memory>
 #include <string>
 #include <vector>
 
 #include "wake/analytics/field_analytics.hpp"
 #include "wake/core/error.hpp"
+#include "wake/export/csv_telemetry.hpp"
+#include "wake/export/vtk_exporter.hpp"
+#include "wake/forces/aerodynamic_telemetry.hpp"
 #include "wake/forces/momentum_exchange.hpp"
 #include "wake/forces/strouhal_analyzer.hpp"
 #include "wake/geometry/naca_airfoil.hpp"
@@ -237,6 +244,80 @@ static void obj_mesh() {
   std::printf("obj_mesh: 12 triangles, %d mismatches\n", mism);
 }
 
+static std::string slurp(const std::string& path) {
+  std::ifstream in(path, std::ios::binary);
+  std::ostringstream s;
+  s << in.rdbuf();
+  return s.str();
+}
+
+// VtkExporter, the .pvd manifest and the telemetry CSV / JSON writers, in
+// the formats of src/export/vtk_exporter.cpp and csv_telemetry.cpp.
+static void exports() {
+  const std::string dir = "/tmp/wake_export_check";
+  std::filesystem::create_directories(dir);
+  const int nx = 24, ny = 16, nz = 12;
+  const std::size_t N = static_cast<std::size_t>(nx) * ny * nz;
+  LatticeGrid3D g(nx, ny, nz, 0.6f);
+  g.initialize(1.0f, 0.03f, 0.0f, 0.0f);
+  g.set_solid_sphere(8.0f, 8.0f, 6.0f, 3.0f);
+  for (int s = 0; s < 5; ++s) g.step(CollisionType::MRT);
+  AlignedBuffer<real_t> wx, wy, wz, q;
+  FieldAnalytics::compute_3d_features(g, wx, wy, wz, q);
+  std::vector<Vec3> vort(N);
+  for (std::size_t i = 0; i < N; ++i) vort[i] = Vec3(wx[i], wy[i], wz[i]);
+  EXPECT(VtkExporter::export_vti_3d(dir + "/a.vti", g, q.data(), vort.data()), "binary vti");
+  const std::string bin = slurp(dir + "/a.vti");
+  // arrays: velocity 3, speed, density, Q, vorticity 3 floats, cell type 1 byte
+  const std::size_t payload = 6 * 8 + N * (4 * (3 + 1 + 1 + 1 + 3) + 1);
+  const std::size_t at = bin.find("<AppendedData encoding=\"raw\">\n   _");
+  EXPECT(at != std::string::npos && bin.size() > payload && bin.size() - payload < at + 200, "vti size %zu",
+         bin.size());
+  float first_speed = 0.0f;  // Speed block follows the Velocity block
+  std::memcpy(&first_speed, bin.data() + bin.find('_', at) + 1 + 8 + 12 * N + 8, sizeof(float));
+  EXPECT(first_speed == g.speed(0, 0, 0), "binary Speed[0] %g vs %g", first_speed, g.speed(0, 0, 0));
+  EXPECT(VtkExporter::export_vti_3d(dir + "/a_ascii.vti", g, nullptr, nullptr, VtkFormat::Ascii), "ascii vti");
+  const std::string txt = slurp(dir + "/a_ascii.vti");
+  EXPECT(txt.find("<DataArray type=\"UInt8\" Name=\"CellType\" NumberOfComponents=\"1\" format=\"ascii\">") !=
+             std::string::npos,
+         "ascii CellType array");
+  EXPECT(VtkExporter::write_pvd_manifest(dir + "/run.pvd", "frame", 0, 20, 10, 0.5f), "pvd");
+  const std::string pvd = slurp(dir + "/run.pvd");
+  EXPECT(pvd.find("<DataSet timestep=\"5\" group=\"\" part=\"0\" file=\"frame_000010.vti\"/>") != std::string::npos,
+         "pvd entry");
+
+  AerodynamicTelemetry tel(3.0f, 1.0f);
+  const real_t cds[3] = {1.0f, 2.0f, 3.0f}, cls[3] = {0.5f, -0.5f, 0.0f};
+  TelemetryExporter ex;
+  for (int k = 0; k < 3; ++k) {
+    AerodynamicForces f;
+    f.cd = cds[k];
+    f.cl = cls[k];
+    f.fx = 0.25f * k;
+    tel.record(10 * k, 0.5f * k, f);
+    ex.record_aero({static_cast<uint64_t>(10 * k), 0.5f * k, f});
+    ex.record_performance({static_cast<uint64_t>(10 * k), 0.4, 30000.0 + k, 0.05f, 1.0f, 0.001f});
+  }
+  EXPECT(tel.compute_mean_cd() == 2.0f && tel.compute_mean_cd(1) == 2.5f, "mean cd");
+  EXPECT(std::fabs(tel.compute_rms_cl() - std::sqrt(0.5f / 3.0f)) < 1e-6f, "cl std %g", tel.compute_rms_cl());
+  tel.export_csv(dir + "/aero.csv");
+  EXPECT(slurp(dir + "/aero.csv").rfind("step,time,fx,fy,fz,cd,cl,cm\n0,0.000000,0.000000,", 0) == 0, "aero csv");
+  EXPECT(ex.write_aero_csv(dir + "/aero2.csv") && slurp(dir + "/aero2.csv") == slurp(dir + "/aero.csv"),
+         "exporter aero csv");
+  EXPECT(ex.write_performance_csv(dir + "/perf.csv") &&
+             slurp(dir + "/perf.csv").find("20,0.400000,30002.000000,0.050000,1.000000,0.001000\n") !=
+                 std::string::npos,
+         "performance csv");
+  EXPECT(ex.write_summary_json(dir + "/summary.json", "sphere", 0.2f, 0.01f), "json");
+  const std::string js = slurp(dir + "/summary.json");
+  EXPECT(js.find("\"mean_drag_coefficient\": 2,") != std::string::npos &&
+             js.find("\"average_mlups\": 30001") != std::string::npos &&
+             js.find("\"strouhal_number\": 0.2") != std::string::npos,
+         "summary json:\n%s", js.c_str());
+  EXPECT(!VtkExporter::export_vti_3d("/nonexistent_dir/x.vti", g), "unwritable path reports false");
+  std::printf("exports: %zu-byte binary vti, %zu-byte ascii vti\n", bin.size(), txt.size());
+}
+
 // rigid rotation curl = 2 omega; pure shear Q = 0 and w_z = -gamma, as
 // tests/unit/test_field_analytics.cpp:8-80
 static void analytics() {
@@ -307,6 +388,7 @@ int main() {
     geometry();
     obj_mesh();
     analytics();
+    exports();
     strouhal();
     taylor_green_3d();
     poiseuille_2d();

@@ -104,3 +104,22 @@ def test_coefficients_closed_form():
     assert float(c["cm"]) == pytest.approx(0.5 / (q * 8.0 * 2.0), rel=1e-6)
     z = an.aerodynamic_coefficients(1.0, 1.0, 0.0, 0.0, 1.0, 1e-5, 1.0)
     assert float(z["cd"]) == 0.0 and float(z["cl"]) == 0.0
+
+
+def test_serve_results_jsonl(tmp_path):
+    """Per-request result records of the serving loop (CPU: no domain)."""
+    import json
+
+    from wake_amd import serve
+
+    a = serve.SimRequest(3, "cavity2d", 64, 100, ra=1e4, tol=1e-3, arrival=0.25)
+    a.admitted, a.finished, a.result = 0.5, 1.75, {"Nu": 2.1}
+    b = serve.SimRequest(1, "bluff3d", 64, 200, re=300.0, semi_axes=(9.0, 6.0, 5.0), tol=1e-5, arrival=0.0)
+    b.admitted, b.finished, b.result = 0.0, 2.0, {"C_D": 0.9, "C_L": 0.01}
+    p = tmp_path / "results.jsonl"
+    serve.write_results_jsonl([a, b], p)
+    rows = [json.loads(line) for line in p.read_text().splitlines()]
+    assert [r["rid"] for r in rows] == [1, 3]
+    assert rows[0]["storage"] == "f32" and rows[1]["storage"] == "f16"
+    assert rows[1]["time_to_result_s"] == 1.5 and rows[1]["result"] == {"Nu": 2.1}
+    assert rows[0]["semi_axes"] == [9.0, 6.0, 5.0] and rows[1]["ra"] == 1e4

@@ -3,6 +3,7 @@
 #include <cstring>
 #include <exception>
 #include <string>
+#include <vector>
 
 #include "wake_hip.h"
 #include "capi_internal.hpp"
@@ -252,6 +253,30 @@ int wake_domain_info(wake_domain* d, long long out[8]) {
     out[5] = static_cast<long long>(g.current_step());
     out[6] = g.storage();
     out[7] = static_cast<long long>(g.device_bytes());
+  });
+}
+int wake_domain_export_vti(wake_domain* d, const char* path, int binary, int with_analytics) {
+  return guard(d, [&](wake_hip::Domain& g) {
+    if (!path) throw wake_hip::HipError("export_vti: null path");
+    const float *v2 = nullptr, *q = nullptr;
+    std::vector<float> v3;
+    if (with_analytics) {
+      if (g.dim() == 2) {
+        v2 = g.analytics(2);
+      } else {
+        const size_t n = static_cast<size_t>(g.nx()) * g.ny() * g.nz();
+        const float *ox = g.analytics(0), *oy = g.analytics(1), *oz = g.analytics(2);
+        q = g.analytics(3);
+        v3.resize(3 * n);
+        for (size_t i = 0; i < n; ++i) {
+          v3[3 * i] = ox[i];
+          v3[3 * i + 1] = oy[i];
+          v3[3 * i + 2] = oz[i];
+        }
+      }
+    }
+    if (!wake_hip::write_vti(g, path, binary != 0, v2, q, v3.empty() ? nullptr : v3.data()))
+      throw wake_hip::HipError(std::string("export_vti: cannot write ") + path);
   });
 }
 int wake_domain_is_periodic(wake_domain* d, int* out) {

@@ -268,4 +268,12 @@ inline uint64_t splitmix64(uint64_t x) {
   return x ^ (x >> 31);
 }
 
+// The domain's current fields as a VTK XML ImageData file (export.cpp):
+// Velocity, Speed, Density, the optional vorticity / Q-criterion arrays
+// (vort2d: one value per node; qcrit: one; vort3d: three, interleaved) and
+// CellType; raw appended binary or ASCII.  False if the file cannot be
+// written.
+bool write_vti(Domain& d, const std::string& path, bool binary, const float* vort2d, const float* qcrit,
+               const float* vort3d);
+
 }  // namespace wake_hip

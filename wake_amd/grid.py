@@ -89,6 +89,12 @@ class _Domain:
     def current_step(self):
         return self.info()[5]
 
+    def export_vti(self, path, binary=True, with_analytics=False):
+        """vtk_exporter.cpp:8-191: the current fields as a .vti file (raw
+        appended binary by default, or the reference's ASCII)."""
+        self._c(self._L.wake_domain_export_vti(self._h, str(path).encode(), 1 if binary else 0,
+                                               1 if with_analytics else 0), "export_vti")
+
     def is_periodic(self):
         out = ctypes.c_int(0)
         self._c(self._L.wake_domain_is_periodic(self._h, ctypes.byref(out)), "is_periodic")

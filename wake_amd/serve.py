@@ -280,6 +280,25 @@ def request_stream(count, rate=20.0, seed=5, step_scale=1.0, n3=(64, 96, 128, 16
     return reqs
 
 
+def write_results_jsonl(done, path):
+    """One JSON object per finished request: its parameters, storage,
+    arrival / admission / finish times (s), time to result and result."""
+    import json
+
+    with open(path, "w") as f:
+        for r in sorted(done, key=lambda r: r.rid):
+            rec = {"rid": r.rid, "kind": r.kind, "n": r.n, "steps": r.steps, "tol": r.tol,
+                   "storage": Storage(r.storage).name.lower(), "arrival_s": r.arrival, "admitted_s": r.admitted,
+                   "finished_s": r.finished,
+                   "time_to_result_s": None if r.finished is None else r.finished - r.arrival,
+                   "result": r.result}
+            if r.kind == "bluff3d":
+                rec.update(re=r.re, semi_axes=list(r.semi_axes))
+            else:
+                rec["ra"] = r.ra
+            f.write(json.dumps(rec) + "\n")
+
+
 def latency_stats(done):
     ttr = np.array([r.finished - r.arrival for r in done])
     return {"p50_s": float(np.percentile(ttr, 50)), "p99_s": float(np.percentile(ttr, 99)),
