@@ -14,6 +14,18 @@ using real_t = float;
 using index_t = std::size_t;
 using int_coord_t = int32_t;
 
+struct Vec2 {
+  real_t x{0.0f}, y{0.0f};
+  constexpr Vec2() = default;
+  constexpr Vec2(real_t a, real_t b) : x(a), y(b) {}
+  constexpr Vec2 operator+(const Vec2& o) const { return {x + o.x, y + o.y}; }
+  constexpr Vec2 operator-(const Vec2& o) const { return {x - o.x, y - o.y}; }
+  constexpr Vec2 operator*(real_t s) const { return {x * s, y * s}; }
+  constexpr Vec2 operator/(real_t s) const { return {x / s, y / s}; }
+  constexpr real_t dot(const Vec2& o) const { return x * o.x + y * o.y; }
+  constexpr real_t norm_sq() const { return x * x + y * y; }
+};
+
 struct Vec3 {
   real_t x{0.0f}, y{0.0f}, z{0.0f};
   constexpr Vec3() = default;

@@ -144,6 +144,23 @@ void compute_vorticity_and_q_criterion_3d(const float ux[], const float uy[], co
                                           float omega_y[], float omega_z[], float q_criterion[], int nx, int ny,
                                           int nz);
 
+/* src/ispc/thermal_kernels.ispc:4 (temperature written at non-solid nodes) */
+void thermal_collide_d2q9(const float g_in[], float g_out[], const uint8_t cell_type[], const float ux[],
+                          const float uy[], float temperature[], float tau_T, int count);
+/* src/ispc/thermal_kernels.ispc:48 */
+void thermal_compute_buoyancy_force_2d(const float temperature[], float gy_body[], float t_ref, float beta,
+                                       float g_gravity, int count);
+/* src/ispc/thermal_kernels.ispc:63 (y_wall outside [0, ny) is reported as an
+ * error instead of writing outside the array) */
+void thermal_apply_isothermal_wall(float g[], int nx, int ny, int y_wall, float t_wall);
+/* src/ispc/ibm_kernels.ispc:12 */
+void ibm_interpolate_velocity_2d(const float ux[], const float uy[], const float marker_x[], const float marker_y[],
+                                 float marker_ux[], float marker_uy[], int nx, int ny, int num_markers);
+/* src/ispc/ibm_kernels.ispc:54 (markers added in order: deterministic sums) */
+void ibm_spread_force_2d(const float marker_x[], const float marker_y[], const float marker_fx[],
+                         const float marker_fy[], const float marker_ds[], float euler_gx[], float euler_gy[], int nx,
+                         int ny, int num_markers);
+
 /* ----------------------------------------------------------------------
  * Part 2: device-resident domains (the LatticeGrid2D / LatticeGrid3D path)
  * -------------------------------------------------------------------- */
@@ -286,6 +303,27 @@ int wake_domain_set_temperature_field(wake_domain* d, const float* t);
 int wake_domain_set_wall_temperature(wake_domain* d, int x0, int y0, int x1, int y1, float t_wall);
 /* work-group width along x for the step kernel (64, 128, 256; 0 = auto) */
 int wake_domain_set_block_x(wake_domain* d, int block_x);
+
+/* The reference's D2Q9 temperature solver (src/thermal/thermal_solver.cpp,
+ * class ThermalSolver2D) with its populations in HBM.  A step
+ * (thermal_solver.cpp:83-123) collides with the fluid domain's current
+ * velocity, applies the isothermal south (y = 0) / north (y = ny - 1) walls,
+ * streams with the fluid domain's cell types and a wall velocity of zero
+ * (the reference's call leaves those two arguments undefined) and computes
+ * the per-node Boussinesq acceleration beta (T - t_ref) g_gravity.  The
+ * fluid domain must be 2D of the same size. */
+typedef struct wake_thermal2d wake_thermal2d;
+int wake_thermal2d_create(int nx, int ny, float tau_thermal, float t_ref, float beta, wake_thermal2d** out);
+int wake_thermal2d_destroy(wake_thermal2d* t);
+/* thermal_solver.cpp:49-60 */
+int wake_thermal2d_initialize(wake_thermal2d* t, float t_init);
+/* set_isothermal_wall_south / _north (a flag of 0 leaves that wall as is) */
+int wake_thermal2d_set_walls(wake_thermal2d* t, int south, float t_hot, int north, float t_cold);
+int wake_thermal2d_step(wake_thermal2d* t, wake_domain* fluid, float g_gravity);
+/* which = 0: temperature, 1: buoyancy acceleration; nx * ny floats */
+int wake_thermal2d_read(wake_thermal2d* t, int which, float* out);
+/* out = {compute_mean_temperature, compute_nusselt_number} */
+int wake_thermal2d_stats(wake_thermal2d* t, double out[2]);
 
 /* ----------------------------------------------------------------------
  * Part 3: batched stepping of many domains (no counterpart in the

@@ -2168,3 +2168,99 @@ long long oracle_voxelize_mesh(uint8_t* flags, int nx, int ny, int nz, const rea
   free(seen);
   return writes;
 }
+
+/* ------------------------------------------------------------------------
+ * The reference's D2Q9 temperature solver kernels and the IBM kernels.
+ * ---------------------------------------------------------------------- */
+
+/* thermal_kernels.ispc:4-44 */
+void oracle_thermal_collide_d2q9(const real* g_in, real* g_out, const uint8_t* cell_type, const real* ux,
+                                 const real* uy, real* temperature, real tau_T, int count) {
+  real omega_T = R(1) / tau_T;
+  for (int i = 0; i < count; ++i) {
+    if (cell_type[i] == CELL_SOLID) {
+      for (int q = 0; q < 9; ++q) g_out[(size_t)q * count + i] = g_in[(size_t)q * count + i];
+      continue;
+    }
+    real t = R(0), g[9];
+    for (int q = 0; q < 9; ++q) {
+      g[q] = g_in[(size_t)q * count + i];
+      t += g[q];
+    }
+    temperature[i] = t;
+    for (int q = 0; q < 9; ++q) {
+      real cu = (real)c9x[q] * ux[i] + (real)c9y[q] * uy[i];
+      real geq = w9[q] * t * (R(1) + R(3) * cu);
+      g_out[(size_t)q * count + i] = g[q] - omega_T * (g[q] - geq);
+    }
+  }
+}
+
+/* thermal_kernels.ispc:48-60 */
+void oracle_thermal_compute_buoyancy_force_2d(const real* temperature, real* gy_body, real t_ref, real beta,
+                                              real g_gravity, int count) {
+  for (int i = 0; i < count; ++i) gy_body[i] += beta * (temperature[i] - t_ref) * g_gravity;
+}
+
+/* thermal_kernels.ispc:63-81: in place, q = 0..8 in turn. */
+void oracle_thermal_apply_isothermal_wall(real* g, int nx, int ny, int y_wall, real t_wall) {
+  size_t n = (size_t)nx * ny;
+  for (int x = 0; x < nx; ++x) {
+    size_t idx = (size_t)y_wall * nx + x;
+    for (int q = 0; q < 9; ++q) {
+      real v = g[q * n + idx];
+      g[opp9[q] * n + idx] = -v + R(2) * w9[q] * t_wall;
+    }
+  }
+}
+
+/* ibm_kernels.ispc:4-8 */
+static real peskin_1d(real r) {
+  real a = RABS(r);
+  if (a >= R(2)) return R(0);
+  return R(0.25) * (R(1) + RCOS(R(0.5) * R(3.14159265358979323846) * a));
+}
+
+/* ibm_kernels.ispc:12-50 */
+void oracle_ibm_interpolate_velocity_2d(const real* ux, const real* uy, const real* marker_x, const real* marker_y,
+                                        real* marker_ux, real* marker_uy, int nx, int ny, int num_markers) {
+  for (int m = 0; m < num_markers; ++m) {
+    real mx = marker_x[m], my = marker_y[m], su = R(0), sv = R(0);
+    for (int iy = -2; iy <= 2; ++iy) {
+      int py = (int)RFLOOR(my) + iy;
+      if (py < 0 || py >= ny) continue;
+      real wy = peskin_1d((real)py - my);
+      for (int ix = -2; ix <= 2; ++ix) {
+        int px = (int)RFLOOR(mx) + ix;
+        int pw = (px % nx + nx) % nx;
+        real w = peskin_1d((real)px - mx) * wy;
+        su += w * ux[(size_t)py * nx + pw];
+        sv += w * uy[(size_t)py * nx + pw];
+      }
+    }
+    marker_ux[m] = su;
+    marker_uy[m] = sv;
+  }
+}
+
+/* ibm_kernels.ispc:54-96: the row weight has no |r| < 2 cutoff (:80). */
+void oracle_ibm_spread_force_2d(const real* marker_x, const real* marker_y, const real* marker_fx,
+                                const real* marker_fy, const real* marker_ds, real* euler_gx, real* euler_gy, int nx,
+                                int ny, int num_markers) {
+  for (int m = 0; m < num_markers; ++m) {
+    real mx = marker_x[m], my = marker_y[m];
+    real fx = marker_fx[m] * marker_ds[m], fy = marker_fy[m] * marker_ds[m];
+    int min_x = (int)RFLOOR(mx) - 2, max_x = (int)RFLOOR(mx) + 2;
+    int min_y = (int)RFLOOR(my) - 2, max_y = (int)RFLOOR(my) + 2;
+    for (int py = min_y; py <= max_y; ++py) {
+      if (py < 0 || py >= ny) continue;
+      real wy = R(0.25) * (R(1) + RCOS(R(0.5) * R(3.14159265358979323846) * RABS((real)py - my)));
+      for (int px = min_x; px <= max_x; ++px) {
+        size_t idx = (size_t)py * nx + (size_t)((px % nx + nx) % nx);
+        real w = peskin_1d((real)px - mx) * wy;
+        euler_gx[idx] += w * fx;
+        euler_gy[idx] += w * fy;
+      }
+    }
+  }
+}

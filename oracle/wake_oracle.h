@@ -178,6 +178,19 @@ int oracle_tri_box_overlap(const real* tri, const real* c, const real* h);
 long long oracle_voxelize_mesh(uint8_t* flags, int nx, int ny, int nz, const real* tris, long long ntri,
                                int fill_interior);
 
+/* thermal_kernels.ispc:4-81 (the reference's D2Q9 temperature solver) */
+void oracle_thermal_collide_d2q9(const real* g_in, real* g_out, const uint8_t* cell_type, const real* ux,
+                                 const real* uy, real* temperature, real tau_T, int count);
+void oracle_thermal_compute_buoyancy_force_2d(const real* temperature, real* gy_body, real t_ref, real beta,
+                                              real g_gravity, int count);
+void oracle_thermal_apply_isothermal_wall(real* g, int nx, int ny, int y_wall, real t_wall);
+/* ibm_kernels.ispc:4-96 */
+void oracle_ibm_interpolate_velocity_2d(const real* ux, const real* uy, const real* marker_x, const real* marker_y,
+                                        real* marker_ux, real* marker_uy, int nx, int ny, int num_markers);
+void oracle_ibm_spread_force_2d(const real* marker_x, const real* marker_y, const real* marker_fx,
+                                const real* marker_fy, const real* marker_ds, real* euler_gx, real* euler_gy, int nx,
+                                int ny, int num_markers);
+
 /* analytics_kernels.ispc:4-110 */
 void oracle_compute_vorticity_2d(const real* ux, const real* uy, real* w, int nx, int ny);
 void oracle_compute_vorticity_and_q_criterion_3d(const real* ux, const real* uy, const real* uz, real* ox, real* oy,

@@ -22,6 +22,8 @@
 #include "wake/geometry/naca_airfoil.hpp"
 #include "wake/geometry/obj_loader.hpp"
 #include "wake/geometry/primitives.hpp"
+#include "wake/ibm/immersed_boundary.hpp"
+#include "wake/thermal/thermal_solver.hpp"
 #include "wake/grid/lattice_grid_2d.hpp"
 #include "wake/grid/lattice_grid_3d.hpp"
 #include "wake/runner/simulation_runner.hpp"
@@ -318,6 +320,71 @@ static void exports() {
   std::printf("exports: %zu-byte binary vti, %zu-byte ascii vti\n", bin.size(), txt.size());
 }
 
+// ThermalSolver2D driven like thermal_convection.cpp:52-71: hot south wall,
+// cold north wall, fluid stepped with a buoyancy from the mean temperature;
+// a copy continues exactly like the original.
+static void thermal_solver() {
+  const int nx = 64, ny = 24;
+  LatticeGrid2D fluid(nx, ny, 0.56f);
+  fluid.initialize(RHO_0, 0.0f, 0.0f);
+  ThermalSolver2D th(nx, ny, 0.53f, 0.5f, 0.05f);
+  th.initialize(0.5f);
+  th.set_isothermal_wall_south(1.0f);
+  th.set_isothermal_wall_north(0.0f);
+  for (int s = 0; s < 50; ++s) {
+    th.step(fluid, -0.001f);
+    fluid.step(CollisionType::TRT, 0.0f, -0.0005f * 0.05f * (th.compute_mean_temperature() - 0.5f));
+  }
+  ThermalSolver2D copy = th;
+  LatticeGrid2D fluid2(nx, ny, 0.56f);
+  fluid2.initialize_custom([&](int x, int y, real_t& r, real_t& ux, real_t& uy) {
+    r = fluid.rho(x, y);
+    ux = fluid.ux(x, y);
+    uy = fluid.uy(x, y);
+  });
+  th.step(fluid, -0.001f);
+  copy.step(fluid2, -0.001f);
+  EXPECT(std::memcmp(th.temperature_data(), copy.temperature_data(), sizeof(float) * nx * ny) == 0,
+         "copied solver diverges");
+  const real_t tm = th.compute_mean_temperature(), nu = th.compute_nusselt_number();
+  EXPECT(tm > 0.3f && tm < 0.7f && std::isfinite(nu) && nu > 0.0f, "mean T %g, Nu %g", tm, nu);
+  EXPECT(th.temperature(nx / 2, 1) > th.temperature(nx / 2, ny - 2), "hot below cold");
+  ThermalSolver2D moved = std::move(copy);
+  EXPECT(moved.temperature_data() != nullptr && moved.buoyancy_data() != nullptr, "moved solver");
+  std::printf("thermal_solver: mean T %g, Nu %g\n", tm, nu);
+}
+
+// A cylinder of markers in a uniform stream: the interpolated velocity is
+// the stream (partition of unity), the body force is U sum ds, and the
+// spread force lands within two cells of the circle.
+static void immersed_boundary() {
+  const int nx = 64, ny = 48;
+  const std::size_t N = static_cast<std::size_t>(nx) * ny;
+  std::vector<real_t> ux(N, 0.05f), uy(N, 0.0f), gx(N, 0.0f), gy(N, 0.0f);
+  ImmersedBoundarySolver ibm(nx, ny);
+  ibm.add_circle(30.0f, 24.0f, 8.0f, 64);
+  ibm.add_polygon({Vec2(50.0f, 10.0f), Vec2(56.0f, 10.0f), Vec2(56.0f, 16.0f)});
+  EXPECT(ibm.num_markers() == 67, "markers %zu", ibm.num_markers());
+  ibm.update(ux.data(), uy.data(), gx.data(), gy.data(), 1.0f);
+  real_t worst = 0.0f, sum_ds = 0.0f;
+  for (const LagrangianMarker& m : ibm.markers()) {
+    worst = std::fmax(worst, std::fabs(m.fluid_vel.x - 0.05f) + std::fabs(m.fluid_vel.y));
+    sum_ds += m.ds;
+  }
+  EXPECT(worst < 1e-6f, "interpolated stream off by %g", worst);
+  const Vec3 f = ibm.compute_net_force();
+  EXPECT(std::fabs(f.x - 0.05f * sum_ds) < 1e-4f * sum_ds && std::fabs(f.y) < 1e-6f, "net force %g %g", f.x, f.y);
+  double far = 0.0, total = 0.0;
+  for (int y = 0; y < ny; ++y)
+    for (int x = 0; x < 46; ++x) {
+      const double r = std::hypot(x - 30.0, y - 24.0), v = gx[static_cast<std::size_t>(y) * nx + x];
+      total += v;
+      if (r > 8.0 + 4.0 || r < 8.0 - 4.0) far += std::fabs(v);  // stencil reach: 2 in x, 3 in y
+    }
+  EXPECT(total < 0.0 && far == 0.0, "spread force total %g, %g outside the band", total, far);
+  std::printf("immersed_boundary: net force %g (U sum ds %g)\n", f.x, 0.05f * sum_ds);
+}
+
 // rigid rotation curl = 2 omega; pure shear Q = 0 and w_z = -gamma, as
 // tests/unit/test_field_analytics.cpp:8-80
 static void analytics() {
@@ -389,6 +456,8 @@ int main() {
     obj_mesh();
     analytics();
     exports();
+    thermal_solver();
+    immersed_boundary();
     strouhal();
     taylor_green_3d();
     poiseuille_2d();

@@ -36,8 +36,9 @@ def test_library_exports_every_declared_function():
 
 
 def test_kernel_exports_match_reference_names():
-    # the 28 LBM exports of src/ispc that the hot path replaces, and the two
-    # analytics exports that read its velocity field
+    # every export of src/ispc: the 28 LBM exports the hot path replaces, the
+    # two analytics exports that read its velocity field, the three thermal
+    # D2Q9 exports and the two immersed-boundary exports
     names = ["d2q9_compute_macroscopic_moments", "d2q9_compute_macroscopic_moments_forced",
              "d2q9_compute_velocity_magnitude", "d2q9_collide_bgk", "d2q9_collide_trt", "d2q9_collide_mrt",
              "d2q9_stream_periodic", "d2q9_stream_with_boundaries", "d2q9_apply_bounce_back",
@@ -49,7 +50,11 @@ def test_kernel_exports_match_reference_names():
              "d3q19_compute_smagorinsky_tau", "d3q19_apply_curved_bfl", "d3q19_compute_momentum_exchange_forces",
              "d3q27_collide", "d3q27_stream_periodic", "d3q27_compute_macroscopic_moments",
              # analytics_kernels.ispc (vorticity and Q-criterion)
-             "compute_vorticity_2d", "compute_vorticity_and_q_criterion_3d"]
+             "compute_vorticity_2d", "compute_vorticity_and_q_criterion_3d",
+             # thermal_kernels.ispc and ibm_kernels.ispc
+             "thermal_collide_d2q9", "thermal_compute_buoyancy_force_2d", "thermal_apply_isothermal_wall",
+             "ibm_interpolate_velocity_2d", "ibm_spread_force_2d"]
+    assert len(names) == 35
     ex = exported_symbols()
     assert all(n in ex for n in names)
     import wake_amd._lib as wl

@@ -9,6 +9,7 @@ from . import _lib
 from .batch import Batch
 from .grid import (CS2, DEFAULT_SMAGORINSKY_CONSTANT, RHO_0, CellType, CollisionType, LatticeGrid2D,
                    LatticeGrid3D, Storage)
+from .thermal import ThermalSolver2D
 
 _lib.lib()  # fail loudly at import when the native library is missing
 
@@ -16,5 +17,5 @@ kernels = _lib
 
 __all__ = [
     "Batch", "CellType", "CollisionType", "LatticeGrid2D", "LatticeGrid3D", "Storage", "RHO_0", "CS2",
-    "DEFAULT_SMAGORINSKY_CONSTANT", "kernels",
+    "DEFAULT_SMAGORINSKY_CONSTANT", "ThermalSolver2D", "kernels",
 ]

@@ -52,6 +52,11 @@ KERNEL_SIGS = {
     "d3q27_compute_macroscopic_moments": "pppppi",
     "compute_vorticity_2d": "pppii",
     "compute_vorticity_and_q_criterion_3d": "pppppppiii",
+    "thermal_collide_d2q9": "ppppppfi",
+    "thermal_compute_buoyancy_force_2d": "ppfffi",
+    "thermal_apply_isothermal_wall": "piiif",
+    "ibm_interpolate_velocity_2d": "ppppppiii",
+    "ibm_spread_force_2d": "pppppppiii",
 }
 _CT = {"p": P, "i": I, "f": F}
 
@@ -107,6 +112,17 @@ DOMAIN_SIGS = {
     "wake_hip_d3q19_minv": ([P], None),
 }
 
+# Part 2: the reference's D2Q9 temperature solver
+THERMAL_SIGS = {
+    "wake_thermal2d_create": ([I, I, F, F, F, P], I),
+    "wake_thermal2d_destroy": ([P], I),
+    "wake_thermal2d_initialize": ([P, F], I),
+    "wake_thermal2d_set_walls": ([P, I, F, I, F], I),
+    "wake_thermal2d_step": ([P, P, F], I),
+    "wake_thermal2d_read": ([P, I, P], I),
+    "wake_thermal2d_stats": ([P, P], I),
+}
+
 # Part 3: batches of domains
 BATCH_SIGS = {
     "wake_batch_create": ([I, I, I, I, I, I, P], I),
@@ -139,7 +155,7 @@ def lib():
         fn = getattr(L, name)
         fn.argtypes = [_CT[c] for c in sig]
         fn.restype = None
-    for name, (args, res) in {**DOMAIN_SIGS, **BATCH_SIGS, **EXTRA_SIGS}.items():
+    for name, (args, res) in {**DOMAIN_SIGS, **THERMAL_SIGS, **BATCH_SIGS, **EXTRA_SIGS}.items():
         if not hasattr(L, name):
             continue
         fn = getattr(L, name)

@@ -259,10 +259,11 @@ void Domain::add_noise(float amplitude, uint64_t seed) {
       f[static_cast<size_t>(q) * N_ + i] += amplitude * static_cast<float>(2.0 * u - 1.0);
     }
   const uint64_t keep = step_count_;
-  const bool fv = fields_valid_;
+  const bool fv = fields_valid_, dfv = dfields_valid_;
   upload_state_from_host(f);
   step_count_ = keep;
   fields_valid_ = fv;
+  dfields_valid_ = dfv;
 }
 
 // ------------------------------------------------------------- cell types
@@ -761,8 +762,8 @@ float Domain::run(int steps, int collision, float gx, float gy, float gz, bool u
 void Domain::synchronize() { check(hipStreamSynchronize(stream_), "synchronize"); }
 
 // ------------------------------------------------------------- read-back
-void Domain::ensure_fields() {
-  if (fields_valid_) return;
+void Domain::ensure_device_fields() {
+  if (fields_valid_ || dfields_valid_) return;
   sync_device_config();
   StepArgs a = args(false);
   a.rho = d_rho_;
@@ -775,10 +776,16 @@ void Domain::ensure_fields() {
   if (thermal_) {
     a.temp = d_temp_;
     check(launch_readback_thermal(storage_, from_s, a, buf_[b], gbuf_[b], stream_), "readback");
-    check(hipMemcpyAsync(h_temp_.data(), d_temp_, N_ * 4, hipMemcpyDeviceToHost, stream_), "d2h");
   } else {
     check(launch_readback(lattice_, storage_, from_s, a, buf_[b], stream_), "readback");
   }
+  dfields_valid_ = true;
+}
+
+void Domain::ensure_fields() {
+  if (fields_valid_) return;
+  ensure_device_fields();
+  if (thermal_) check(hipMemcpyAsync(h_temp_.data(), d_temp_, N_ * 4, hipMemcpyDeviceToHost, stream_), "d2h");
   float* d[5] = {d_rho_, d_ux_, d_uy_, d_uz_, d_speed_};
   std::vector<float>* h[5] = {&h_rho_, &h_ux_, &h_uy_, &h_uz_, &h_speed_};
   for (int k = 0; k < 5; ++k) check(hipMemcpyAsync(h[k]->data(), d[k], N_ * 4, hipMemcpyDeviceToHost, stream_), "d2h");
@@ -790,7 +797,7 @@ void Domain::ensure_fields() {
 void Domain::update_macroscopic() {
   const float keep[3] = {g_last_[0], g_last_[1], g_last_[2]};
   g_last_[0] = g_last_[1] = g_last_[2] = 0.0f;
-  fields_valid_ = false;
+  fields_valid_ = dfields_valid_ = false;
   ensure_fields();
   g_last_[0] = keep[0];
   g_last_[1] = keep[1];
@@ -819,6 +826,18 @@ const float* Domain::analytics(int which) {
     an_valid_ = true;
   }
   return h_an_[which].data();
+}
+
+const float* Domain::device_field(int which) {
+  if (which < 0 || which > 4) throw HipError("unknown device field");
+  ensure_device_fields();
+  const float* d[5] = {d_rho_, d_ux_, d_uy_, d_uz_, d_speed_};
+  return d[which];
+}
+
+const uint8_t* Domain::device_cell_types() {
+  sync_device_config();
+  return d_flags_;
 }
 
 const float* Domain::rho() { ensure_fields(); return h_rho_.data(); }
@@ -992,7 +1011,7 @@ void Domain::set_temperature_field(const float* t) {
     check(hipMemcpyAsync(gbuf_[k], host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice, stream_),
           "upload temperature");
   check(hipStreamSynchronize(stream_), "upload temperature");
-  fields_valid_ = false;
+  fields_valid_ = dfields_valid_ = false;
 }
 
 void Domain::set_wall_temperature(int x0, int y0, int x1, int y1, float t_wall) {

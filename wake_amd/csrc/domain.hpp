@@ -109,6 +109,12 @@ class Domain {
   // (analytics_kernels.ispc:4-110): which = 0 omega_x, 1 omega_y, 2 omega_z
   // (the 2D vorticity), 3 Q-criterion (3D).
   const float* analytics(int which);
+  // Device arrays of the current moments (0 rho, 1 ux, 2 uy, 3 uz, 4 |u|),
+  // refreshed like the host mirrors but without the copy to the host, and
+  // the device cell types; valid until the next step or configuration
+  // change.  Work on them belongs on stream().
+  const float* device_field(int which);
+  const uint8_t* device_cell_types();
   void update_macroscopic();
   // FP64 sums over Fluid nodes of rho and rho |u|^2 / 2, and max |u|.
   void diagnostics(double out[3]);
@@ -179,7 +185,8 @@ class Domain {
   void sync_device_config();
   void ensure_fields();
   void upload_state_from_host(const std::vector<float>& f_ref);
-  void invalidate_mirrors() { fields_valid_ = fsrc_valid_ = fdst_valid_ = an_valid_ = false; }
+  void invalidate_mirrors() { fields_valid_ = dfields_valid_ = fsrc_valid_ = fdst_valid_ = an_valid_ = false; }
+  void ensure_device_fields();
   void* alloc(size_t bytes);
   void release(void* p);
   void upload_prev(const std::function<float(size_t, int)>& value);
@@ -258,6 +265,7 @@ class Domain {
   std::vector<float> h_rho_, h_ux_, h_uy_, h_uz_, h_speed_, h_fsrc_, h_fdst_;
   std::vector<float> h_prev_;  // staging of the outlet store (kept until the upload completes)
   bool fields_valid_ = false, fsrc_valid_ = false, fdst_valid_ = false;
+  bool dfields_valid_ = false;  // device moment arrays current (implied by fields_valid_)
 };
 
 // splitmix64 finaliser (same constants as the device inlet turbulence).

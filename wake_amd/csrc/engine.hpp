@@ -131,6 +131,16 @@ hipError_t launch_voxelize_mesh(const float* tris, long long ntri, bool fill_int
 hipError_t launch_build_sphere_links(int lattice, const uint8_t* d_flags, int nx, int ny, int nz, bool per_y,
                                      bool per_z, float cx, float cy, float cz, float r, hipStream_t s,
                                      std::vector<int>& node, std::vector<int>& qdir, std::vector<float>& delta);
+// D2Q9 temperature populations of the reference's thermal solver
+// (thermal_kernels.ispc:4-81) and the D2Q9 wall streaming
+// (d2q9_streaming.ispc:33-95) on device pointers (ispc_compat.hip).
+hipError_t launch_thermal_collide_d2q9(const float* g_in, float* g_out, const uint8_t* ct, const float* ux,
+                                       const float* uy, float* temp, float tau_t, int count, hipStream_t s);
+hipError_t launch_thermal_buoyancy_2d(const float* temp, float* gy, float t_ref, float beta, float g_gravity,
+                                      int count, hipStream_t s);
+hipError_t launch_thermal_isothermal_wall(float* g, int nx, int ny, int y_wall, float t_wall, hipStream_t s);
+hipError_t launch_stream_walls_d2q9(const float* src, float* dst, const uint8_t* ct, int nx, int ny, float uwx,
+                                    float uwy, hipStream_t s);
 // FP64 sums over Fluid nodes of rho and rho |u|^2 / 2, and max |u|.
 hipError_t launch_diagnostics(const uint8_t* flags, const float* rho, const float* speed, long long n,
                               double* partial, int nblocks, double* out3, hipStream_t s);

@@ -20,6 +20,8 @@
 #include "wake/geometry/naca_airfoil.hpp"
 #include "wake/geometry/obj_loader.hpp"
 #include "wake/geometry/primitives.hpp"
+#include "wake/ibm/immersed_boundary.hpp"
+#include "wake_hip.h"
 
 namespace wake {
 
@@ -33,6 +35,86 @@ auto guarded(F&& f) -> decltype(f()) {
   }
 }
 }  // namespace
+
+// ------------------------------------------------------- immersed boundary
+// immersed_boundary.cpp:22-128; the two kernels are the ISPC-named exports.
+namespace {
+void ibm_check(const char* what) {
+  if (wake_hip_clear_error() != 0) throw WakeException(std::string(what) + ": " + wake_hip_last_error());
+}
+}  // namespace
+
+ImmersedBoundarySolver::ImmersedBoundarySolver(int nx, int ny) : nx_(nx), ny_(ny) {}
+
+// immersed_boundary.cpp:25-39: num_points markers on the circle, ds = R dtheta
+void ImmersedBoundarySolver::add_circle(real_t cx, real_t cy, real_t radius, int num_points) {
+  if (num_points < 3) return;
+  const real_t dth = TWO_PI / static_cast<real_t>(num_points);
+  for (int i = 0; i < num_points; ++i) {
+    const real_t th = static_cast<real_t>(i) * dth;
+    LagrangianMarker m;
+    m.pos = Vec3(cx + radius * std::cos(th), cy + radius * std::sin(th), 0.0f);
+    m.ds = radius * dth;
+    markers_.push_back(m);
+  }
+  sync_buffers();
+}
+
+// immersed_boundary.cpp:41-57: one marker per vertex, ds = length of the
+// edge to the next vertex (closed polygon)
+void ImmersedBoundarySolver::add_polygon(const std::vector<Vec2>& points) {
+  if (points.size() < 2) return;
+  for (std::size_t i = 0; i < points.size(); ++i) {
+    const Vec2 d = points[(i + 1) % points.size()] - points[i];
+    LagrangianMarker m;
+    m.pos = Vec3(points[i].x, points[i].y, 0.0f);
+    m.ds = std::sqrt(d.norm_sq());
+    markers_.push_back(m);
+  }
+  sync_buffers();
+}
+
+void ImmersedBoundarySolver::sync_buffers() {
+  const std::size_t n = markers_.size();
+  for (auto* b : {&marker_x_, &marker_y_, &marker_ux_, &marker_uy_, &marker_fx_, &marker_fy_, &marker_ds_}) b->resize(n);
+  for (std::size_t i = 0; i < n; ++i) {
+    marker_x_[i] = markers_[i].pos.x;
+    marker_y_[i] = markers_[i].pos.y;
+    marker_ds_[i] = markers_[i].ds;
+  }
+}
+
+void ImmersedBoundarySolver::update(const real_t* euler_ux, const real_t* euler_uy, real_t* euler_gx,
+                                    real_t* euler_gy, real_t dt) {
+  const int n = static_cast<int>(markers_.size());
+  if (n == 0) return;
+  (void)wake_hip_clear_error();
+  ibm_interpolate_velocity_2d(euler_ux, euler_uy, marker_x_.data(), marker_y_.data(), marker_ux_.data(),
+                              marker_uy_.data(), nx_, ny_, n);
+  ibm_check("ibm_interpolate_velocity_2d");
+  const real_t inv_dt = 1.0f / dt;
+  for (int i = 0; i < n; ++i) {
+    LagrangianMarker& m = markers_[i];
+    m.fluid_vel.x = marker_ux_[i];
+    m.fluid_vel.y = marker_uy_[i];
+    marker_fx_[i] = (m.target_vel.x - marker_ux_[i]) * inv_dt;
+    marker_fy_[i] = (m.target_vel.y - marker_uy_[i]) * inv_dt;
+    m.force.x = marker_fx_[i] * m.ds;
+    m.force.y = marker_fy_[i] * m.ds;
+  }
+  ibm_spread_force_2d(marker_x_.data(), marker_y_.data(), marker_fx_.data(), marker_fy_.data(), marker_ds_.data(),
+                      euler_gx, euler_gy, nx_, ny_, n);
+  ibm_check("ibm_spread_force_2d");
+}
+
+Vec3 ImmersedBoundarySolver::compute_net_force() const {
+  Vec3 f(0.0f, 0.0f, 0.0f);
+  for (const LagrangianMarker& m : markers_) {
+    f.x -= m.force.x;
+    f.y -= m.force.y;
+  }
+  return f;
+}
 
 // ------------------------------------------------------------- OBJ meshes
 // obj_loader.cpp:11-19

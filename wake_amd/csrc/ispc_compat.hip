@@ -8,6 +8,7 @@
 // per-node arithmetic is the shared operator set of node_ops.hpp.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -117,6 +118,16 @@ class Call {
 };
 
 inline unsigned blocks(long long n) { return static_cast<unsigned>((n + 255) / 256); }
+
+// Marker positions must be finite and their stencils addressable (the
+// kernels take floor() of them as int); the grid must be non-empty.
+bool ibm_markers_ok(const float* mx, const float* my, int nx, int ny, int nm, const char* name) {
+  bool ok = mx && my && nx > 0 && ny > 0;
+  for (int m = 0; ok && m < nm; ++m)
+    ok = std::isfinite(mx[m]) && std::isfinite(my[m]) && std::fabs(mx[m]) < 1e8f && std::fabs(my[m]) < 1e8f;
+  if (!ok) record_error(std::string("wake: ") + name + ": marker positions must be finite and the grid non-empty");
+  return ok;
+}
 
 // ----------------------------------------------------------------- kernels
 template <class L, bool FORCED>
@@ -502,7 +513,150 @@ void les_call(const char* name, const float* f, const float* rho, const float* u
   c.end();
 }
 
+// ------------------------------------------- thermal D2Q9 and IBM kernels
+// thermal_kernels.ispc:4-44: BGK of the temperature populations g with the
+// first-order equilibrium w_q T (1 + 3 c_q.u); T = sum g is written out.
+// Solid nodes copy g.
+__global__ __launch_bounds__(256) void k_thermal_collide_d2q9(const float* __restrict__ g_in, float* __restrict__ g_out,
+                                                              const uint8_t* __restrict__ ct,
+                                                              const float* __restrict__ ux,
+                                                              const float* __restrict__ uy, float* __restrict__ temp,
+                                                              float omega, int count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const size_t n = static_cast<size_t>(count);
+  if (ct[i] == 1) {
+    for (int q = 0; q < 9; ++q) g_out[q * n + i] = g_in[q * n + i];
+    return;
+  }
+  float g[9], t = 0.0f;
+  for (int q = 0; q < 9; ++q) {
+    g[q] = g_in[q * n + i];
+    t += g[q];
+  }
+  temp[i] = t;
+  const float vx = ux[i], vy = uy[i];
+  for (int q = 0; q < 9; ++q) {
+    const float cu = static_cast<float>(D2Q9::cx[q]) * vx + static_cast<float>(D2Q9::cy[q]) * vy;
+    const float geq = D2Q9::w[q] * t * (1.0f + 3.0f * cu);
+    g_out[q * n + i] = g[q] - omega * (g[q] - geq);
+  }
+}
+
+// thermal_kernels.ispc:48-60
+__global__ __launch_bounds__(256) void k_thermal_buoyancy_2d(const float* __restrict__ temp, float* __restrict__ gy,
+                                                             float t_ref, float beta, float g_gravity, int count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) gy[i] += beta * (temp[i] - t_ref) * g_gravity;
+}
+
+// thermal_kernels.ispc:63-81: on row y_wall, for q = 0..8 in turn,
+// g[opp q] = -g[q] + 2 w_q T_wall in place -- later q read what earlier q
+// wrote, so g_1, g_2, g_5, g_6 come back to (almost) their old values and
+// only g_0 and the opposite directions change.
+__global__ __launch_bounds__(256) void k_thermal_isothermal_wall(float* __restrict__ g, int nx, int ny, int y_wall,
+                                                                 float t_wall) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= nx) return;
+  const size_t n = static_cast<size_t>(nx) * ny, idx = static_cast<size_t>(y_wall) * nx + x;
+  for (int q = 0; q < 9; ++q) {
+    const float v = g[q * n + idx];
+    g[D2Q9::opp_ref[q] * n + idx] = -v + 2.0f * D2Q9::w[q] * t_wall;
+  }
+}
+
+// ibm_kernels.ispc:4-8: the regularised 4-point cosine delta
+__device__ inline float peskin_1d(float r) {
+  const float a = fabsf(r);
+  if (a >= 2.0f) return 0.0f;
+  return 0.25f * (1.0f + cosf(0.5f * 3.14159265358979323846f * a));
+}
+
+// ibm_kernels.ispc:12-50: one lane per marker, 5 x 5 stencil from
+// floor(x) - 2, x periodic, rows outside [0, ny) skipped.
+__global__ __launch_bounds__(256) void k_ibm_interpolate_2d(const float* __restrict__ ux, const float* __restrict__ uy,
+                                                            const float* __restrict__ mx, const float* __restrict__ my,
+                                                            float* __restrict__ mux, float* __restrict__ muy, int nx,
+                                                            int ny, int nm) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nm) return;
+  const float x = mx[m], y = my[m];
+  const int fx = static_cast<int>(floorf(x)), fy = static_cast<int>(floorf(y));
+  float su = 0.0f, sv = 0.0f;
+  for (int iy = -2; iy <= 2; ++iy) {
+    const int py = fy + iy;
+    if (py < 0 || py >= ny) continue;
+    const float wy = peskin_1d(static_cast<float>(py) - y);
+    for (int ix = -2; ix <= 2; ++ix) {
+      const int px = fx + ix;
+      const int pw = (px % nx + nx) % nx;
+      const float w = peskin_1d(static_cast<float>(px) - x) * wy;
+      const size_t e = static_cast<size_t>(py) * nx + pw;
+      su += w * ux[e];
+      sv += w * uy[e];
+    }
+  }
+  mux[m] = su;
+  muy[m] = sv;
+}
+
+// ibm_kernels.ispc:54-96: markers in order, each adding its 5 x 5 stencil
+// of weight * f * ds.  The row weight is the cosine without the |r| < 2
+// cutoff (ibm_kernels.ispc:80), as there, so the row floor(y) - 2 gets a
+// weight too.  One work-group: lane l < 25 owns stencil cell l; a barrier
+// between markers keeps the additions to a cell in marker order.  With
+// nx < 5 a stencil row wraps onto itself, and lane 0 adds it alone.
+__global__ __launch_bounds__(64) void k_ibm_spread_2d(const float* __restrict__ mx, const float* __restrict__ my,
+                                                      const float* __restrict__ mfx, const float* __restrict__ mfy,
+                                                      const float* __restrict__ mds, float* __restrict__ gx,
+                                                      float* __restrict__ gy, int nx, int ny, int nm) {
+  const int l = threadIdx.x;
+  // stencil cells this lane adds: [c0, c1)
+  const int c0 = nx < 5 ? 0 : l;
+  const int c1 = nx < 5 ? (l == 0 ? 25 : 0) : (l < 25 ? l + 1 : 0);
+  for (int m = 0; m < nm; ++m) {
+    const float x = mx[m], y = my[m];
+    const float fx = mfx[m] * mds[m], fy = mfy[m] * mds[m];
+    const int x0 = static_cast<int>(floorf(x)) - 2, y0 = static_cast<int>(floorf(y)) - 2;
+    for (int c = c0; c < c1; ++c) {
+      const int py = y0 + c / 5, px = x0 + c % 5;
+      if (py < 0 || py >= ny) continue;
+      const float wy = 0.25f * (1.0f + cosf(0.5f * 3.14159265358979323846f * fabsf(static_cast<float>(py) - y)));
+      const float w = peskin_1d(static_cast<float>(px) - x) * wy;
+      const size_t e = static_cast<size_t>(py) * nx + (px % nx + nx) % nx;
+      atomicAdd(gx + e, w * fx);
+      atomicAdd(gy + e, w * fy);
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace
+
+hipError_t launch_thermal_collide_d2q9(const float* g_in, float* g_out, const uint8_t* ct, const float* ux,
+                                       const float* uy, float* temp, float tau_t, int count, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  k_thermal_collide_d2q9<<<blocks(count), 256, 0, s>>>(g_in, g_out, ct, ux, uy, temp, 1.0f / tau_t, count);
+  return hipGetLastError();
+}
+hipError_t launch_thermal_buoyancy_2d(const float* temp, float* gy, float t_ref, float beta, float g_gravity,
+                                      int count, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  k_thermal_buoyancy_2d<<<blocks(count), 256, 0, s>>>(temp, gy, t_ref, beta, g_gravity, count);
+  return hipGetLastError();
+}
+hipError_t launch_thermal_isothermal_wall(float* g, int nx, int ny, int y_wall, float t_wall, hipStream_t s) {
+  if (nx <= 0 || y_wall < 0 || y_wall >= ny) return hipErrorInvalidValue;
+  k_thermal_isothermal_wall<<<blocks(nx), 256, 0, s>>>(g, nx, ny, y_wall, t_wall);
+  return hipGetLastError();
+}
+hipError_t launch_stream_walls_d2q9(const float* src, float* dst, const uint8_t* ct, int nx, int ny, float uwx,
+                                    float uwy, hipStream_t s) {
+  const int bx = nx >= 256 ? 256 : 64;
+  k_stream_walls<D2Q9><<<dim3((nx + bx - 1) / bx, ny, 1), bx, 0, s>>>(src, dst, ct, nx, ny, 1, uwx, uwy, 0.0f);
+  return hipGetLastError();
+}
+
 }  // namespace wake_hip
 
 using namespace wake_hip;
@@ -710,6 +864,85 @@ void d3q27_stream_periodic(const float f_src[], float f_dst[], int nx, int ny, i
 void d3q27_compute_macroscopic_moments(const float f[], float rho[], float ux[], float uy[], float uz[],
                                        int count) {
   moments_call<D3Q27, false>(__func__, f, rho, ux, uy, uz, 0, 0, 0, count);
+}
+
+// thermal_kernels.ispc:4-44
+void thermal_collide_d2q9(const float g_in[], float g_out[], const uint8_t cell_type[], const float ux[],
+                          const float uy[], float temperature[], float tau_T, int count) {
+  Call c(__func__);
+  const size_t n = static_cast<size_t>(count > 0 ? count : 0);
+  auto gi = c.in(g_in, 9 * n);
+  auto go = c.out(g_out, 9 * n);
+  auto t = c.in(cell_type, n);
+  auto a = c.in(ux, n);
+  auto b = c.in(uy, n);
+  // solid nodes leave their temperature untouched: stage the caller's array
+  auto T = c.inout(temperature, n);
+  if (!c.begin()) return;
+  (void)launch_thermal_collide_d2q9(c.dev(gi), c.dev(go), c.dev(t), c.dev(a), c.dev(b), c.dev(T), tau_T, count,
+                                    nullptr);
+  c.end();
+}
+// thermal_kernels.ispc:48-60
+void thermal_compute_buoyancy_force_2d(const float temperature[], float gy_body[], float t_ref, float beta,
+                                       float g_gravity, int count) {
+  Call c(__func__);
+  const size_t n = static_cast<size_t>(count > 0 ? count : 0);
+  auto T = c.in(temperature, n);
+  auto gy = c.inout(gy_body, n);
+  if (!c.begin()) return;
+  (void)launch_thermal_buoyancy_2d(c.dev(T), c.dev(gy), t_ref, beta, g_gravity, count, nullptr);
+  c.end();
+}
+// thermal_kernels.ispc:63-81 (a wall row outside [0, ny) is an error here)
+void thermal_apply_isothermal_wall(float g[], int nx, int ny, int y_wall, float t_wall) {
+  if (nx <= 0 || ny <= 0 || y_wall < 0 || y_wall >= ny) {
+    record_error("wake: thermal_apply_isothermal_wall: y_wall outside the grid");
+    return;
+  }
+  Call c(__func__);
+  auto gi = c.inout(g, 9 * static_cast<size_t>(nx) * ny);
+  if (!c.begin()) return;
+  (void)launch_thermal_isothermal_wall(c.dev(gi), nx, ny, y_wall, t_wall, nullptr);
+  c.end();
+}
+// ibm_kernels.ispc:12-50
+void ibm_interpolate_velocity_2d(const float ux[], const float uy[], const float marker_x[], const float marker_y[],
+                                 float marker_ux[], float marker_uy[], int nx, int ny, int num_markers) {
+  if (num_markers <= 0) return;
+  if (!ibm_markers_ok(marker_x, marker_y, nx, ny, num_markers, __func__)) return;
+  Call c(__func__);
+  const size_t n = static_cast<size_t>(nx) * ny, m = static_cast<size_t>(num_markers);
+  auto a = c.in(ux, n);
+  auto b = c.in(uy, n);
+  auto x = c.in(marker_x, m);
+  auto y = c.in(marker_y, m);
+  auto ou = c.out(marker_ux, m);
+  auto ov = c.out(marker_uy, m);
+  if (!c.begin()) return;
+  k_ibm_interpolate_2d<<<blocks(num_markers), 256>>>(c.dev(a), c.dev(b), c.dev(x), c.dev(y), c.dev(ou), c.dev(ov), nx,
+                                                     ny, num_markers);
+  c.end();
+}
+// ibm_kernels.ispc:54-96
+void ibm_spread_force_2d(const float marker_x[], const float marker_y[], const float marker_fx[],
+                         const float marker_fy[], const float marker_ds[], float euler_gx[], float euler_gy[], int nx,
+                         int ny, int num_markers) {
+  if (num_markers <= 0) return;
+  if (!ibm_markers_ok(marker_x, marker_y, nx, ny, num_markers, __func__)) return;
+  Call c(__func__);
+  const size_t n = static_cast<size_t>(nx) * ny, m = static_cast<size_t>(num_markers);
+  auto x = c.in(marker_x, m);
+  auto y = c.in(marker_y, m);
+  auto fx = c.in(marker_fx, m);
+  auto fy = c.in(marker_fy, m);
+  auto ds = c.in(marker_ds, m);
+  auto gx = c.inout(euler_gx, n);
+  auto gy = c.inout(euler_gy, n);
+  if (!c.begin()) return;
+  k_ibm_spread_2d<<<1, 64>>>(c.dev(x), c.dev(y), c.dev(fx), c.dev(fy), c.dev(ds), c.dev(gx), c.dev(gy), nx, ny,
+                             num_markers);
+  c.end();
 }
 
 }  // extern "C"
