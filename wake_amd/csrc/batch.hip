@@ -45,8 +45,8 @@ __global__ __launch_bounds__(256) void k_step_batch(const BatchSlot* __restrict_
       form_s_at<L, St>(a, in, x, y, z, n, c, true, cur, f);
     }
     if (!(c & kClsSolid)) {
-      if constexpr (LES) collide_les<L, COLL>(f, a.tau, a.cs, a.gx, a.gy, a.gz);
-      else collide<L, COLL>(f, a.tau, a.gx, a.gy, a.gz);
+      if constexpr (LES) collide_les<L, COLL, St::kFast>(f, a.tau, a.cs, a.gx, a.gy, a.gz);
+      else collide<L, COLL, false, St::kFast>(f, a.tau, a.gx, a.gy, a.gz);
     }
     static_for<L::Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;

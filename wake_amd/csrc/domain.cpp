@@ -579,6 +579,8 @@ StepArgs Domain::args(bool commit) const {
   a.outflow = outflow;
   a.rho_out = rho_out_;
   a.u_conv = u_conv_;
+  a.conv_alpha = u_conv_ / (1.0f + u_conv_);
+  a.conv_beta = 1.0f / (1.0f + u_conv_);
   a.by0 = a.per_y ? 0 : 1;
   a.by1 = a.per_y ? ny_ : ny_ - 1;
   a.bz0 = dim_ == 2 ? 0 : (a.per_z ? 0 : 1);

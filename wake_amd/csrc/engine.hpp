@@ -55,6 +55,7 @@ struct StepArgs {
   unsigned long long s_step;  // reference step index k of the S_k being formed
   int outflow;                // 1: Zou-He pressure (2D), 2: convective
   float rho_out, u_conv;
+  float conv_alpha, conv_beta;  // convective outlet: u/(1+u) and 1/(1+u), formed on the host
   int by0, by1, bz0, bz1;     // rows an open boundary acts on
   int commit_prev;            // convective: store S_k(outlet) as next step's f_prev
   // D2Q5 temperature coupled by a Boussinesq force (2D only)

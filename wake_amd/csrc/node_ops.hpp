@@ -34,6 +34,20 @@ __device__ __forceinline__ float cmul(float v) {
 }
 
 // c_q . (ux, uy, uz) summing only the non-zero components, in x, y, z order.
+// Reciprocal and square root: correctly rounded, or the hardware's 1-ulp
+// instructions (FAST) where the populations are stored in 16 bits and that
+// storage rounding dwarfs them.
+template <bool FAST>
+__device__ __forceinline__ float recip(float x) {
+  if constexpr (FAST) return __builtin_amdgcn_rcpf(x);
+  else return 1.0f / x;
+}
+template <bool FAST>
+__device__ __forceinline__ float sqrt_of(float x) {
+  if constexpr (FAST) return __builtin_amdgcn_sqrtf(x);
+  else return sqrtf(x);
+}
+
 template <class L, int q>
 __device__ __forceinline__ float cdot(float ux, float uy, float uz) {
   constexpr int a = L::cx[q], b = L::cy[q], c = L::cz[q];
@@ -68,9 +82,10 @@ __device__ __forceinline__ void density_momentum(const float (&f)[L::Q], float& 
 }
 
 // Velocity with the Guo half-force shift (d3q19_moments.ispc:60-64).
+template <bool FAST = false>
 __device__ __forceinline__ void guo_velocity(float rho, float jx, float jy, float jz, float gx, float gy, float gz,
                                              float& ux, float& uy, float& uz) {
-  const float inv_rho = 1.0f / rho;
+  const float inv_rho = recip<FAST>(rho);
   ux = (jx + 0.5f * rho * gx) * inv_rho;
   uy = (jy + 0.5f * rho * gy) * inv_rho;
   uz = (jz + 0.5f * rho * gz) * inv_rho;
@@ -123,7 +138,7 @@ __device__ __forceinline__ float smagorinsky_tau(const float (&f)[L::Q], float r
 // D3Q19 and D3Q27 (their weights are isotropic to fourth order).  Used by the
 // LES-coupled BGK/TRT step: no equilibrium is formed, so none is kept live
 // beside the collision's own.
-template <class L>
+template <class L, bool FAST = false>
 __device__ __forceinline__ float smagorinsky_tau_moments(const float (&f)[L::Q], float rho, float ux, float uy,
                                                          float uz, float tau0, float cs) {
   float sxx = 0.0f, syy = 0.0f, szz = 0.0f, sxy = 0.0f, syz = 0.0f, sxz = 0.0f;
@@ -154,10 +169,11 @@ __device__ __forceinline__ float smagorinsky_tau_moments(const float (&f)[L::Q],
     const float pxz = sxz - rho * ux * uz;
     tensor_sum = (pxx * pxx + pyy * pyy + pzz * pzz) + 2.0f * (pxy * pxy + pyz * pyz + pxz * pxz);
   }
-  const float q_stress = sqrtf(2.0f * tensor_sum);
+  const float q_stress = sqrt_of<FAST>(2.0f * tensor_sum);
   const float prefactor = 18.0f * 1.4142135623730951f * (cs * cs);
-  const float disc = tau0 * tau0 + (prefactor / rho) * q_stress;
-  return 0.5f * (tau0 + sqrtf(disc));
+  const float p_rho = FAST ? prefactor * recip<FAST>(rho) : prefactor / rho;
+  const float disc = tau0 * tau0 + p_rho * q_stress;
+  return 0.5f * (tau0 + sqrt_of<FAST>(disc));
 }
 
 // TRT antisymmetric tau from the grid (lattice_grid_3d.cpp:328-330), Lambda = 1/4.
@@ -173,13 +189,13 @@ struct Macro {
 
 // BGK with Guo forcing, in place (d3q19_collision_bgk.ispc:4-75,
 // d2q9_collision_bgk.ispc:5-86).
-template <class L>
+template <class L, bool FAST = false>
 __device__ __forceinline__ Macro collide_bgk(float (&f)[L::Q], float tau, float gx, float gy, float gz) {
-  const float omega = 1.0f / tau;
+  const float omega = recip<FAST>(tau);
   const float pre = 1.0f - 0.5f * omega;
   float rho, jx, jy, jz, ux, uy, uz;
   density_momentum<L>(f, rho, jx, jy, jz);
-  guo_velocity(rho, jx, jy, jz, gx, gy, gz, ux, uy, uz);
+  guo_velocity<FAST>(rho, jx, jy, jz, gx, gy, gz, ux, uy, uz);
   const float u_dot_g = ux * gx + uy * gy + uz * gz;
   const float usq = ux * ux + uy * uy + uz * uz;
   static_for<L::Q>([&](auto qc) {
@@ -199,14 +215,14 @@ __device__ __forceinline__ Macro collide_bgk(float (&f)[L::Q], float tau, float 
 // equilibria formed on the fly, so no equilibrium or output array is held
 // (the D3Q27 kernel stays within 5 waves per SIMD); every output is the
 // same expression of the same operands as a per-population loop.
-template <class L, bool REF_PAIRS = false>
+template <class L, bool REF_PAIRS = false, bool FAST = false>
 __device__ __forceinline__ Macro collide_trt(float (&f)[L::Q], float tau_s, float tau_a, float gx, float gy,
                                              float gz) {
-  const float omega_s = 1.0f / tau_s, omega_a = 1.0f / tau_a;
+  const float omega_s = recip<FAST>(tau_s), omega_a = recip<FAST>(tau_a);
   const float pre_s = 1.0f - 0.5f * omega_s, pre_a = 1.0f - 0.5f * omega_a;
   float rho, jx, jy, jz, ux, uy, uz;
   density_momentum<L>(f, rho, jx, jy, jz);
-  guo_velocity(rho, jx, jy, jz, gx, gy, gz, ux, uy, uz);
+  guo_velocity<FAST>(rho, jx, jy, jz, gx, gy, gz, ux, uy, uz);
   const float u_dot_g = ux * gx + uy * gy + uz * gz;
   const float usq = ux * ux + uy * uy + uz * uz;
   auto relax = [&](auto qc, float fq, float fo, float feq_q, float feo) {
@@ -239,7 +255,7 @@ __device__ __forceinline__ Macro collide_trt(float (&f)[L::Q], float tau_s, floa
 // With LES = true the shear rate comes from the Smagorinsky tau, the stress
 // taken from the moment deviations (sum c^2 (f - feq) = dm1 / 3,
 // Pi_xx - Pi_yy = dm7, Pi_xy = dm8) as in collide_mrt_d3q19.
-template <bool LES = false>
+template <bool LES = false, bool FAST = false>
 __device__ __forceinline__ Macro collide_mrt_d2q9(float (&f)[9], float tau_shear, float tau_bulk, float gx,
                                                   float gy, float cs = 0.0f) {
   const float s_e = 1.0f / tau_bulk, s_eps = 1.4f, s_q = 1.6f;
@@ -254,7 +270,7 @@ __device__ __forceinline__ Macro collide_mrt_d2q9(float (&f)[9], float tau_shear
   float m7 = (f1 - f2 + f3 - f4);
   float m8 = (f5 - f6 + f7 - f8);
   const float rho = m0;
-  const float inv_rho = 1.0f / rho;
+  const float inv_rho = recip<FAST>(rho);
   const float ux = (m3 + 0.5f * rho * gx) * inv_rho;
   const float uy = (m5 + 0.5f * rho * gy) * inv_rho;
   const float ux2 = ux * ux, uy2 = uy * uy, u2 = ux2 + uy2;
@@ -270,12 +286,13 @@ __device__ __forceinline__ Macro collide_mrt_d2q9(float (&f)[9], float tau_shear
     const float dm7 = m7 - meq7;
     const float pxx = 0.5f * (dc2 + dm7), pyy = 0.5f * (dc2 - dm7), pxy = m8 - meq8;
     const float tensor_sum = pxx * pxx + 2.0f * pxy * pxy + pyy * pyy;
-    const float q_stress = sqrtf(2.0f * tensor_sum);
+    const float q_stress = sqrt_of<FAST>(2.0f * tensor_sum);
     const float prefactor = 18.0f * 1.4142135623730951f * (cs * cs);
-    const float disc = tau_shear * tau_shear + (prefactor / rho) * q_stress;
-    s_p = 1.0f / (0.5f * (tau_shear + sqrtf(disc)));
+    const float p_rho = FAST ? prefactor * inv_rho : prefactor / rho;
+    const float disc = tau_shear * tau_shear + p_rho * q_stress;
+    s_p = recip<FAST>(0.5f * (tau_shear + sqrt_of<FAST>(disc)));
   } else {
-    s_p = 1.0f / tau_shear;
+    s_p = recip<FAST>(tau_shear);
   }
   const float F_pre = 1.0f - 0.5f * s_p;
   const float F_m3 = rho * gx, F_m5 = rho * gy;
@@ -323,7 +340,7 @@ struct D3Q19OddMoment {
 //   Pi_yy, Pi_zz = ((dm1 / 19 - Pi_xx) +- dm11) / 2,
 //   Pi_xy, Pi_yz, Pi_xz = dm13, dm14, dm15,
 // which is sum_q c c (f - feq) of d3q19_les.ispc:139-159 without forming feq.
-template <bool LES = false>
+template <bool LES = false, bool FAST = false>
 __device__ __forceinline__ Macro collide_mrt_d3q19(float (&f)[19], float tau_shear, float tau_bulk, float gx,
                                                    float gy, float gz, float cs = 0.0f) {
   const float s_e = 1.0f / tau_bulk, s_eps = 1.4f, s_q = 1.2f, s_pi = 1.4f, s_m = 1.98f;
@@ -357,7 +374,7 @@ __device__ __forceinline__ Macro collide_mrt_d3q19(float (&f)[19], float tau_she
   m[18] = (-f[11] - f[12] + f[13] + f[14] + f[15] + f[16] - f[17] - f[18]);
 
   const float rho = m[0];
-  const float inv_rho = 1.0f / rho;
+  const float inv_rho = recip<FAST>(rho);
   const float ux = (m[3] + 0.5f * rho * gx) * inv_rho;
   const float uy = (m[5] + 0.5f * rho * gy) * inv_rho;
   const float uz = (m[7] + 0.5f * rho * gz) * inv_rho;
@@ -377,12 +394,13 @@ __device__ __forceinline__ Macro collide_mrt_d3q19(float (&f)[19], float tau_she
     const float pyz = m[14] - rho * (uy * uz);
     const float pxz = m[15] - rho * (ux * uz);
     const float tensor_sum = (pxx * pxx + pyy * pyy + pzz * pzz) + 2.0f * (pxy * pxy + pyz * pyz + pxz * pxz);
-    const float q_stress = sqrtf(2.0f * tensor_sum);
+    const float q_stress = sqrt_of<FAST>(2.0f * tensor_sum);
     const float prefactor = 18.0f * 1.4142135623730951f * (cs * cs);
-    const float disc = tau_shear * tau_shear + (prefactor / rho) * q_stress;
-    s_nu = 1.0f / (0.5f * (tau_shear + sqrtf(disc)));
+    const float p_rho = FAST ? prefactor * inv_rho : prefactor / rho;
+    const float disc = tau_shear * tau_shear + p_rho * q_stress;
+    s_nu = recip<FAST>(0.5f * (tau_shear + sqrt_of<FAST>(disc)));
   } else {
-    s_nu = 1.0f / tau_shear;
+    s_nu = recip<FAST>(tau_shear);
   }
   m[1] = m[1] - s_e * (m[1] - meq1);
   m[2] = m[2] - s_eps * (m[2] - rho * (3.0f - 11.0f / 2.0f * u2));
@@ -434,32 +452,33 @@ __device__ __forceinline__ Macro collide_mrt_d3q19(float (&f)[19], float tau_she
 
 // Dispatch one collision on a node.  tau_eff is the shear tau (the scalar tau
 // of the step, or the per-node Smagorinsky value in LES-coupled mode).
-template <class L, int COLL, bool REF_PAIRS = false>
+template <class L, int COLL, bool REF_PAIRS = false, bool FAST = false>
 __device__ __forceinline__ Macro collide(float (&f)[L::Q], float tau_eff, float gx, float gy, float gz) {
   if constexpr (COLL == kBGK) {
-    return collide_bgk<L>(f, tau_eff, gx, gy, gz);
+    return collide_bgk<L, FAST>(f, tau_eff, gx, gy, gz);
   } else if constexpr (COLL == kTRT || L::Q == 27) {
-    return collide_trt<L, REF_PAIRS>(f, tau_eff, trt_tau_a(tau_eff), gx, gy, gz);
+    return collide_trt<L, REF_PAIRS, FAST>(f, tau_eff, trt_tau_a(tau_eff), gx, gy, gz);
   } else if constexpr (L::Q == 9) {
-    return collide_mrt_d2q9(f, tau_eff, kTauBulk2D, gx, gy);
+    return collide_mrt_d2q9<false, FAST>(f, tau_eff, kTauBulk2D, gx, gy);
   } else {
-    return collide_mrt_d3q19(f, tau_eff, kTauBulk3D, gx, gy, gz);
+    return collide_mrt_d3q19<false, FAST>(f, tau_eff, kTauBulk3D, gx, gy, gz);
   }
 }
 
 // Collision with LES-coupled shear tau (tau0 is the base tau).  MRT reads the
 // stress from its moments; BGK and TRT form it from feq.
-template <class L, int COLL>
+template <class L, int COLL, bool FAST = false>
 __device__ __forceinline__ Macro collide_les(float (&f)[L::Q], float tau0, float cs, float gx, float gy, float gz) {
   if constexpr (COLL == kMRT && L::Q == 19) {
-    return collide_mrt_d3q19<true>(f, tau0, kTauBulk3D, gx, gy, gz, cs);
+    return collide_mrt_d3q19<true, FAST>(f, tau0, kTauBulk3D, gx, gy, gz, cs);
   } else if constexpr (COLL == kMRT && L::Q == 9) {
-    return collide_mrt_d2q9<true>(f, tau0, kTauBulk2D, gx, gy, cs);
+    return collide_mrt_d2q9<true, FAST>(f, tau0, kTauBulk2D, gx, gy, cs);
   } else {
     float rho, jx, jy, jz, ux, uy, uz;
     density_momentum<L>(f, rho, jx, jy, jz);
-    guo_velocity(rho, jx, jy, jz, gx, gy, gz, ux, uy, uz);
-    return collide<L, COLL>(f, smagorinsky_tau_moments<L>(f, rho, ux, uy, uz, tau0, cs), gx, gy, gz);
+    guo_velocity<FAST>(rho, jx, jy, jz, gx, gy, gz, ux, uy, uz);
+    return collide<L, COLL, false, FAST>(f, smagorinsky_tau_moments<L, FAST>(f, rho, ux, uy, uz, tau0, cs), gx, gy,
+                                         gz);
   }
 }
 

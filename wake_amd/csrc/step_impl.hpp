@@ -48,6 +48,7 @@ __device__ __forceinline__ T ld_pop(const T* __restrict__ p, long long i) {
 
 struct StF32 {
   using T = float;
+  static constexpr bool kFast = false;  // correctly rounded reciprocals and roots
   template <class L, int q>
   __device__ __forceinline__ static float ld(const T* __restrict__ p, long long i) {
     return ld_pop(p, i);
@@ -61,6 +62,7 @@ struct StF32 {
 // FP16 storage of the deviation f_q - w_q; all arithmetic stays FP32.
 struct StF16 {
   using T = _Float16;
+  static constexpr bool kFast = true;  // 1-ulp reciprocals and roots (storage rounds far more)
   template <class L, int q>
   __device__ __forceinline__ static float ld(const T* __restrict__ p, long long i) {
     return L::w[q] + static_cast<float>(ld_pop(p, i));
@@ -74,6 +76,7 @@ struct StF16 {
 // BF16 storage of the deviation f_q - w_q (round to nearest even).
 struct StBF16 {
   using T = uint16_t;
+  static constexpr bool kFast = true;
   template <class L, int q>
   __device__ __forceinline__ static float ld(const T* __restrict__ p, long long i) {
     return L::w[q] + __uint_as_float(static_cast<uint32_t>(ld_pop(p, i)) << 16);
@@ -322,8 +325,7 @@ __device__ __forceinline__ void form_s_at(const StepArgs& a, const typename St::
   }
   if (a.outflow != 2) return;
   constexpr int QP = prev_stride(L::Q);
-  const float alpha = a.u_conv / (1.0f + a.u_conv);
-  const float beta = 1.0f / (1.0f + a.u_conv);
+  const float alpha = a.conv_alpha, beta = a.conv_beta;
   const bool is_out = (c & kClsOutlet) != 0;
   float4* rec = reinterpret_cast<float4*>(a.f_prev + (static_cast<long long>(z) * a.ny + y) * QP);
   auto load_prev = [&](float (&prev)[QP]) {
@@ -467,8 +469,8 @@ __global__ __launch_bounds__(256) WAKE_STEP_ATTR void k_step(const StepArgs a, c
     const int n = nn;
     if (k + 1 < WAKE_ZK && z + 1 < a.nz) fetch(z + 1, fn, cn, nn);
     if (!(c & kClsSolid)) {
-      if constexpr (LES) collide_les<L, COLL>(f, a.tau, a.cs, a.gx, a.gy, a.gz);
-      else collide<L, COLL>(f, a.tau, a.gx, a.gy, a.gz);
+      if constexpr (LES) collide_les<L, COLL, St::kFast>(f, a.tau, a.cs, a.gx, a.gy, a.gz);
+      else collide<L, COLL, false, St::kFast>(f, a.tau, a.gx, a.gy, a.gz);
     }
     static_for<L::Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
