@@ -49,6 +49,9 @@ __device__ __forceinline__ T ld_pop(const T* __restrict__ p, long long i) {
 struct StF32 {
   using T = float;
   static constexpr bool kFast = false;  // correctly rounded reciprocals and roots
+  static constexpr bool kBuf = false;   // addressed through global pointers
+  __device__ __forceinline__ static float from_bits(unsigned b) { return __uint_as_float(b); }
+  __device__ __forceinline__ static unsigned to_bits(float d) { return __float_as_uint(d); }
   template <class L, int q>
   __device__ __forceinline__ static float ld(const T* __restrict__ p, long long i) {
     return ld_pop(p, i);
@@ -63,6 +66,13 @@ struct StF32 {
 struct StF16 {
   using T = _Float16;
   static constexpr bool kFast = true;  // 1-ulp reciprocals and roots (storage rounds far more)
+  static constexpr bool kBuf = false;
+  __device__ __forceinline__ static float from_bits(unsigned short h) {
+    return static_cast<float>(__builtin_bit_cast(_Float16, h));
+  }
+  __device__ __forceinline__ static unsigned short to_bits(float d) {
+    return __builtin_bit_cast(unsigned short, static_cast<_Float16>(d));
+  }
   template <class L, int q>
   __device__ __forceinline__ static float ld(const T* __restrict__ p, long long i) {
     return L::w[q] + static_cast<float>(ld_pop(p, i));
@@ -77,6 +87,14 @@ struct StF16 {
 struct StBF16 {
   using T = uint16_t;
   static constexpr bool kFast = true;
+  static constexpr bool kBuf = false;
+  __device__ __forceinline__ static float from_bits(unsigned short h) {
+    return __uint_as_float(static_cast<uint32_t>(h) << 16);
+  }
+  __device__ __forceinline__ static unsigned short to_bits(float d) {
+    const uint32_t b = __float_as_uint(d);
+    return static_cast<unsigned short>((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+  }
   template <class L, int q>
   __device__ __forceinline__ static float ld(const T* __restrict__ p, long long i) {
     return L::w[q] + __uint_as_float(static_cast<uint32_t>(ld_pop(p, i)) << 16);
@@ -88,6 +106,44 @@ struct StBF16 {
     __builtin_nontemporal_store(static_cast<uint16_t>(r), p + i);
   }
 };
+
+// The 16-bit storages addressed through a buffer descriptor (population
+// buffers under 4 GiB): a 32-bit lane offset and a scalar offset per access,
+// so the bulk pull and the store need no per-lane 64-bit address arithmetic.
+// The descriptor's extent bounds every access to the buffer.
+template <class Base>
+struct StBuf : Base {
+  static constexpr bool kBuf = true;
+  // raw descriptor word 3 of the gfx9 family (32-bit data format, no swizzle)
+  static constexpr int kRsrcWord3 = 0x00020000;
+  static constexpr int kNonTemporal = 2;  // cache-policy bit "nt"
+  __device__ __forceinline__ static __amdgpu_buffer_rsrc_t rsrc(const void* base, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes), kRsrcWord3);
+  }
+  template <class L, int q>
+  __device__ __forceinline__ static float ldb(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    if constexpr (sizeof(typename Base::T) == 4) {
+      return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, static_cast<int>(voff), static_cast<int>(soff), 0));
+    } else {
+      const unsigned short h =
+          __builtin_amdgcn_raw_buffer_load_b16(r, static_cast<int>(voff), static_cast<int>(soff), 0);
+      return L::w[q] + Base::from_bits(h);
+    }
+  }
+  template <class L, int q>
+  __device__ __forceinline__ static void stb(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float v) {
+    if constexpr (sizeof(typename Base::T) == 4) {
+      __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, static_cast<int>(voff), static_cast<int>(soff),
+                                            kNonTemporal);
+    } else {
+      __builtin_amdgcn_raw_buffer_store_b16(Base::to_bits(v - L::w[q]), r, static_cast<int>(voff),
+                                            static_cast<int>(soff), kNonTemporal);
+    }
+  }
+};
+using StF32Buf = StBuf<StF32>;
+using StF16Buf = StBuf<StF16>;
+using StBF16Buf = StBuf<StBF16>;
 
 // ------------------------------------------------------------- RNG (inlet)
 // splitmix64 finaliser of seed ^ counter; top 24 bits as a uniform in [0,1).
@@ -116,14 +172,30 @@ __device__ __forceinline__ void gather(const StepArgs& a, const typename St::T* 
   // Bulk pull, issued for every node together with the class byte (wrapped
   // addresses are always in bounds); special nodes patch it below, so these
   // loads never wait on the class byte.
-  static_for<L::Q>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    constexpr int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
-    const int sx = cx == 0 ? x : (cx > 0 ? xm : xp);
-    const int sy = cy == 0 ? y : wrap(y - cy, ny);
-    const int sz = cz == 0 ? z : wrap(z - cz, nz);
-    f[q] = St::template ld<L, q>(A, q * S + static_cast<long long>((sz * ny + sy) * nx + sx));
-  });
+  if constexpr (St::kBuf) {
+    // byte offsets: the column per lane, the plane and row per work-group row
+    const auto r = St::rsrc(A, static_cast<unsigned>(L::Q * S * sizeof(typename St::T)));
+    constexpr unsigned E = sizeof(typename St::T);
+    const unsigned vx = static_cast<unsigned>(x) * E, vm = static_cast<unsigned>(xm) * E,
+                   vp = static_cast<unsigned>(xp) * E;
+    static_for<L::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
+      const int sy = cy == 0 ? y : wrap(y - cy, ny);
+      const int sz = cz == 0 ? z : wrap(z - cz, nz);
+      const unsigned so = static_cast<unsigned>(q * S + static_cast<long long>((sz * ny + sy) * nx)) * E;
+      f[q] = St::template ldb<L, q>(r, cx == 0 ? vx : (cx > 0 ? vm : vp), so);
+    });
+  } else {
+    static_for<L::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
+      const int sx = cx == 0 ? x : (cx > 0 ? xm : xp);
+      const int sy = cy == 0 ? y : wrap(y - cy, ny);
+      const int sz = cz == 0 ? z : wrap(z - cz, nz);
+      f[q] = St::template ld<L, q>(A, q * S + static_cast<long long>((sz * ny + sy) * nx + sx));
+    });
+  }
   if (!(c & (kClsFrozen | kClsSlow | kClsBfl))) return;
   if (c & kClsFrozen) {
     static_for<L::Q>([&](auto qc) {
@@ -472,10 +544,20 @@ __global__ __launch_bounds__(256) WAKE_STEP_ATTR void k_step(const StepArgs a, c
       if constexpr (LES) collide_les<L, COLL, St::kFast>(f, a.tau, a.cs, a.gx, a.gy, a.gz);
       else collide<L, COLL, false, St::kFast>(f, a.tau, a.gx, a.gy, a.gz);
     }
-    static_for<L::Q>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      St::template st<L, q>(aout, q * S + n, f[q]);
-    });
+    if constexpr (St::kBuf) {
+      const auto r = St::rsrc(aout, static_cast<unsigned>(L::Q * S * sizeof(typename St::T)));
+      constexpr unsigned E = sizeof(typename St::T);
+      const long long row = static_cast<long long>((z * a.ny + y) * a.nx);
+      static_for<L::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        St::template stb<L, q>(r, static_cast<unsigned>(x) * E, static_cast<unsigned>(q * S + row) * E, f[q]);
+      });
+    } else {
+      static_for<L::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        St::template st<L, q>(aout, q * S + n, f[q]);
+      });
+    }
   }
 }
 

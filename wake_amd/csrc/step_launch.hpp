@@ -35,6 +35,25 @@ hipError_t with_storage(int storage, F&& f) {
   }
 }
 
+// The step kernel's storage types: the 16-bit ones with descriptor
+// addressing when `buf` (the buffer fits a 32-bit byte offset).
+#ifndef WAKE_BUF_F32
+#define WAKE_BUF_F32 0
+#endif
+template <class F>
+hipError_t with_step_storage(int storage, bool buf, F&& f) {
+  switch (storage) {
+#if WAKE_BUF_F32
+    case kStoreF32: return buf ? f(StF32Buf{}) : f(StF32{});
+#else
+    case kStoreF32: return f(StF32{});
+#endif
+    case kStoreF16: return buf ? f(StF16Buf{}) : f(StF16{});
+    case kStoreBF16: return buf ? f(StBF16Buf{}) : f(StBF16{});
+    default: return hipErrorInvalidValue;
+  }
+}
+
 template <class L, class St, int COLL>
 hipError_t step_dispatch(bool les, bool pull, const StepArgs& a, const void* in, void* out, int bx, hipStream_t s) {
   using T = typename St::T;
@@ -62,7 +81,10 @@ struct Launch {
   static hipError_t step(int storage, int collision, bool les, bool pull, const StepArgs& a, const void* a_in,
                          void* a_out, int block_x, hipStream_t s) {
     const int bx = pick_block(a.nx, block_x);
-    return with_storage(storage, [&](auto st) {
+    // populations in a buffer under 4 GiB: descriptor addressing
+    const unsigned long long elem = storage == kStoreF32 ? 4ull : 2ull;
+    const bool buf = static_cast<unsigned long long>(L::Q) * a.stride * elem < (1ull << 32);
+    return with_step_storage(storage, buf, [&](auto st) {
       using St = decltype(st);
       if (collision == kBGK) return step_dispatch<L, St, kBGK>(les, pull, a, a_in, a_out, bx, s);
       if constexpr (L::Q == 27) {
