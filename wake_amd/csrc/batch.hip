@@ -48,10 +48,20 @@ __global__ __launch_bounds__(256) void k_step_batch(const BatchSlot* __restrict_
       if constexpr (LES) collide_les<L, COLL, St::kFast>(f, a.tau, a.cs, a.gx, a.gy, a.gz);
       else collide<L, COLL, false, St::kFast>(f, a.tau, a.gx, a.gy, a.gz);
     }
-    static_for<L::Q>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      St::template st<L, q>(out, q * a.stride + n, f[q]);
-    });
+    if constexpr (St::kBuf) {
+      const auto r = St::rsrc(out, static_cast<unsigned>(L::Q * a.stride * sizeof(T)));
+      static_for<L::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        St::template stb<L, q>(r, static_cast<unsigned>(x) * sizeof(T),
+                               static_cast<unsigned>(q * a.stride + static_cast<long long>(row) * a.nx) * sizeof(T),
+                               f[q]);
+      });
+    } else {
+      static_for<L::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        St::template st<L, q>(out, q * a.stride + n, f[q]);
+      });
+    }
   }
 }
 
@@ -93,6 +103,17 @@ struct StTag {
   using type = St;
 };
 
+// the 16-bit storages with descriptor addressing when `buf`
+template <class F>
+hipError_t with_step_storage_b(int storage, bool buf, F&& f) {
+  switch (storage) {
+    case kStoreF32: return f(StTag<StF32>{});
+    case kStoreF16: return buf ? f(StTag<StF16Buf>{}) : f(StTag<StF16>{});
+    case kStoreBF16: return buf ? f(StTag<StBF16Buf>{}) : f(StTag<StBF16>{});
+    default: return hipErrorInvalidValue;
+  }
+}
+
 template <class F>
 hipError_t with_storage_b(int storage, F&& f) {
   switch (storage) {
@@ -104,9 +125,9 @@ hipError_t with_storage_b(int storage, F&& f) {
 }
 
 template <class L>
-hipError_t step_batch_l(int storage, int collision, bool les, bool thermal, const BatchSlot* slots,
+hipError_t step_batch_l(int storage, int collision, bool les, bool thermal, bool buf, const BatchSlot* slots,
                         const int* tile_slot, int ntiles, int i, int block, int grid_x, hipStream_t s) {
-  return with_storage_b(storage, [&](auto tag) {
+  return with_step_storage_b(storage, buf && !thermal, [&](auto tag) {
     using St = typename decltype(tag)::type;
     auto go = [&](auto coll) {
       constexpr int C = decltype(coll)::value;
@@ -144,7 +165,7 @@ hipError_t mem_batch_l(int storage, const BatchSlot* slots, int nslots, int i, i
 
 }  // namespace
 
-hipError_t launch_step_batch(int lattice, int storage, int collision, bool les, bool thermal,
+hipError_t launch_step_batch(int lattice, int storage, int collision, bool les, bool thermal, bool buf,
                              const BatchSlot* slots, const int* tile_slot, int ntiles, int i, int block, int grid_x,
                              hipStream_t s) {
   if (ntiles <= 0) return hipSuccess;
@@ -152,11 +173,14 @@ hipError_t launch_step_batch(int lattice, int storage, int collision, bool les, 
   if (grid_x <= 0 || grid_x > ntiles) grid_x = ntiles;
   switch (lattice) {
     case 9:
-      return step_batch_l<D2Q9>(storage, collision, les, thermal, slots, tile_slot, ntiles, i, block, grid_x, s);
+      return step_batch_l<D2Q9>(storage, collision, les, thermal, buf, slots, tile_slot, ntiles, i, block, grid_x,
+                                s);
     case 19:
-      return step_batch_l<D3Q19>(storage, collision, les, thermal, slots, tile_slot, ntiles, i, block, grid_x, s);
+      return step_batch_l<D3Q19>(storage, collision, les, thermal, buf, slots, tile_slot, ntiles, i, block, grid_x,
+                                s);
     case 27:
-      return step_batch_l<D3Q27>(storage, collision, les, thermal, slots, tile_slot, ntiles, i, block, grid_x, s);
+      return step_batch_l<D3Q27>(storage, collision, les, thermal, buf, slots, tile_slot, ntiles, i, block, grid_x,
+                                s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -40,7 +40,9 @@ struct BatchSlot {
 // One launch = step i of the chunk for every slot.  grid_x work-groups of
 // `block` lanes walk the tiles with a stride of grid_x (grid_x = ntiles: one
 // work-group per tile).
-hipError_t launch_step_batch(int lattice, int storage, int collision, bool les, bool thermal,
+// buf: every member's population buffer is under 4 GiB, so the 16-bit
+// storages may use descriptor addressing.
+hipError_t launch_step_batch(int lattice, int storage, int collision, bool les, bool thermal, bool buf,
                              const BatchSlot* slots, const int* tile_slot, int ntiles, int i, int block, int grid_x,
                              hipStream_t s);
 // Forces on S after step i of the chunk, one work-group per slot:

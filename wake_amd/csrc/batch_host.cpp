@@ -285,6 +285,7 @@ class Batch {
     std::vector<BatchSlot> slots(n);
     std::vector<int> tile_slot;
     any_forces_ = false;
+    buf_ok_ = true;
     for (int s = 0; s < n; ++s) {
       Member& m = members_[s];
       BatchSlot& b = slots[s];
@@ -293,6 +294,9 @@ class Batch {
       bool s_state = false;
       m.d->batch_prepare(m.g[0], m.g[1], m.g[2], m.smag, b.a, b.buf, b.gbuf, parity, s_state);
       b.a.tau = m.d->tau();
+      // descriptor addressing needs every member's buffer under 4 GiB
+      const unsigned long long elem = storage_ == kStoreF32 ? 4ull : 2ull;
+      if (static_cast<unsigned long long>(m.d->lattice()) * b.a.stride * elem >= (1ull << 32)) buf_ok_ = false;
       b.step = m.d->current_step();
       b.stop_at = m.target;
       b.s_at = s_state ? b.step : kNoStep;
@@ -347,7 +351,7 @@ class Batch {
   void enqueue_chunk(int k) {
     const int n = static_cast<int>(members_.size());
     for (int i = 0; i < k; ++i) {
-      check(launch_step_batch(lattice_, storage_, collision_, les_, thermal_, d_slots_, d_tiles_, ntiles_, i,
+      check(launch_step_batch(lattice_, storage_, collision_, les_, thermal_, buf_ok_, d_slots_, d_tiles_, ntiles_, i,
                               shape_.block, grid_x(), stream_),
             "batch step");
       if (any_forces_) check(launch_mem_batch(lattice_, storage_, d_slots_, n, i, chunk_, d_hist_, stream_), "batch forces");
@@ -423,6 +427,7 @@ class Batch {
   LaunchShape shape_;
   bool layout_dirty_ = true;
   bool any_forces_ = false;
+  bool buf_ok_ = true;
   bool pending_ = false;
   int pending_k_ = 0;
   BatchSlot* d_slots_ = nullptr;
