@@ -303,6 +303,10 @@ int wake_domain_set_temperature_field(wake_domain* d, const float* t);
 int wake_domain_set_wall_temperature(wake_domain* d, int x0, int y0, int x1, int y1, float t_wall);
 /* work-group width along x for the step kernel (64, 128, 256; 0 = auto) */
 int wake_domain_set_block_x(wake_domain* d, int block_x);
+/* FP16 / BF16 storage: 1 (default) addresses the populations through a
+ * buffer descriptor when the buffer is under 4 GiB; 0 always through 64-bit
+ * pointers, as larger buffers are.  The results are the same. */
+int wake_domain_set_buffer_addressing(wake_domain* d, int enable);
 
 /* The reference's D2Q9 temperature solver (src/thermal/thermal_solver.cpp,
  * class ThermalSolver2D) with its populations in HBM.  A step

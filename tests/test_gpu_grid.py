@@ -411,6 +411,61 @@ def test_reduced_precision_storage_vs_fp64(wa, storage, name):
     assert errs[storage] <= bound, f"{name} velocity rel L2 vs FP64 {errs[storage]:.2e}"
 
 
+@pytest.mark.parametrize("storage", [1, 2])
+@pytest.mark.parametrize("lattice", [19, 27])
+def test_16bit_storage_descriptor_and_pointer_addressing_agree(wa, storage, lattice):
+    """Buffers under 4 GiB take descriptor addressing, larger ones 64-bit
+    pointers; forced on a small domain, both give the same bits (sphere,
+    inlet with turbulence, convective outlet, periodic y/z, LES)."""
+    runs = []
+    for enable in (True, False):
+        g = wa.LatticeGrid3D(53, 30, 27, 0.52, lattice=lattice, storage=storage)
+        g.set_buffer_addressing(enable)
+        g.set_periodic_axes(True, True)
+        g.initialize(1.0, 0.05, 0.0, 0.0)
+        g.set_les_coupled(True)
+        g.set_inflow_west(0.05, 0.0)
+        g.set_inflow_turbulence(0.01, 3)
+        g.set_convective_outflow_east(0.05)
+        g.set_solid_sphere(15.0, 15.0, 13.0, 4.5)
+        g.build_curved_sphere_links(15.0, 15.0, 13.0, 4.5)
+        coll = "MRT" if lattice == 19 else "TRT"
+        for _ in range(60):
+            g.step(coll, 0.0, 0.0, 0.0, True, 0.17)
+        runs.append((g.f_src_data().copy(), g.ux_data().copy()))
+    assert np.array_equal(runs[0][0].view(np.uint32), runs[1][0].view(np.uint32))
+    assert np.array_equal(runs[0][1].view(np.uint32), runs[1][1].view(np.uint32))
+
+
+@pytest.mark.parametrize("storage", [0, 1, 2])
+def test_ragged_sizes_match_oracle(wa, storage):
+    """Row lengths that are not a multiple of any work-group width (37, 71,
+    131) with walls, a sphere and BFL links: the fused step matches the
+    oracle (FP32 storage to the FP32 bar, 16-bit storage to its FP64 bound)."""
+    for nx, ny, nz in ((37, 13, 11), (71, 9, 7), (131, 6, 5)):
+        c = (nx / 3.0, ny / 2.0, nz / 2.0)
+        r = min(ny, nz) / 4.0
+        g = wa.LatticeGrid3D(nx, ny, nz, 0.62, storage=storage)
+        os_ = [ob.OracleGrid(3, nx, ny, nz, tau=0.62, bits=b) for b in (32, 64, "32fma")]
+        for t in (g, *os_):
+            t.initialize(1.0, 0.03, 0.0, 0.0)
+            t.set_solid_sphere(*c, r)
+            assert t.build_curved_sphere_links(*c, r) > 0
+        for _ in range(120):
+            g.step("MRT", 0.0, 0.0, 0.0, False, 0.14)
+        for o in os_:
+            o.run(120, "MRT")
+        fl = os_[0].cell_type() == 0
+        vel = lambda a, b, cc: np.concatenate([a[fl], b[fl], cc[fl]])
+        gv = vel(g.ux_data(), g.uy_data(), g.uz_data())
+        o32, o64, o32fma = (vel(o.ux(), o.uy(), o.uz()) for o in os_)
+        if storage == 0:
+            check_fp32_parity(gv, o32, o64, f"ragged {nx}x{ny}x{nz} u", o32fma)
+        else:
+            err = rel_l2(gv, o64)
+            assert err <= (1e-3 if storage == 1 else 1e-2), (nx, err)
+
+
 def test_diagnostics_and_update_macroscopic(wa):
     p = Pair(wa, 3, 30, 20, 16, tau=0.7)
     p.both("initialize", 1.0, 0.03, 0.0, 0.0)

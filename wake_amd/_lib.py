@@ -79,6 +79,7 @@ DOMAIN_SIGS = {
     "wake_domain_set_solid_naca": ([P, I, F, F, F, F, I, I], I),
     "wake_domain_voxelize_mesh": ([P, P, LL, I], I),
     "wake_domain_is_periodic": ([P, P], I),
+    "wake_domain_set_buffer_addressing": ([P, I], I),
     "wake_domain_export_vti": ([P, ctypes.c_char_p, I, I], I),
     "wake_domain_set_solid_circle": ([P, F, F, F], I),
     "wake_domain_set_solid_cylinder_z": ([P, F, F, F], I),

@@ -282,6 +282,9 @@ int wake_domain_export_vti(wake_domain* d, const char* path, int binary, int wit
 int wake_domain_is_periodic(wake_domain* d, int* out) {
   return guard(d, [&](wake_hip::Domain& g) { *out = g.is_periodic() ? 1 : 0; });
 }
+int wake_domain_set_buffer_addressing(wake_domain* d, int enable) {
+  return guard(d, [&](wake_hip::Domain& g) { g.set_buffer_addressing(enable != 0); });
+}
 int wake_domain_set_block_x(wake_domain* d, int block_x) {
   return guard(d, [&](wake_hip::Domain& g) { g.set_block_x(block_x); });
 }

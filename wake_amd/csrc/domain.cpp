@@ -586,6 +586,7 @@ StepArgs Domain::args(bool commit) const {
   a.bz0 = dim_ == 2 ? 0 : (a.per_z ? 0 : 1);
   a.bz1 = dim_ == 2 ? 1 : (a.per_z ? nz_ : nz_ - 1);
   a.commit_prev = commit ? 1 : 0;
+  a.global_addr = buffer_addressing_ ? 0 : 1;
   a.f_prev = d_prev_;
   a.flags = d_flags_;
   a.cls = d_cls_;

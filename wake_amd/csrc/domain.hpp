@@ -68,6 +68,9 @@ class Domain {
   // Triangle mesh (ntri triangles of 3 xyz vertices) voxelized on the
   // device: surface cells, plus the enclosed cells with fill_interior.
   void voxelize_mesh(const float* tris, long long ntri, bool fill_interior);
+  // 16-bit storage: address the populations through a buffer descriptor
+  // when the buffer is under 4 GiB (default) or always through pointers.
+  void set_buffer_addressing(bool enable) { buffer_addressing_ = enable; }
   void set_solid_circle(float cx, float cy, float r);
   void set_solid_cylinder_z(float cx, float cy, float r);
   void set_moving_wall_top(float uwx, float uwz);
@@ -202,6 +205,7 @@ class Domain {
   uint64_t version_ = 0;
   int block_x_ = 0;
   bool time_kernels_ = false;
+  bool buffer_addressing_ = true;
   size_t kev_used_ = 0;
   std::vector<hipEvent_t> kev_;
   float last_kernel_ms_ = 0.0f;

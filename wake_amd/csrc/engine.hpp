@@ -58,6 +58,7 @@ struct StepArgs {
   float conv_alpha, conv_beta;  // convective outlet: u/(1+u) and 1/(1+u), formed on the host
   int by0, by1, bz0, bz1;     // rows an open boundary acts on
   int commit_prev;            // convective: store S_k(outlet) as next step's f_prev
+  int global_addr;            // host side: keep 16-bit storage on global pointers (no descriptor)
   // D2Q5 temperature coupled by a Boussinesq force (2D only)
   float tau_g, gbeta, t_ref;  // temperature relaxation, a_y = gbeta (T - t_ref)
   const float* t_wall;        // per node: wall temperature of a solid node, NaN = adiabatic

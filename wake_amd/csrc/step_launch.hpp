@@ -83,7 +83,7 @@ struct Launch {
     const int bx = pick_block(a.nx, block_x);
     // populations in a buffer under 4 GiB: descriptor addressing
     const unsigned long long elem = storage == kStoreF32 ? 4ull : 2ull;
-    const bool buf = static_cast<unsigned long long>(L::Q) * a.stride * elem < (1ull << 32);
+    const bool buf = !a.global_addr && static_cast<unsigned long long>(L::Q) * a.stride * elem < (1ull << 32);
     return with_step_storage(storage, buf, [&](auto st) {
       using St = decltype(st);
       if (collision == kBGK) return step_dispatch<L, St, kBGK>(les, pull, a, a_in, a_out, bx, s);

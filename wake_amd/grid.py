@@ -95,6 +95,11 @@ class _Domain:
         self._c(self._L.wake_domain_export_vti(self._h, str(path).encode(), 1 if binary else 0,
                                                1 if with_analytics else 0), "export_vti")
 
+    def set_buffer_addressing(self, enable=True):
+        """16-bit storage: buffer-descriptor addressing (default, buffers
+        under 4 GiB) or 64-bit pointers (as larger buffers use)."""
+        self._c(self._L.wake_domain_set_buffer_addressing(self._h, 1 if enable else 0), "set_buffer_addressing")
+
     def is_periodic(self):
         out = ctypes.c_int(0)
         self._c(self._L.wake_domain_is_periodic(self._h, ctypes.byref(out)), "is_periodic")
