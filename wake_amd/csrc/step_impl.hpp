@@ -36,6 +36,19 @@ namespace wake_hip {
 #ifndef WAKE_WAVES_PER_EU
 #define WAKE_WAVES_PER_EU 0
 #endif
+// WAKE_NT_STORES = 0 makes the population stores ordinary (cached) stores.
+#ifndef WAKE_NT_STORES
+#define WAKE_NT_STORES 1
+#endif
+
+template <class T>
+__device__ __forceinline__ void st_pop(T* __restrict__ p, long long i, T v) {
+#if WAKE_NT_STORES
+  __builtin_nontemporal_store(v, p + i);
+#else
+  p[i] = v;
+#endif
+}
 
 template <class T>
 __device__ __forceinline__ T ld_pop(const T* __restrict__ p, long long i) {
@@ -57,9 +70,7 @@ struct StF32 {
     return ld_pop(p, i);
   }
   template <class L, int q>
-  __device__ __forceinline__ static void st(T* __restrict__ p, long long i, float v) {
-    __builtin_nontemporal_store(v, p + i);
-  }
+  __device__ __forceinline__ static void st(T* __restrict__ p, long long i, float v) { st_pop(p, i, v); }
 };
 
 // FP16 storage of the deviation f_q - w_q; all arithmetic stays FP32.
@@ -79,7 +90,7 @@ struct StF16 {
   }
   template <class L, int q>
   __device__ __forceinline__ static void st(T* __restrict__ p, long long i, float v) {
-    __builtin_nontemporal_store(static_cast<_Float16>(v - L::w[q]), p + i);
+    st_pop(p, i, static_cast<_Float16>(v - L::w[q]));
   }
 };
 
@@ -103,7 +114,7 @@ struct StBF16 {
   __device__ __forceinline__ static void st(T* __restrict__ p, long long i, float v) {
     const uint32_t b = __float_as_uint(v - L::w[q]);
     const uint32_t r = (b + 0x7FFFu + ((b >> 16) & 1u)) >> 16;
-    __builtin_nontemporal_store(static_cast<uint16_t>(r), p + i);
+    st_pop(p, i, static_cast<uint16_t>(r));
   }
 };
 
@@ -116,7 +127,7 @@ struct StBuf : Base {
   static constexpr bool kBuf = true;
   // raw descriptor word 3 of the gfx9 family (32-bit data format, no swizzle)
   static constexpr int kRsrcWord3 = 0x00020000;
-  static constexpr int kNonTemporal = 2;  // cache-policy bit "nt"
+  static constexpr int kNonTemporal = WAKE_NT_STORES ? 2 : 0;  // cache-policy bit "nt"
   __device__ __forceinline__ static __amdgpu_buffer_rsrc_t rsrc(const void* base, unsigned bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes), kRsrcWord3);
   }
