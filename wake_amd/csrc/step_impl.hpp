@@ -102,9 +102,9 @@ struct StBF16 {
   __device__ __forceinline__ static float from_bits(unsigned short h) {
     return __uint_as_float(static_cast<uint32_t>(h) << 16);
   }
+  // round to nearest even: the hardware conversion (v_cvt_pk_bf16_f32)
   __device__ __forceinline__ static unsigned short to_bits(float d) {
-    const uint32_t b = __float_as_uint(d);
-    return static_cast<unsigned short>((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+    return __builtin_bit_cast(unsigned short, static_cast<__bf16>(d));
   }
   template <class L, int q>
   __device__ __forceinline__ static float ld(const T* __restrict__ p, long long i) {
@@ -112,9 +112,7 @@ struct StBF16 {
   }
   template <class L, int q>
   __device__ __forceinline__ static void st(T* __restrict__ p, long long i, float v) {
-    const uint32_t b = __float_as_uint(v - L::w[q]);
-    const uint32_t r = (b + 0x7FFFu + ((b >> 16) & 1u)) >> 16;
-    st_pop(p, i, static_cast<uint16_t>(r));
+    st_pop(p, i, static_cast<uint16_t>(to_bits(v - L::w[q])));
   }
 };
 
