@@ -42,7 +42,7 @@ class ThermalD2Q9 {
   bool south_ = false, north_ = false;
   float t_hot_ = 1.0f, t_cold_ = 0.0f;
   float *g_src_ = nullptr, *g_dst_ = nullptr, *temp_ = nullptr, *gy_ = nullptr;
-  hipStream_t stream_ = nullptr;  // the fluid domain's stream of the last step
+  hipStream_t stream_ = nullptr;  // the fluid domain's stream, used only inside step()
   std::vector<float> h_[2];
   bool h_valid_[2] = {false, false};
 };

@@ -38,22 +38,24 @@ ThermalD2Q9::ThermalD2Q9(const ThermalD2Q9& o) : ThermalD2Q9(o.nx_, o.ny_, o.tau
   north_ = o.north_;
   t_hot_ = o.t_hot_;
   t_cold_ = o.t_cold_;
-  if (o.stream_) ck(hipStreamSynchronize(o.stream_), "sync");
+  ck(hipDeviceSynchronize(), "sync");
   ck(hipMemcpy(g_src_, o.g_src_, 9 * N_ * sizeof(float), hipMemcpyDeviceToDevice), "copy");
   ck(hipMemcpy(g_dst_, o.g_dst_, 9 * N_ * sizeof(float), hipMemcpyDeviceToDevice), "copy");
   ck(hipMemcpy(temp_, o.temp_, N_ * sizeof(float), hipMemcpyDeviceToDevice), "copy");
   ck(hipMemcpy(gy_, o.gy_, N_ * sizeof(float), hipMemcpyDeviceToDevice), "copy");
 }
 
+// Waits are device-wide: the fluid domain whose stream ran the last step
+// may already be gone (its stream handle with it).
 ThermalD2Q9::~ThermalD2Q9() {
-  if (stream_) (void)hipStreamSynchronize(stream_);
+  (void)hipDeviceSynchronize();
   for (float* p : {g_src_, g_dst_, temp_, gy_})
     if (p) (void)hipFree(p);
 }
 
 // thermal_solver.cpp:49-60: g = w_q T in both buffers, T = t_init, no force.
 void ThermalD2Q9::initialize(float t_init) {
-  if (stream_) ck(hipStreamSynchronize(stream_), "sync");
+  ck(hipDeviceSynchronize(), "sync");
   std::vector<float> g(9 * N_);
   for (int q = 0; q < 9; ++q) std::fill(g.begin() + q * N_, g.begin() + (q + 1) * N_, D2Q9::w[q] * t_init);
   ck(hipMemcpy(g_src_, g.data(), g.size() * sizeof(float), hipMemcpyHostToDevice), "upload");
@@ -85,7 +87,7 @@ void ThermalD2Q9::step(Domain& fluid, float g_gravity) {
 const float* ThermalD2Q9::host(int which) {
   if (!h_valid_[which]) {
     h_[which].resize(N_);
-    if (stream_) ck(hipStreamSynchronize(stream_), "sync");
+    ck(hipDeviceSynchronize(), "sync");
     ck(hipMemcpy(h_[which].data(), which == 0 ? temp_ : gy_, N_ * sizeof(float), hipMemcpyDeviceToHost), "d2h");
     h_valid_[which] = true;
   }
