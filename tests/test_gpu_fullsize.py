@@ -76,9 +76,9 @@ def test_bgk_periodic_256_conserves_mass_and_momentum(wa):
     n = float(N ** 3)
     print(f"mass drift {abs(m - m0) / m0:.2e}, momentum drift "
           f"{max(abs(a - b) for a, b in zip(j, j0)) / n:.2e} per node")
-    # FP32: the weights' rounded sum differs from 1 by a few 1e-8, which
-    # relaxation adds to the mass each step (6.4e-7 over these 30 steps);
-    # an indexing or streaming error would move mass by orders more
+    # FP32: the rounded weights do not sum to exactly 1 (2.4e-7 off, summed
+    # in order), and relaxation feeds that into the mass every step (6.4e-7
+    # over these 30 steps); an indexing or streaming error moves far more
     assert abs(m - m0) / m0 < 2e-6
     assert max(abs(a - b) for a, b in zip(j, j0)) / n < 1e-7
     assert abs(g.compute_total_mass() - m) / m < 1e-6
