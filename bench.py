@@ -45,10 +45,11 @@ def log(msg):
 
 def measured_traffic(n, storage):
     """HBM bytes per k_step launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this workload (256^3 FP32 only), else None."""
-    if n != 256 or storage != 0:
+    WRITE_SIZE summary of this workload (256^3, FP32 or FP16), else None."""
+    names = {0: "k_step_d3q19_sphere256_pmc.md", 1: "k_step_d3q19_sphere256_fp16_pmc.md"}
+    if n != 256 or storage not in names:
         return None
-    path = os.path.join(ROOT, "profiles", "k_step_d3q19_sphere256_pmc.md")
+    path = os.path.join(ROOT, "profiles", names[storage])
     try:
         with open(path) as f:
             for line in f:
@@ -503,8 +504,10 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(n, storage),
-            "traffic_source": "profiles/k_step_d3q19_sphere256_pmc.md (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, "
-                              "copy-calibrated)",
+            "traffic_source": (None if measured_traffic(n, storage) is None else
+                               ("profiles/" + ("k_step_d3q19_sphere256_pmc.md" if storage == 0 else
+                                               "k_step_d3q19_sphere256_fp16_pmc.md") +
+                                " (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, copy-calibrated)")),
             "bytes_per_node": bpn,
             "kernel": "k_step (fused pull + MRT/LES collide), HIP events per launch",
             "kernel_ms": round(kernel_ms, 5),
