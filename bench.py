@@ -14,6 +14,7 @@ one-domain-at-a-time path.  `--workload serve` replays a Poisson stream of
 mixed 3D bluff-body and 2D thermal-cavity requests through the
 continuous-batching serving loop and then through the plain sequential
 loop, and reports simulations per second and p50/p99 time-to-result.
+`--workload slab` times one GPU's z-slab share of the 256^3 sphere workload.
 `--workload wing` times the D3Q27 NACA-0012 wing channel (384x192x192)
 with FP32/FP16/BF16 storage and states each storage's error against the
 FP64 oracle on the same flow at reduced size.
@@ -391,10 +392,59 @@ def main_wing(args):
     print(json.dumps(out), flush=True)
 
 
+def main_slab(args):
+    """One GPU's share of the 256^3 sphere workload split into 8 z-slabs:
+    256 x 256 x 32 owned planes plus 2 halo planes (z 96..127 and its
+    neighbours, the slab that holds the sphere's lower half), stepped by the
+    same fused kernel.  Periodic z stands in for the halo exchange (a halo
+    plane is pulled like an interior plane).  Reported beside it: the bytes
+    each face would send per step (the 5 populations with c_z = +-1 of one
+    256 x 256 plane) and their time over one xGMI link at 153 GB/s, an
+    estimate: nothing here uses a second GPU."""
+    import wake_amd
+
+    storage = {"fp32": 0, "fp16": 1, "bf16": 2}[args.storage]
+    n, owned, z0, R = 256, 32, 96, 24.0
+    nz = owned + 2
+    cz = 128.0 - z0 + 1.0  # the sphere centre in slab coordinates (plane 0 is the lower halo)
+    g = wake_amd.LatticeGrid3D(n, n, nz, 0.505, storage=storage)
+    g.set_periodic_axes(True, True)
+    g.initialize(1.0, 0.05, 0.0, 0.0)
+    g.set_les_coupled(True)
+    g.set_inflow_west(0.05, 0.0)
+    g.set_inflow_turbulence(0.01, 2)
+    g.set_convective_outflow_east(0.05)
+    g.set_solid_sphere(64.0, 128.0, cz, R)
+    nl = g.build_curved_sphere_links(64.0, 128.0, cz, R)
+    g.run(args.warmup, "MRT", use_les=True, smag=0.17)
+    g.synchronize()
+    ms = g.run(args.steps, "MRT", use_les=True, smag=0.17, timed=True)
+    kernel_ms = g.kernel_time()
+    owned_nodes = n * n * owned
+    elem = 4 if storage == 0 else 2
+    face = 5 * n * n * elem
+    mlups = owned_nodes * args.steps / (ms * 1e-3) / 1e6
+    log(f"slab 256x256x{owned}+2: {ms / args.steps * 1e3:.1f} us/step (kernel {kernel_ms * 1e3:.1f} us), "
+        f"{mlups:.0f} MLUPS on the owned nodes; face {face / 1e6:.3f} MB each way, "
+        f"{face / 153e9 * 1e6:.1f} us per link (estimate)")
+    out = {
+        "metric": METRIC, "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms / args.steps, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": args.storage, "data": "synthetic",
+        "config": {"workload": f"one GPU's z-slab of the 256^3 D3Q19 MRT+LES sphere workload split 8 ways: "
+                               f"256x256x{owned} owned + 2 halo planes (z {z0}..{z0 + owned - 1})",
+                   "owned_nodes": owned_nodes, "bfl_links": nl, "kernel_ms": round(kernel_ms, 5),
+                   "face_bytes_per_direction": face,
+                   "xgmi_us_per_face_estimate": round(face / 153e9 * 1e6, 2),
+                   "parallelism": "single GPU (one slab)"},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["sphere", "batched-wakes", "serve", "wing"], default="sphere")
+    ap.add_argument("--workload", choices=["sphere", "batched-wakes", "serve", "wing", "slab"], default="sphere")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=256)
@@ -426,6 +476,11 @@ def main():
         return
     if args.workload == "serve":
         main_serve(args)
+        return
+    if args.workload == "slab":
+        if args.steps is None:
+            args.steps = 2000
+        main_slab(args)
         return
     if args.workload == "batched-wakes":
         if args.steps is None:
