@@ -187,7 +187,8 @@ enum wake_field {
 
 /* LatticeGrid3D(nx, ny, nz, tau) (lattice_grid_3d.cpp:46-64) for dim = 3,
  * LatticeGrid2D(nx, ny, tau) (lattice_grid_2d.cpp:46-62) for dim = 2.
- * lattice: 9 (2D), 19 or 27 (3D).  storage: WAKE_STORE_F32, or FP16 / BF16
+ * lattice: 9 (2D), 19 or 27 (3D), 0 for the default of the dimension (9, 19);
+ * any other value is an error.  storage: WAKE_STORE_F32, or FP16 / BF16
  * storage of f_q - w_q with FP32 arithmetic.  plane_pad < 0 picks the
  * default padding of the population plane stride. */
 /* Device bytes a domain of this shape allocates (with the temperature

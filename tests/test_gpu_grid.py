@@ -516,3 +516,31 @@ def test_geometry_flags_bit_exact_naca_and_ellipsoid(wa):
     for t in (g4, o4):
         t.set_solid_ellipsoid(30.2, 27.9, 24.4, 14.0, 9.0, 6.5, axes)
     assert np.array_equal(g4.cell_type_data(), o4.cell_type())
+
+
+def test_c_abi_rejects_invalid_arguments(wa):
+    """Invalid arguments across the C ABI return a non-zero code with a
+    message and leave the library usable."""
+    import ctypes
+
+    L = wa._lib.lib()
+    h = ctypes.c_void_p()
+    bad_creates = [(4, 19, 8, 8, 8, 0.6, 0), (3, 7, 8, 8, 8, 0.6, 0), (2, 19, 8, 8, 1, 0.6, 0),
+                   (3, 19, 0, 8, 8, 0.6, 0), (3, 19, 8, 8, 8, 0.6, 5), (3, 19, 2048, 2048, 1024, 0.6, 0)]
+    for args in bad_creates:
+        rc = L.wake_domain_create(*args, ctypes.c_longlong(-1), ctypes.byref(h))
+        assert rc != 0, args
+        assert L.wake_hip_last_error().decode(), args
+        L.wake_hip_clear_error()
+    g = wa.LatticeGrid3D(12, 10, 8, 0.6)
+    nan_tri = np.full((1, 3, 3), np.nan, np.float32)
+    with pytest.raises(RuntimeError):
+        g.voxelize_mesh(nan_tri)
+    with pytest.raises(RuntimeError):
+        g.set_solid_naca(12345, 10.0, 0.0, 2.0, 5.0)
+    with pytest.raises(RuntimeError):
+        g.export_vti("/nonexistent_dir/x.vti")
+    # still usable
+    g.initialize(1.0, 0.01, 0.0, 0.0)
+    g.step("BGK", 0.0, 0.0, 0.0, False, 0.14)
+    assert np.isfinite(g.ux_data()).all()

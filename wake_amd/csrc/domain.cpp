@@ -65,8 +65,11 @@ size_t elem_size(int storage) { return storage == kStoreF32 ? 4 : 2; }
 Domain::Domain(int dim, int lattice, int nx, int ny, int nz, float tau, int storage, long long plane_pad)
     : dim_(dim), lattice_(lattice), nx_(nx), ny_(ny), nz_(dim == 2 ? 1 : nz), storage_(storage), tau_(tau) {
   if (dim != 2 && dim != 3) throw HipError("dimension must be 2 or 3");
+  // 0 picks the default lattice of the dimension (D2Q9, D3Q19)
+  if (dim == 2 && lattice != 0 && lattice != 9) throw HipError("a 2D domain uses D2Q9 (lattice 9)");
+  if (dim == 3 && lattice != 0 && lattice != 19 && lattice != 27) throw HipError("a 3D domain uses lattice 19 or 27");
   if (dim == 2) lattice_ = 9;
-  else if (lattice != 19 && lattice != 27) lattice_ = 19;
+  else if (lattice == 0) lattice_ = 19;
   if (storage < kStoreF32 || storage > kStoreBF16) throw HipError("unknown population storage format");
   if (nx_ < 2 || ny_ < 1 || nz_ < 1) throw HipError("grid dimensions must be positive (nx >= 2)");
   Q_ = lattice_;
