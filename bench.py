@@ -522,6 +522,11 @@ def main():
     ok = all(math.isfinite(v) for v in (cd, cl)) and math.isfinite(g.compute_total_mass())
     log(f"C_D {cd:.4f} C_L {cl:.5f} finite={ok}")
 
+    from wake_amd.hbm_probe import ceilings
+    ceil = ceilings(n, n, n)
+    log(f"HBM ceilings on this GPU: contiguous copy {ceil['copy_contiguous_gbps']:.0f} GB/s, "
+        f"D3Q19 pull copy {ceil['pull_d3q19_gbps']:.0f} GB/s; step kernel {achieved:.0f} GB/s")
+
     cpu = None
     if args.cpu_steps > 0:
         log(f"CPU baseline: oracle {args.cpu_n}^3, {args.cpu_steps} steps")
@@ -567,6 +572,12 @@ def main():
             "kernel": "k_step (fused pull + MRT/LES collide), HIP events per launch",
             "kernel_ms": round(kernel_ms, 5),
             "step_ms_events": round(ms / args.steps, 5),
+            "practical_ceiling": {
+                "copy_contiguous_gbps": round(ceil["copy_contiguous_gbps"], 1),
+                "pull_d3q19_gbps": round(ceil["pull_d3q19_gbps"], 1),
+                "frac_of_pull_d3q19": round(achieved / ceil["pull_d3q19_gbps"], 4),
+                "source": f"wake_hip_hbm_probe on this GPU, {n}^3 nodes x 152 B, 20 launches each",
+            },
         },
         "cpu_baseline": cpu,
     }

@@ -83,6 +83,21 @@ def test_package_imports_and_fails_loudly_without_gpu():
         wake_amd.kernels.kernel("d2q9_compute_macroscopic_moments", f, r, r.copy(), r.copy(), 4)
 
 
+def test_hbm_probe_rejects_invalid_arguments():
+    """Checked on the host before any GPU call, so it runs here too."""
+    import wake_amd
+
+    L = wake_amd._lib.lib()
+    ms = ctypes.c_float(-1.0)
+    for args in ((2, 64, 64, 64, 5), (0, 0, 64, 64, 5), (1, 2048, 64, 64, 5), (1, 64, 64, 70000, 5),
+                 (0, 64, 64, 64, 0)):
+        assert L.wake_hip_hbm_probe(*args, ctypes.byref(ms)) != 0, args
+        assert L.wake_hip_last_error()
+    assert L.wake_hip_hbm_probe(0, 64, 64, 64, 5, None) != 0
+    assert ms.value == -1.0
+    L.wake_hip_clear_error()
+
+
 def test_missing_library_raises(tmp_path, monkeypatch):
     import importlib
 

@@ -544,3 +544,21 @@ def test_c_abi_rejects_invalid_arguments(wa):
     g.initialize(1.0, 0.01, 0.0, 0.0)
     g.step("BGK", 0.0, 0.0, 0.0, False, 0.14)
     assert np.isfinite(g.ux_data()).all()
+
+
+def test_hbm_probe_ceilings(wa):
+    """The bandwidth probe bench.py reports beside the step kernel: both
+    shapes run, give a plausible HBM rate, and the pull (19 shifted plane
+    streams) is not faster than the contiguous copy of the same bytes."""
+    from wake_amd.hbm_probe import ceilings
+
+    # 256 x 256 x 192: two 478 MB buffers, well beyond the 256 MB last-level
+    # cache, so the rate is HBM's (at 128^3 both buffers stay cache-resident)
+    c = ceilings(256, 256, 192, reps=5)
+    print(c)
+    assert 500.0 < c["pull_d3q19_gbps"] < 8000.0 * 1.05
+    assert 500.0 < c["copy_contiguous_gbps"] < 8000.0 * 1.05
+    assert c["pull_d3q19_gbps"] < 1.1 * c["copy_contiguous_gbps"]
+    # a non-multiple-of-64 row and odd sizes (partial waves, wrap at every edge)
+    c = ceilings(100, 7, 5, reps=2)
+    assert all(v > 0 for v in c.values())

@@ -111,6 +111,7 @@ DOMAIN_SIGS = {
     "wake_hip_clear_error": ([], I),
     "wake_hip_device_count": ([], I),
     "wake_hip_d3q19_minv": ([P], None),
+    "wake_hip_hbm_probe": ([I, I, I, I, I, P], I),
 }
 
 # Part 2: the reference's D2Q9 temperature solver

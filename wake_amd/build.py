@@ -34,7 +34,7 @@ HOST_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-ffp-contract=off", f"--offload-arc
 SOURCES = [
     "step_d3q19.hip", "step_d3q27.hip", "step_d2q9.hip", "step_kernels.hip", "thermal.hip", "analytics.hip",
     "ispc_compat.hip", "batch.hip", "domain.cpp", "capi.cpp", "lattice_grid.cpp", "runner.cpp", "batch_host.cpp",
-    "host_api.cpp", "geometry.hip", "export.cpp", "thermal_solver.cpp",
+    "host_api.cpp", "geometry.hip", "export.cpp", "thermal_solver.cpp", "hbm_probe.hip",
 ]
 # The voxelizer's overlap test and the link builder's wall fractions must
 # round every product and sum on its own, with IEEE division and sqrt.

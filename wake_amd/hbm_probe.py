@@ -1,4 +1,5 @@
-"""HBM byte-counter calibration: one plain device copy of a known size.
+"""HBM byte-counter calibration (one plain device copy of a known size),
+and the practical bandwidth ceilings of the step's access pattern.
 
     rocprofv3 --pmc FETCH_SIZE -f csv -d OUT -o copy -- python -m wake_amd.hbm_probe --mib 1024
 
@@ -8,6 +9,25 @@ per dispatch, divided by that size, is the factor by which the counters
 relate to bytes moved on this GPU.
 """
 import argparse
+import ctypes
+
+BYTES_PER_NODE = 152  # 19 floats read and written per node
+
+
+def ceilings(nx=256, ny=256, nz=256, reps=20):
+    """The two practical HBM ceilings on the current GPU, in GB/s at 152
+    bytes per node (wake_hip_hbm_probe): a contiguous copy in the fastest
+    copy shape, and the D3Q19 pull over 19 SoA planes without arithmetic."""
+    from . import _lib
+
+    L = _lib.lib()
+    out = {}
+    for shape, key in ((0, "copy_contiguous_gbps"), (1, "pull_d3q19_gbps")):
+        ms = ctypes.c_float()
+        if L.wake_hip_hbm_probe(shape, nx, ny, nz, reps, ctypes.byref(ms)) != 0:
+            raise RuntimeError(L.wake_hip_last_error().decode())
+        out[key] = BYTES_PER_NODE * nx * ny * nz / (ms.value * 1e-3) / 1e9
+    return out
 
 
 def main():
