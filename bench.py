@@ -16,8 +16,8 @@ continuous-batching serving loop and then through the plain sequential
 loop, and reports simulations per second and p50/p99 time-to-result.
 `--workload slab` times one GPU's z-slab share of the 256^3 sphere workload.
 `--workload wing` times the D3Q27 NACA-0012 wing channel (384x192x192)
-with FP32/FP16/BF16 storage and states each storage's error against the
-FP64 oracle on the same flow at reduced size.
+with FP32/FP16/BF16 storage (tests/precision_wing.py states each
+storage's error against the FP64 oracle on the same flow at reduced size).
 
 Prints ONE JSON line.  The roofline entry divides the algorithmic bytes of
 a step (one read and one write of every population: 152 B/node for D3Q19
@@ -303,16 +303,9 @@ def wing_setup(g, nx, ny, chord, u=0.04, aoa=8.0):
 def main_wing(args):
     """NACA 0012 wing at 8 deg, D3Q27 TRT + LES, Re 2000 on the chord: the
     full 384x192x192 domain timed with FP32, FP16 and BF16 population
-    storage, and the same flow at 1/scale size run with each storage and with
-    the FP64 oracle to state each storage's error (velocity field, C_D, C_L,
-    Strouhal number of the lift history)."""
-    import numpy as np
-
+    storage.  Each storage's error against the FP64 oracle on the same flow
+    at reduced size comes from tests/precision_wing.py."""
     import wake_amd
-    from wake_amd import analysis
-
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_bind as ob
 
     u, re = 0.04, 2000.0
     names = {0: "fp32", 1: "fp16", 2: "bf16"}
@@ -323,61 +316,13 @@ def main_wing(args):
         g = wing_setup(wake_amd.LatticeGrid3D(nx, ny, nz, tau, lattice=27, storage=st), nx, ny, chord)
         g.run(10, "TRT", use_les=True, smag=0.17)
         ms = g.run(args.steps, "TRT", use_les=True, smag=0.17, timed=True)
+        kernel_ms = g.kernel_time()
+        bpn = 216 if st == 0 else 108
         perf[names[st]] = {"MLUPS": round(nx * ny * nz * args.steps / (ms * 1e-3) / 1e6, 1),
-                           "kernel_us": round(g.kernel_time() * 1e3, 1)}
+                           "kernel_us": round(kernel_ms * 1e3, 1),
+                           "kernel_gbps": round(bpn * nx * ny * nz / (kernel_ms * 1e-3) / 1e9, 1)}
         log(f"full {nx}x{ny}x{nz} {names[st]}: {perf[names[st]]}")
         del g
-    # error study at reduced size
-    k = args.wing_scale
-    sx, sy, sz, sc = nx // k, ny // k, nz // k, chord / k
-    stau = 3.0 * u * sc / re + 0.5
-    steps, every = args.wing_steps, 5
-    area = 0.5 * u * u * sc * sz
-
-    def coeffs(F):
-        return F[:, 0] / area, F[:, 1] / area
-
-    # checkpoints: field errors while the runs are still correlated, and
-    # force statistics over the whole history
-    checks = sorted({c for c in (steps // 8, steps // 4, steps // 2, steps) if c > 0})
-    res, fields = {}, {}
-    fl = None
-    for st in (0, 1, 2):
-        g = wing_setup(wake_amd.LatticeGrid3D(sx, sy, sz, stau, lattice=27, storage=st), sx, sy, sc)
-        b = wake_amd.Batch(lattice=27, storage=st, collision="TRT", use_les=True, chunk=100)
-        b.add(g, steps, smag=0.17, forces=(0.0, 0.0, 0.0, 0.0))
-        done = 0
-        for c in checks:
-            b.run(c - done)
-            done = c
-            fl = g.cell_type_data() == 0
-            fields[(names[st], c)] = np.concatenate([g.ux_data()[fl], g.uy_data()[fl], g.uz_data()[fl]])
-        F = b.forces(0)[every - 1::every]
-        res[names[st]] = coeffs(F)
-        del b, g
-    log(f"reduced {sx}x{sy}x{sz}: GPU runs done; FP64 oracle {steps} steps on the host")
-    t0 = time.perf_counter()
-    o = wing_setup(ob.OracleGrid(3, sx, sy, sz, tau=stau, lattice=27, bits=64), sx, sy, sc)
-    hist, ref_fields = [], {}
-    for i in range(steps // every):
-        o.run(every, "TRT", use_les=True, smag=0.17)
-        hist.append(o.momentum_exchange(0.0, 0.0, 0.0, 0.0))
-        if (i + 1) * every in checks:
-            ref_fields[(i + 1) * every] = np.concatenate([o.ux()[fl], o.uy()[fl], o.uz()[fl]])
-    log(f"FP64 oracle done in {time.perf_counter() - t0:.0f} s")
-    ref_cd, ref_cl = coeffs(np.array(hist))
-    half = len(ref_cd) // 2
-    st_of = lambda cl: analysis.strouhal_number(
-        analysis.analyze_frequency(cl[half:].astype(np.float32), float(every)), sc, u)
-    ref = {"C_D": float(np.mean(ref_cd[half:])), "C_L": float(np.mean(ref_cl[half:])), "St": st_of(ref_cl)}
-    errors = {}
-    for nm, (cd, cl) in res.items():
-        v = {"C_D": float(np.mean(cd[half:])), "C_L": float(np.mean(cl[half:])), "St": st_of(cl)}
-        l2 = {str(c): float(np.linalg.norm(fields[(nm, c)] - ref_fields[c]) / np.linalg.norm(ref_fields[c]))
-              for c in checks}
-        errors[nm] = {"velocity_rel_l2_at_step": l2,
-                      **{f"{q}_rel": (abs(v[q] / ref[q] - 1) if ref[q] else None) for q in v}, **v}
-        log(f"{nm} vs FP64: {errors[nm]}")
     out = {
         "metric": METRIC, "value": perf["fp32"]["MLUPS"], "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
         "warmup": 10, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -385,8 +330,8 @@ def main_wing(args):
         "config": {"workload": f"D3Q27 TRT(Lambda=1/4)+LES channel, NACA 0012 extruded at 8 deg, chord {chord:g}, "
                                f"{nx}x{ny}x{nz}, inlet u=0.04, Re=2000",
                    "storage_perf": perf,
-                   "error_study": {"size": [sx, sy, sz], "chord": sc, "steps": steps, "fp64_reference": ref,
-                                   "errors": errors},
+                   "bytes_per_node": {"fp32": 216, "fp16": 108, "bf16": 108},
+                   "error_study": "tests/precision_wing.py (FP64 oracle at 1/4 size)",
                    "parallelism": "single GPU"},
     }
     print(json.dumps(out), flush=True)
@@ -460,8 +405,6 @@ def main():
     ap.add_argument("--step-scale", type=float, default=1.0)
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--results", default=None, help="serve: write per-request results (JSON lines) here")
-    ap.add_argument("--wing-scale", type=int, default=4)
-    ap.add_argument("--wing-steps", type=int, default=5000)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))

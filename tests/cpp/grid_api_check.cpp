@@ -316,6 +316,12 @@ static void exports() {
              js.find("\"average_mlups\": 30001") != std::string::npos &&
              js.find("\"strouhal_number\": 0.2") != std::string::npos,
          "summary json:\n%s", js.c_str());
+  // a name with a quote and a backslash stays one JSON string; a NaN is null
+  EXPECT(ex.write_summary_json(dir + "/summary2.json", "run \"a\\b\"\n", std::nanf(""), 0.01f), "json 2");
+  const std::string js2 = slurp(dir + "/summary2.json");
+  EXPECT(js2.find("\"simulation\": \"run \\\"a\\\\b\\\"\\u000a\",") != std::string::npos &&
+             js2.find("\"strouhal_number\": null") != std::string::npos,
+         "escaped json:\n%s", js2.c_str());
   EXPECT(!VtkExporter::export_vti_3d("/nonexistent_dir/x.vti", g), "unwritable path reports false");
   std::printf("exports: %zu-byte binary vti, %zu-byte ascii vti\n", bin.size(), txt.size());
 }

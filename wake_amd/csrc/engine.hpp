@@ -138,8 +138,10 @@ hipError_t launch_build_sphere_links(int lattice, const uint8_t* d_flags, int nx
 // (d2q9_streaming.ispc:33-95) on device pointers (ispc_compat.hip).
 hipError_t launch_thermal_collide_d2q9(const float* g_in, float* g_out, const uint8_t* ct, const float* ux,
                                        const float* uy, float* temp, float tau_t, int count, hipStream_t s);
+// accumulate: gy += force (the export); else gy = force (no separate clear,
+// so a captured step records no memset).
 hipError_t launch_thermal_buoyancy_2d(const float* temp, float* gy, float t_ref, float beta, float g_gravity,
-                                      int count, hipStream_t s);
+                                      int count, bool accumulate, hipStream_t s);
 hipError_t launch_thermal_isothermal_wall(float* g, int nx, int ny, int y_wall, float t_wall, hipStream_t s);
 hipError_t launch_stream_walls_d2q9(const float* src, float* dst, const uint8_t* ct, int nx, int ny, float uwx,
                                     float uwy, hipStream_t s);

@@ -6,8 +6,10 @@
 // domain's host mirrors (one read-back kernel and one copy per field after
 // a step); the .vti is written either as the reference's ASCII text or as
 // one raw little-endian appended block per array.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <fstream>
 #include <iomanip>
 #include <sstream>
@@ -146,7 +148,7 @@ bool VtkExporter::write_pvd_manifest(const std::string& manifest_path, const std
   }
   out << "  </Collection>\n";
   out << "</VTKFile>\n";
-  return true;
+  return static_cast<bool>(out);
 }
 
 // ------------------------------------------------------ aerodynamic history
@@ -213,7 +215,7 @@ bool TelemetryExporter::write_aero_csv(const std::string& filepath) const {
   std::ofstream out(filepath);
   if (!out.is_open()) return false;
   aero_rows(out, aero_samples_);
-  return true;
+  return static_cast<bool>(out);
 }
 
 bool TelemetryExporter::write_performance_csv(const std::string& filepath) const {
@@ -224,8 +226,35 @@ bool TelemetryExporter::write_performance_csv(const std::string& filepath) const
   for (const PerformanceMetric& p : perf_samples_)
     out << p.step << "," << p.step_time_ms << "," << p.mlups << "," << p.max_speed << "," << p.total_mass << ","
         << p.kinetic_energy << "\n";
-  return true;
+  return static_cast<bool>(out);
 }
+
+namespace {
+// A JSON string literal: quotes, backslashes and control characters escaped.
+std::string json_string(const std::string& v) {
+  std::string o = "\"";
+  for (const unsigned char c : v) {
+    if (c == '"' || c == '\\') {
+      o += '\\';
+      o += static_cast<char>(c);
+    } else if (c < 0x20) {
+      char buf[8];
+      std::snprintf(buf, sizeof(buf), "\\u%04x", c);
+      o += buf;
+    } else {
+      o += static_cast<char>(c);
+    }
+  }
+  return o + "\"";
+}
+// A JSON number, or null for a NaN or an infinity (JSON has neither).
+std::string json_number(double v) {
+  if (!std::isfinite(v)) return "null";
+  std::ostringstream o;
+  o << v;
+  return o.str();
+}
+}  // namespace
 
 bool TelemetryExporter::write_summary_json(const std::string& filepath, const std::string& simulation_name,
                                            real_t strouhal_number, real_t shedding_frequency) const {
@@ -251,21 +280,21 @@ bool TelemetryExporter::write_summary_json(const std::string& filepath, const st
     avg_mlups = sum / static_cast<double>(perf_samples_.size());
   }
   out << "{\n";
-  out << "  \"simulation\": \"" << simulation_name << "\",\n";
+  out << "  \"simulation\": " << json_string(simulation_name) << ",\n";
   out << "  \"total_samples\": " << aero_samples_.size() << ",\n";
   out << "  \"performance\": {\n";
-  out << "    \"average_mlups\": " << avg_mlups << "\n";
+  out << "    \"average_mlups\": " << json_number(avg_mlups) << "\n";
   out << "  },\n";
   out << "  \"aerodynamics\": {\n";
-  out << "    \"mean_drag_coefficient\": " << mean_cd << ",\n";
-  out << "    \"max_drag_coefficient\": " << max_cd << ",\n";
-  out << "    \"min_drag_coefficient\": " << min_cd << ",\n";
-  out << "    \"rms_lift_coefficient\": " << rms_cl << ",\n";
-  out << "    \"shedding_frequency\": " << shedding_frequency << ",\n";
-  out << "    \"strouhal_number\": " << strouhal_number << "\n";
+  out << "    \"mean_drag_coefficient\": " << json_number(mean_cd) << ",\n";
+  out << "    \"max_drag_coefficient\": " << json_number(max_cd) << ",\n";
+  out << "    \"min_drag_coefficient\": " << json_number(min_cd) << ",\n";
+  out << "    \"rms_lift_coefficient\": " << json_number(rms_cl) << ",\n";
+  out << "    \"shedding_frequency\": " << json_number(shedding_frequency) << ",\n";
+  out << "    \"strouhal_number\": " << json_number(strouhal_number) << "\n";
   out << "  }\n";
   out << "}\n";
-  return true;
+  return static_cast<bool>(out);
 }
 
 }  // namespace wake

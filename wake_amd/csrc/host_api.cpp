@@ -129,6 +129,8 @@ bool ObjLoader::load_file(const std::string& filepath) {
 // obj_loader.cpp:21-83: `v x y z` vertices; `f` faces of v, v/vt, v/vt/vn or
 // v//vn tokens, 1-based (negative: from the end; 0 is left as 0),
 // triangulated as a fan; triangles with an index out of range are dropped.
+// A face token that is not an integer throws std::invalid_argument (from
+// std::stoi), as the reference's parser does (obj_loader.cpp:50-52).
 bool ObjLoader::load_from_string(const std::string& obj_data) {
   vertices_.clear();
   triangles_.clear();

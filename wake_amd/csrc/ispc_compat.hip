@@ -543,11 +543,13 @@ __global__ __launch_bounds__(256) void k_thermal_collide_d2q9(const float* __res
   }
 }
 
-// thermal_kernels.ispc:48-60
+// thermal_kernels.ispc:48-60.  ACC: add to gy, as the export does; else
+// gy = 0 + force, the same bits as clearing gy first (the solver's step).
+template <bool ACC>
 __global__ __launch_bounds__(256) void k_thermal_buoyancy_2d(const float* __restrict__ temp, float* __restrict__ gy,
                                                              float t_ref, float beta, float g_gravity, int count) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) gy[i] += beta * (temp[i] - t_ref) * g_gravity;
+  if (i < count) gy[i] = (ACC ? gy[i] : 0.0f) + beta * (temp[i] - t_ref) * g_gravity;
 }
 
 // thermal_kernels.ispc:63-81: on row y_wall, for q = 0..8 in turn,
@@ -640,9 +642,10 @@ hipError_t launch_thermal_collide_d2q9(const float* g_in, float* g_out, const ui
   return hipGetLastError();
 }
 hipError_t launch_thermal_buoyancy_2d(const float* temp, float* gy, float t_ref, float beta, float g_gravity,
-                                      int count, hipStream_t s) {
+                                      int count, bool accumulate, hipStream_t s) {
   if (count <= 0) return hipSuccess;
-  k_thermal_buoyancy_2d<<<blocks(count), 256, 0, s>>>(temp, gy, t_ref, beta, g_gravity, count);
+  if (accumulate) k_thermal_buoyancy_2d<true><<<blocks(count), 256, 0, s>>>(temp, gy, t_ref, beta, g_gravity, count);
+  else k_thermal_buoyancy_2d<false><<<blocks(count), 256, 0, s>>>(temp, gy, t_ref, beta, g_gravity, count);
   return hipGetLastError();
 }
 hipError_t launch_thermal_isothermal_wall(float* g, int nx, int ny, int y_wall, float t_wall, hipStream_t s) {
@@ -891,7 +894,7 @@ void thermal_compute_buoyancy_force_2d(const float temperature[], float gy_body[
   auto T = c.in(temperature, n);
   auto gy = c.inout(gy_body, n);
   if (!c.begin()) return;
-  (void)launch_thermal_buoyancy_2d(c.dev(T), c.dev(gy), t_ref, beta, g_gravity, count, nullptr);
+  (void)launch_thermal_buoyancy_2d(c.dev(T), c.dev(gy), t_ref, beta, g_gravity, count, true, nullptr);
   c.end();
 }
 // thermal_kernels.ispc:63-81 (a wall row outside [0, ny) is an error here)

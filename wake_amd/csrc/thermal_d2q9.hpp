@@ -36,6 +36,7 @@ class ThermalD2Q9 {
   int ny() const { return ny_; }
 
  private:
+  void release() noexcept;  // wait for the device, free every buffer
   int nx_, ny_;
   size_t N_;
   float tau_, t_ref_, beta_;

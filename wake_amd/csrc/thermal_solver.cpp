@@ -26,11 +26,16 @@ void ck(hipError_t e, const char* what) {
 ThermalD2Q9::ThermalD2Q9(int nx, int ny, float tau_thermal, float t_ref, float beta)
     : nx_(nx), ny_(ny), N_(static_cast<size_t>(nx) * ny), tau_(tau_thermal), t_ref_(t_ref), beta_(beta) {
   if (nx <= 0 || ny <= 0) throw HipError("thermal solver: grid dimensions must be positive");
-  ck(hipMalloc(&g_src_, 9 * N_ * sizeof(float)), "hipMalloc");
-  ck(hipMalloc(&g_dst_, 9 * N_ * sizeof(float)), "hipMalloc");
-  ck(hipMalloc(&temp_, N_ * sizeof(float)), "hipMalloc");
-  ck(hipMalloc(&gy_, N_ * sizeof(float)), "hipMalloc");
-  initialize(t_ref);
+  try {
+    ck(hipMalloc(&g_src_, 9 * N_ * sizeof(float)), "hipMalloc");
+    ck(hipMalloc(&g_dst_, 9 * N_ * sizeof(float)), "hipMalloc");
+    ck(hipMalloc(&temp_, N_ * sizeof(float)), "hipMalloc");
+    ck(hipMalloc(&gy_, N_ * sizeof(float)), "hipMalloc");
+    initialize(t_ref);
+  } catch (...) {
+    release();  // the destructor does not run for a constructor that throws
+    throw;
+  }
 }
 
 ThermalD2Q9::ThermalD2Q9(const ThermalD2Q9& o) : ThermalD2Q9(o.nx_, o.ny_, o.tau_, o.t_ref_, o.beta_) {
@@ -47,10 +52,14 @@ ThermalD2Q9::ThermalD2Q9(const ThermalD2Q9& o) : ThermalD2Q9(o.nx_, o.ny_, o.tau
 
 // Waits are device-wide: the fluid domain whose stream ran the last step
 // may already be gone (its stream handle with it).
-ThermalD2Q9::~ThermalD2Q9() {
+ThermalD2Q9::~ThermalD2Q9() { release(); }
+
+void ThermalD2Q9::release() noexcept {
   (void)hipDeviceSynchronize();
-  for (float* p : {g_src_, g_dst_, temp_, gy_})
-    if (p) (void)hipFree(p);
+  for (float** p : {&g_src_, &g_dst_, &temp_, &gy_}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
 }
 
 // thermal_solver.cpp:49-60: g = w_q T in both buffers, T = t_init, no force.
@@ -79,8 +88,9 @@ void ThermalD2Q9::step(Domain& fluid, float g_gravity) {
   if (south_) ck(launch_thermal_isothermal_wall(g_dst_, nx_, ny_, 0, t_hot_, stream_), "south wall");
   if (north_) ck(launch_thermal_isothermal_wall(g_dst_, nx_, ny_, ny_ - 1, t_cold_, stream_), "north wall");
   ck(launch_stream_walls_d2q9(g_dst_, g_src_, ct, nx_, ny_, 0.0f, 0.0f, stream_), "stream");
-  ck(hipMemsetAsync(gy_, 0, N_ * sizeof(float), stream_), "clear buoyancy");
-  ck(launch_thermal_buoyancy_2d(temp_, gy_, t_ref_, beta_, g_gravity, n, stream_), "buoyancy");
+  // thermal_solver.cpp:114-122 clears the force and adds the buoyancy; one
+  // kernel writes 0 + force (same bits)
+  ck(launch_thermal_buoyancy_2d(temp_, gy_, t_ref_, beta_, g_gravity, n, false, stream_), "buoyancy");
   h_valid_[0] = h_valid_[1] = false;
 }
 
@@ -228,6 +238,7 @@ int wake_thermal2d_read(wake_thermal2d* t, int which, float* out) {
 }
 int wake_thermal2d_stats(wake_thermal2d* t, double out[2]) {
   return tguard(t, [&](wake_hip::ThermalD2Q9& s) {
+    if (!out) throw wake_hip::HipError("thermal stats: null output");
     out[0] = s.mean_temperature();
     out[1] = s.nusselt();
   });
