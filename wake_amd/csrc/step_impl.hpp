@@ -515,9 +515,14 @@ template <class L, class St>
 struct StepWaves {
   static constexpr int value = 0;
 };
+// D3Q27 FP32 (82 VGPRs, 5 waves) is HBM-bound: a floor of 6 waves (80
+// VGPRs, 2 spilled) measures the same, 7 (72, 10 spilled) is 9% slower.
+#ifndef WAKE_Q27_F32_WAVES
+#define WAKE_Q27_F32_WAVES 0
+#endif
 template <class St>
 struct StepWaves<D3Q27, St> {
-  static constexpr int value = sizeof(typename St::T) == 2 ? WAKE_Q27_HALF_WAVES : 0;
+  static constexpr int value = sizeof(typename St::T) == 2 ? WAKE_Q27_HALF_WAVES : WAKE_Q27_F32_WAVES;
 };
 
 #if WAKE_WAVES_PER_EU > 0
