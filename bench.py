@@ -467,8 +467,10 @@ def main():
 
     from wake_amd.hbm_probe import ceilings
     ceil = ceilings(n, n, n)
+    pull_key = "pull_d3q19_gbps" if storage == 0 else "pull_d3q19_16bit_gbps"
     log(f"HBM ceilings on this GPU: contiguous copy {ceil['copy_contiguous_gbps']:.0f} GB/s, "
-        f"D3Q19 pull copy {ceil['pull_d3q19_gbps']:.0f} GB/s; step kernel {achieved:.0f} GB/s")
+        f"D3Q19 pull copy {ceil['pull_d3q19_gbps']:.0f} GB/s (16-bit {ceil['pull_d3q19_16bit_gbps']:.0f} GB/s); "
+        f"step kernel {achieved:.0f} GB/s")
 
     cpu = None
     if args.cpu_steps > 0:
@@ -518,8 +520,10 @@ def main():
             "practical_ceiling": {
                 "copy_contiguous_gbps": round(ceil["copy_contiguous_gbps"], 1),
                 "pull_d3q19_gbps": round(ceil["pull_d3q19_gbps"], 1),
-                "frac_of_pull_d3q19": round(achieved / ceil["pull_d3q19_gbps"], 4),
-                "source": f"wake_hip_hbm_probe on this GPU, {n}^3 nodes x 152 B, 20 launches each",
+                "pull_d3q19_16bit_gbps": round(ceil["pull_d3q19_16bit_gbps"], 1),
+                "frac_of_pull_same_width": round(achieved / ceil[pull_key], 4),
+                "source": f"wake_hip_hbm_probe on this GPU, {n}^3 nodes, 20 launches each (152 B/node; "
+                          f"76 B/node for the 16-bit pull)",
             },
         },
         "cpu_baseline": cpu,

@@ -48,14 +48,18 @@ int wake_hip_device_count(void);
 void wake_hip_d3q19_minv(float out[361]);
 /* Practical HBM ceilings on the current GPU, for comparison with the step
  * kernel; no counterpart in the reference (a measurement utility).  Times
- * `reps` launches (after 3 warm-up launches) of a copy moving 19 floats in
- * and out per node of an nx x ny x nz grid, 152 bytes per node like the FP32
- * D3Q19 step, and returns the average milliseconds per launch.  shape 0: a
- * contiguous copy of 19 * nx * ny * nz floats in the fastest copy shape;
- * shape 1: the D3Q19 pull over 19 SoA planes (population q from x - c_q,
- * periodic), the step's memory access without arithmetic.  Allocates and
- * frees two buffers of 76 * nx * ny * nz bytes.  Needs 1 <= nx <= 1024,
- * 1 <= nz <= 65535; returns 0 or an error code. */
+ * `reps` launches (after 3 warm-up launches) of a copy moving 19 values in
+ * and out per node of an nx x ny x nz grid and returns the average
+ * milliseconds per launch.
+ *   shape 0: a contiguous copy of 19 * nx * ny * nz floats in the fastest
+ *            copy shape (152 bytes per node);
+ *   shape 1: the D3Q19 pull over 19 SoA planes of floats (population q from
+ *            x - c_q, periodic), the FP32 step's memory access without
+ *            arithmetic (152 bytes per node);
+ *   shape 2: the same pull with 16-bit values, the FP16 / BF16 step's
+ *            access (76 bytes per node).
+ * Allocates and frees two buffers of 76 * nx * ny * nz bytes.  Needs
+ * 1 <= nx <= 1024 and 1 <= nz <= 65535; returns 0 or an error code. */
 int wake_hip_hbm_probe(int shape, int nx, int ny, int nz, int reps, float* ms_per_launch);
 
 /* ----------------------------------------------------------------------

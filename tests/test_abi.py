@@ -89,7 +89,7 @@ def test_hbm_probe_rejects_invalid_arguments():
 
     L = wake_amd._lib.lib()
     ms = ctypes.c_float(-1.0)
-    for args in ((2, 64, 64, 64, 5), (0, 0, 64, 64, 5), (1, 2048, 64, 64, 5), (1, 64, 64, 70000, 5),
+    for args in ((3, 64, 64, 64, 5), (-1, 64, 64, 64, 5), (0, 0, 64, 64, 5), (1, 2048, 64, 64, 5), (1, 64, 64, 70000, 5),
                  (0, 64, 64, 64, 0)):
         assert L.wake_hip_hbm_probe(*args, ctypes.byref(ms)) != 0, args
         assert L.wake_hip_last_error()

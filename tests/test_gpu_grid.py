@@ -559,6 +559,7 @@ def test_hbm_probe_ceilings(wa):
     assert 500.0 < c["pull_d3q19_gbps"] < 8000.0 * 1.05
     assert 500.0 < c["copy_contiguous_gbps"] < 8000.0 * 1.05
     assert c["pull_d3q19_gbps"] < 1.1 * c["copy_contiguous_gbps"]
+    assert 500.0 < c["pull_d3q19_16bit_gbps"] < 8000.0 * 1.05
     # a non-multiple-of-64 row and odd sizes (partial waves, wrap at every edge)
     c = ceilings(100, 7, 5, reps=2)
     assert all(v > 0 for v in c.values())

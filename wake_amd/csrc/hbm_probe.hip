@@ -1,4 +1,4 @@
-// hbm_probe.hip -- the two practical HBM ceilings the step kernel is measured
+// hbm_probe.hip -- the practical HBM ceilings the step kernel is measured
 // against, timed on the GPU it runs on.
 //
 // * contiguous copy: four floats per thread, the work-group's 1,024 floats
@@ -7,9 +7,11 @@
 //   copies run slower);
 // * D3Q19 pull copy: 19 SoA planes of nx*ny*nz floats, each node taking
 //   population q from x - c_q (periodic wrap) into its own slot, no
-//   arithmetic: the step kernel's memory access with nothing else.
+//   arithmetic: the step kernel's memory access with nothing else;
+// * the same pull with 16-bit values: the FP16 / BF16 step's access.
 //
-// Both move 19 * 4 bytes in and out per node, the FP32 step's 152 B/node.
+// The first two move 19 * 4 bytes in and out per node, the FP32 step's
+// 152 B/node; the third 19 * 2, the 16-bit step's 76 B/node.
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -40,15 +42,17 @@ __global__ __launch_bounds__(kCopyBlock) void k_probe_stream(const float* __rest
   }
 }
 
-// One work-group per (row y, z), one thread per x; nx <= 1024.
-__global__ __launch_bounds__(1024) void k_probe_pull19(const float* __restrict__ a, float* __restrict__ b, int nx,
-                                                       int ny, int nz) {
+// One work-group per (row y, z), one thread per x; nx <= 1024.  T: float
+// (the FP32 step's access) or a 16-bit value (the FP16 / BF16 step's).
+template <class T>
+__global__ __launch_bounds__(1024) void k_probe_pull19(const T* __restrict__ a, T* __restrict__ b, int nx, int ny,
+                                                       int nz) {
   using L = D3Q19;
   const int x = threadIdx.x, y = blockIdx.x, z = blockIdx.y;
   if (x >= nx) return;
   const long long plane = static_cast<long long>(nx) * ny * nz;
   const long long n = (static_cast<long long>(z) * ny + y) * nx + x;
-  float v[L::Q];
+  T v[L::Q];
 #pragma unroll
   for (int q = 0; q < L::Q; ++q) {
     int sx = x - L::cx[q], sy = y - L::cy[q], sz = z - L::cz[q];
@@ -63,7 +67,7 @@ __global__ __launch_bounds__(1024) void k_probe_pull19(const float* __restrict__
 
 hipError_t run_probe(int shape, int nx, int ny, int nz, int reps, float* ms_out) {
   const long long count = 19LL * nx * ny * nz;
-  float* a = nullptr;
+  float* a = nullptr;  // count floats; shape 2 uses the first count halves
   float* b = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipError_t err = hipMalloc(&a, count * sizeof(float));
@@ -75,8 +79,11 @@ hipError_t run_probe(int shape, int nx, int ny, int nz, int reps, float* ms_out)
     if (shape == 0) {
       const long long per = kCopyBlock * kCopyPerThread;
       k_probe_stream<<<static_cast<unsigned>((count + per - 1) / per), kCopyBlock>>>(a, b, count);
+    } else if (shape == 1) {
+      k_probe_pull19<float><<<dim3(ny, nz), nx>>>(a, b, nx, ny, nz);
     } else {
-      k_probe_pull19<<<dim3(ny, nz), nx>>>(a, b, nx, ny, nz);
+      k_probe_pull19<unsigned short><<<dim3(ny, nz), nx>>>(reinterpret_cast<const unsigned short*>(a),
+                                                           reinterpret_cast<unsigned short*>(b), nx, ny, nz);
     }
   };
   if (err == hipSuccess) {
@@ -105,7 +112,7 @@ hipError_t run_probe(int shape, int nx, int ny, int nz, int reps, float* ms_out)
 
 extern "C" int wake_hip_hbm_probe(int shape, int nx, int ny, int nz, int reps, float* ms_per_launch) {
   if (!ms_per_launch) return wake_hip::set_last_error("wake_hip_hbm_probe: null output");
-  if (shape != 0 && shape != 1) return wake_hip::set_last_error("wake_hip_hbm_probe: shape must be 0 or 1");
+  if (shape < 0 || shape > 2) return wake_hip::set_last_error("wake_hip_hbm_probe: shape must be 0, 1 or 2");
   if (nx < 1 || ny < 1 || nz < 1 || nx > 1024 || reps < 1)
     return wake_hip::set_last_error("wake_hip_hbm_probe: need 1 <= nx <= 1024, ny, nz >= 1 and reps >= 1");
   if (nz > 65535) return wake_hip::set_last_error("wake_hip_hbm_probe: nz above the 65,535 grid limit");

@@ -11,22 +11,24 @@ relate to bytes moved on this GPU.
 import argparse
 import ctypes
 
-BYTES_PER_NODE = 152  # 19 floats read and written per node
+BYTES_PER_NODE = 152  # 19 floats read and written per node (FP32)
 
 
 def ceilings(nx=256, ny=256, nz=256, reps=20):
-    """The two practical HBM ceilings on the current GPU, in GB/s at 152
-    bytes per node (wake_hip_hbm_probe): a contiguous copy in the fastest
-    copy shape, and the D3Q19 pull over 19 SoA planes without arithmetic."""
+    """The practical HBM ceilings on the current GPU, in GB/s
+    (wake_hip_hbm_probe): a contiguous copy in the fastest copy shape and
+    the D3Q19 pull over 19 SoA planes of floats, both at 152 bytes per node,
+    and the same pull with 16-bit values at 76 bytes per node."""
     from . import _lib
 
     L = _lib.lib()
     out = {}
-    for shape, key in ((0, "copy_contiguous_gbps"), (1, "pull_d3q19_gbps")):
+    for shape, key, bpn in ((0, "copy_contiguous_gbps", BYTES_PER_NODE), (1, "pull_d3q19_gbps", BYTES_PER_NODE),
+                            (2, "pull_d3q19_16bit_gbps", BYTES_PER_NODE // 2)):
         ms = ctypes.c_float()
         if L.wake_hip_hbm_probe(shape, nx, ny, nz, reps, ctypes.byref(ms)) != 0:
             raise RuntimeError(L.wake_hip_last_error().decode())
-        out[key] = BYTES_PER_NODE * nx * ny * nz / (ms.value * 1e-3) / 1e9
+        out[key] = bpn * nx * ny * nz / (ms.value * 1e-3) / 1e9
     return out
 
 
