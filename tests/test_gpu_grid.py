@@ -563,3 +563,33 @@ def test_hbm_probe_ceilings(wa):
     # a non-multiple-of-64 row and odd sizes (partial waves, wrap at every edge)
     c = ceilings(100, 7, 5, reps=2)
     assert all(v > 0 for v in c.values())
+
+
+def test_failed_allocation_releases_memory(wa):
+    """A domain too large for HBM (1290^3 D3Q19 FP32: two 163 GB population
+    buffers on a 288 GB GPU) fails with an error, frees what it had
+    allocated before the failing hipMalloc, and leaves the library usable."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")  # the runtime libwake_hip.so already loaded
+
+    def mem_info():
+        free, total = ctypes.c_size_t(), ctypes.c_size_t()
+        assert hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0
+        return free.value, total.value
+
+    free0, total = mem_info()
+    n = 1290
+    need = 2 * 19 * n ** 3 * 4
+    if need < total:
+        pytest.skip(f"this GPU holds {total / 1e9:.0f} GB: the domain would fit")
+    with pytest.raises(RuntimeError):
+        wa.LatticeGrid3D(n, n, n, 0.6)
+    wa._lib.lib().wake_hip_clear_error()
+    free1, _ = mem_info()
+    print(f"free before {free0 / 1e9:.2f} GB, after the failed create {free1 / 1e9:.2f} GB")
+    assert free1 > free0 - 256 * 2 ** 20  # nothing of the first 163 GB buffer is left behind
+    g = wa.LatticeGrid3D(16, 12, 10, 0.6)
+    g.initialize(1.0, 0.02, 0.0, 0.0)
+    g.step("BGK", 0.0, 0.0, 0.0, False, 0.14)
+    assert np.isfinite(g.ux_data()).all()

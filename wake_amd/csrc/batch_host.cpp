@@ -37,12 +37,17 @@ class Batch {
     if (lattice != 9 && lattice != 19 && lattice != 27) throw HipError("batch lattice must be 9, 19 or 27");
     if (thermal && lattice != 9) throw HipError("thermal batches are D2Q9 + D2Q5");
     if (chunk < 1 || chunk > 4096) throw HipError("batch chunk must be in [1, 4096]");
-    check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
-    int dev = 0;
-    check(hipGetDevice(&dev), "hipGetDevice");
-    check(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev), "device attribute");
-    check(hipEventCreate(&e0_), "event");
-    check(hipEventCreate(&e1_), "event");
+    try {
+      check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+      int dev = 0;
+      check(hipGetDevice(&dev), "hipGetDevice");
+      check(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev), "device attribute");
+      check(hipEventCreate(&e0_), "event");
+      check(hipEventCreate(&e1_), "event");
+    } catch (...) {
+      release();  // the destructor does not run for a constructor that throws
+      throw;
+    }
   }
 
   ~Batch() {
@@ -54,14 +59,7 @@ class Batch {
       } catch (...) {
       }
     }
-    drop_graph();
-    if (d_slots_) (void)hipFree(d_slots_);
-    if (d_tiles_) (void)hipFree(d_tiles_);
-    if (d_hist_) (void)hipFree(d_hist_);
-    if (h_hist_) (void)hipHostFree(h_hist_);
-    if (e0_) (void)hipEventDestroy(e0_);
-    if (e1_) (void)hipEventDestroy(e1_);
-    if (stream_) (void)hipStreamDestroy(stream_);
+    release();
   }
 
   int add(Domain* d, uint64_t steps, float gx, float gy, float gz, float smag, bool forces, float cx, float cy,
@@ -263,6 +261,26 @@ class Batch {
   static std::mutex& cache_mutex() {
     static std::mutex m;
     return m;
+  }
+
+  // Free every device and host resource (the stream has drained).
+  void release() noexcept {
+    drop_graph();
+    if (d_slots_) (void)hipFree(d_slots_);
+    if (d_tiles_) (void)hipFree(d_tiles_);
+    if (d_hist_) (void)hipFree(d_hist_);
+    if (h_hist_) (void)hipHostFree(h_hist_);
+    if (e0_) (void)hipEventDestroy(e0_);
+    if (e1_) (void)hipEventDestroy(e1_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+    d_slots_ = nullptr;
+    d_tiles_ = nullptr;
+    d_hist_ = nullptr;
+    h_hist_ = nullptr;
+    e0_ = nullptr;
+    e1_ = nullptr;
+    stream_ = nullptr;
+    (void)hipGetLastError();  // nothing of the teardown reaches the next launch check
   }
 
   void drop_graph() {

@@ -11,8 +11,15 @@
 
 namespace wake_hip {
 
+// A failed runtime call also stays the thread's "last error" until read,
+// and kernel launches are checked with hipGetLastError(); reset it here so
+// a handled failure (e.g. an allocation refused for lack of HBM) is not
+// reported again by the next launch.
 void check(hipError_t e, const char* what) {
-  if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+  }
 }
 
 namespace {
@@ -79,6 +86,15 @@ Domain::Domain(int dim, int lattice, int nx, int ny, int nz, float tau, int stor
   const long long base = static_cast<long long>((N_ + 255) / 256 * 256);
   stride_ = base + (plane_pad >= 0 ? plane_pad : (N_ >= (1u << 20) ? 1024 : 0));
   is_periodic_ = (dim == 3);
+  try {
+    allocate_and_initialize();
+  } catch (...) {
+    release();  // the destructor does not run for a constructor that throws
+    throw;
+  }
+}
+
+void Domain::allocate_and_initialize() {
   check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
   for (int b = 0; b < 2; ++b) buf_[b] = alloc(static_cast<size_t>(Q_) * stride_ * elem_);
   d_flags_ = static_cast<uint8_t*>(alloc(N_));
@@ -101,11 +117,17 @@ Domain::Domain(int dim, int lattice, int nx, int ny, int nz, float tau, int stor
   initialize(1.0f, 0.0f, 0.0f, 0.0f);
 }
 
-Domain::~Domain() {
+Domain::~Domain() { release(); }
+
+void Domain::release() noexcept {
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (hipEvent_t e : kev_) (void)hipEventDestroy(e);
+  kev_.clear();
   for (void* p : allocations_) (void)hipFree(p);
+  allocations_.clear();
   if (stream_) (void)hipStreamDestroy(stream_);
+  stream_ = nullptr;
+  (void)hipGetLastError();  // nothing of the teardown reaches the next launch check
 }
 
 void* Domain::alloc(size_t bytes) {

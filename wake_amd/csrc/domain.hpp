@@ -191,6 +191,8 @@ class Domain {
   void invalidate_mirrors() { fields_valid_ = dfields_valid_ = fsrc_valid_ = fdst_valid_ = an_valid_ = false; }
   void ensure_device_fields();
   void* alloc(size_t bytes);
+  void allocate_and_initialize();  // constructor body after validation
+  void release() noexcept;         // wait for the stream, free everything
   void release(void* p);
   void upload_prev(const std::function<float(size_t, int)>& value);
   const std::pair<std::vector<int>, std::vector<int>>& mem_link_list(float cx, float cy, float cz, float r_max);

@@ -117,6 +117,7 @@ extern "C" int wake_hip_hbm_probe(int shape, int nx, int ny, int nz, int reps, f
     return wake_hip::set_last_error("wake_hip_hbm_probe: need 1 <= nx <= 1024, ny, nz >= 1 and reps >= 1");
   if (nz > 65535) return wake_hip::set_last_error("wake_hip_hbm_probe: nz above the 65,535 grid limit");
   const hipError_t err = wake_hip::run_probe(shape, nx, ny, nz, reps, ms_per_launch);
+  (void)hipGetLastError();  // a failure here is reported once, not by the next launch check
   if (err != hipSuccess)
     return wake_hip::set_last_error(std::string("wake_hip_hbm_probe: ") + hipGetErrorString(err));
   return 0;

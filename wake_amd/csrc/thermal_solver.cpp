@@ -18,8 +18,12 @@
 namespace wake_hip {
 
 namespace {
+// Resets the runtime's last error as check() does (domain.cpp).
 void ck(hipError_t e, const char* what) {
-  if (e != hipSuccess) throw HipError(std::string("thermal solver: ") + what + ": " + hipGetErrorString(e));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw HipError(std::string("thermal solver: ") + what + ": " + hipGetErrorString(e));
+  }
 }
 }  // namespace
 
@@ -60,6 +64,7 @@ void ThermalD2Q9::release() noexcept {
     if (*p) (void)hipFree(*p);
     *p = nullptr;
   }
+  (void)hipGetLastError();
 }
 
 // thermal_solver.cpp:49-60: g = w_q T in both buffers, T = t_init, no force.
