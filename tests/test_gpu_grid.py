@@ -391,23 +391,24 @@ def test_d3q27_bgk_periodic(wa):
     p.check_fields()
 
 
+@pytest.mark.parametrize("lattice,coll", [(19, "MRT"), (27, "TRT")])
 @pytest.mark.parametrize("storage,name", [(1, "fp16"), (2, "bf16")])
-def test_reduced_precision_storage_vs_fp64(wa, storage, name):
+def test_reduced_precision_storage_vs_fp64(wa, storage, name, lattice, coll):
     """FP16/BF16 storage of f - w with FP32 arithmetic against the FP64
     oracle; the FP32-storage run is reported beside it."""
     nx, ny, nz, R = 48, 24, 24, 5.0
     errs = {}
     for st in (0, storage):
-        p = Pair(wa, 3, nx, ny, nz, tau=0.6, storage=st, bits=64)
+        p = Pair(wa, 3, nx, ny, nz, tau=0.6, storage=st, bits=64, lattice=lattice)
         p.both("initialize", 1.0, 0.04, 0.0, 0.0)
         p.both("set_solid_sphere", 12.0, 12.0, 12.0, R)
         p.both("build_curved_sphere_links", 12.0, 12.0, 12.0, R)
-        p.step(300, "MRT")
+        p.step(300, coll)
         gu, ou = p.fields()
         errs[st] = rel_l2(np.concatenate(gu[1:]), np.concatenate(ou[1:]))
     assert errs[0] <= FIELD_TOL
     bound = 1e-3 if name == "fp16" else 1e-2
-    print(f"{name}: velocity rel L2 vs FP64 = {errs[storage]:.3e} (fp32 storage: {errs[0]:.3e})")
+    print(f"D3Q{lattice} {coll} {name}: velocity rel L2 vs FP64 = {errs[storage]:.3e} (fp32 storage: {errs[0]:.3e})")
     assert errs[storage] <= bound, f"{name} velocity rel L2 vs FP64 {errs[storage]:.2e}"
 
 
