@@ -109,3 +109,56 @@ def test_missing_library_raises(tmp_path, monkeypatch):
         wl.lib()
     monkeypatch.undo()
     importlib.reload(wl)
+
+
+def test_kernel_exports_on_empty_ranges_do_nothing():
+    """An ISPC `foreach` over an empty range does nothing, so the exports
+    return at once for count <= 0 or a zero dimension: no GPU call (none is
+    usable here, so any would record an error), outputs untouched, and the
+    momentum-exchange sums come back as zero."""
+    import wake_amd
+
+    K = wake_amd.kernels
+    f = np.full(27 * 4, 7.0, np.float32)
+    out = np.full(27 * 4, 3.0, np.float32)
+    r, u, v, w = (np.full(4, 5.0, np.float32) for _ in range(4))
+    ct = np.zeros(4, np.uint8)
+    for count in (0, -3):
+        K.kernel("d2q9_compute_macroscopic_moments", f, r, u, v, count)
+        K.kernel("d3q19_collide_mrt", f, out, ct, r, u, v, w, 0.6, 1.19, 0.0, 0.0, 0.0, count)
+        K.kernel("d3q27_collide", f, out, ct, r, u, v, w, 0.6, 0.7, 0.0, 0.0, 0.0, count)
+        K.kernel("d3q19_compute_smagorinsky_tau", f, r, u, v, w, out, 0.6, 0.17, count)
+        K.kernel("d2q9_compute_velocity_magnitude", u, v, out, count)
+        K.kernel("d2q9_apply_bounce_back", f, ct, 0.0, 0.0, count)
+        K.kernel("thermal_collide_d2q9", f, out, ct, u, v, r, 0.8, count)
+        K.kernel("thermal_compute_buoyancy_force_2d", r, out, 0.5, 0.05, -0.001, count)
+    for dims in ((0, 4, 4), (4, 0, 4), (4, 4, 0)):
+        K.kernel("d3q19_stream_periodic", f, out, *dims)
+        K.kernel("d3q27_stream_periodic", f, out, *dims)
+        K.kernel("d3q19_stream_with_boundaries", f, out, ct, *dims, 0.0, 0.0, 0.0)
+        K.kernel("compute_vorticity_and_q_criterion_3d", u, v, w, r, r, r, r, *dims)
+    K.kernel("d2q9_apply_zou_he_inflow_west", f, r, u, v, 0.05, 0.0, 4, 2)
+    K.kernel("d2q9_apply_zou_he_outflow_east", f, r, u, v, 1.0, 0, 4)
+    K.kernel("d2q9_apply_convective_outflow_east", f, f, 0.05, 4, 1)
+    assert (out == 3.0).all() and (r == 5.0).all() and (u == 5.0).all() and (f == 7.0).all()
+    F = [np.full(1, 9.0, np.float32) for _ in range(3)]
+    K.kernel("d3q19_compute_momentum_exchange_forces", f, ct, 0, 4, 4, 1.0, 1.0, 1.0, 2.0, *F)
+    assert all(x[0] == 0.0 for x in F)
+    K.kernel("d3q19_apply_curved_bfl", f, f, np.zeros(1, np.int32), np.ones(1, np.int32),
+             np.full(1, 0.5, np.float32), r, r, r, 4, 4, 4, 0)
+
+
+def test_curved_bfl_export_rejects_links_outside_the_grid():
+    """The BFL kernel addresses f by the caller's link node and direction;
+    a link outside the grid is an error before any GPU call."""
+    import wake_amd
+
+    K = wake_amd.kernels
+    n = 4 * 4 * 4
+    f = np.zeros(19 * n, np.float32)
+    d = np.full(2, 0.5, np.float32)
+    z = np.zeros(2, np.float32)
+    for nodes, qs in (([0, n], [1, 2]), ([0, -1], [1, 2]), ([0, 5], [1, 19]), ([0, 5], [-2, 3])):
+        with pytest.raises(RuntimeError, match="outside"):
+            K.kernel("d3q19_apply_curved_bfl", f, f, np.array(nodes, np.int32), np.array(qs, np.int32), d, z, z, z,
+                     4, 4, 4, 2)

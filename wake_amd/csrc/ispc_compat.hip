@@ -421,6 +421,10 @@ constexpr int kMemBlocks = 256;
 template <class L>
 void mem_grid(const char* name, const float* f, const uint8_t* ct, int nx, int ny, int nz, float cx, float cy,
               float cz, float rmax, float* o0, float* o1, float* o2) {
+  if (nx <= 0 || ny <= 0 || nz <= 0) {  // no node: the sums are zero
+    *o0 = *o1 = *o2 = 0.0f;
+    return;
+  }
   const long long N = static_cast<long long>(nx) * ny * nz;
   float res[3] = {0, 0, 0};
   std::vector<double> partial(3 * kMemBlocks);
@@ -441,6 +445,7 @@ void mem_grid(const char* name, const float* f, const uint8_t* ct, int nx, int n
 template <class L, int COLL, bool REF_PAIRS = false>
 void collide_call(const char* name, const float* f_in, float* f_out, const uint8_t* ct, float* rho, float* ux,
                   float* uy, float* uz, float tau, float tau2, float gx, float gy, float gz, int count) {
+  if (count <= 0) return;
   Call c(name);
   const size_t n = static_cast<size_t>(count);
   auto fi = c.in(f_in, L::Q * n);
@@ -459,6 +464,7 @@ void collide_call(const char* name, const float* f_in, float* f_out, const uint8
 template <class L, bool FORCED>
 void moments_call(const char* name, const float* f, float* rho, float* ux, float* uy, float* uz, float gx, float gy,
                   float gz, int count) {
+  if (count <= 0) return;
   Call c(name);
   const size_t n = static_cast<size_t>(count);
   auto fi = c.in(f, L::Q * n);
@@ -473,6 +479,7 @@ void moments_call(const char* name, const float* f, float* rho, float* ux, float
 
 template <class L>
 void stream_periodic_call(const char* name, const float* src, float* dst, int nx, int ny, int nz) {
+  if (nx <= 0 || ny <= 0 || nz <= 0) return;
   Call c(name);
   const size_t N = static_cast<size_t>(nx) * ny * nz;
   auto s = c.in(src, L::Q * N);
@@ -486,6 +493,7 @@ void stream_periodic_call(const char* name, const float* src, float* dst, int nx
 template <class L>
 void stream_walls_call(const char* name, const float* src, float* dst, const uint8_t* ct, int nx, int ny, int nz,
                        float uwx, float uwy, float uwz) {
+  if (nx <= 0 || ny <= 0 || nz <= 0) return;
   Call c(name);
   const size_t N = static_cast<size_t>(nx) * ny * nz;
   auto s = c.in(src, L::Q * N);
@@ -500,6 +508,7 @@ void stream_walls_call(const char* name, const float* src, float* dst, const uin
 template <class L>
 void les_call(const char* name, const float* f, const float* rho, const float* ux, const float* uy, const float* uz,
               float* tau_eff, float tau0, float cs, int count) {
+  if (count <= 0) return;
   Call c(name);
   const size_t n = static_cast<size_t>(count);
   auto fi = c.in(f, L::Q * n);
@@ -675,6 +684,7 @@ void d2q9_compute_macroscopic_moments_forced(const float f[], float rho[], float
 }
 // analytics_kernels.ispc:4-29
 void compute_vorticity_2d(const float ux[], const float uy[], float vorticity[], int nx, int ny) {
+  if (nx <= 0 || ny <= 0) return;
   Call c(__func__);
   const size_t n = static_cast<size_t>(nx) * ny;
   auto a = c.in(ux, n);
@@ -688,6 +698,7 @@ void compute_vorticity_2d(const float ux[], const float uy[], float vorticity[],
 void compute_vorticity_and_q_criterion_3d(const float ux[], const float uy[], const float uz[], float omega_x[],
                                           float omega_y[], float omega_z[], float q_criterion[], int nx, int ny,
                                           int nz) {
+  if (nx <= 0 || ny <= 0 || nz <= 0) return;
   Call c(__func__);
   const size_t n = static_cast<size_t>(nx) * ny * nz;
   auto a = c.in(ux, n);
@@ -703,6 +714,7 @@ void compute_vorticity_and_q_criterion_3d(const float ux[], const float uy[], co
   c.end();
 }
 void d2q9_compute_velocity_magnitude(const float ux[], const float uy[], float speed[], int count) {
+  if (count <= 0) return;
   Call c(__func__);
   auto a = c.in(ux, count);
   auto b = c.in(uy, count);
@@ -732,6 +744,7 @@ void d2q9_stream_with_boundaries(const float f_src[], float f_dst[], const uint8
   stream_walls_call<D2Q9>(__func__, f_src, f_dst, cell_type, nx, ny, 1, uw_x, uw_y, 0.0f);
 }
 void d2q9_apply_bounce_back(float f[], const uint8_t cell_type[], float uw_x, float uw_y, int count) {
+  if (count <= 0) return;
   Call c(__func__);
   auto fi = c.inout(f, 9 * static_cast<size_t>(count));
   auto t = c.in(cell_type, count);
@@ -741,6 +754,7 @@ void d2q9_apply_bounce_back(float f[], const uint8_t cell_type[], float uw_x, fl
 }
 void d2q9_apply_zou_he_inflow_west(float f[], float rho[], float ux[], float uy[], float u_in_x, float u_in_y,
                                    int nx, int ny) {
+  if (nx <= 0 || ny <= 2) return;  // rows 1 .. ny-2 only
   const size_t N = static_cast<size_t>(nx) * ny;
   Call c(__func__);
   auto fi = c.inout(f, 9 * N);
@@ -753,6 +767,7 @@ void d2q9_apply_zou_he_inflow_west(float f[], float rho[], float ux[], float uy[
 }
 void d2q9_apply_zou_he_outflow_east(float f[], float rho[], float ux[], float uy[], float rho_out, int nx,
                                     int ny) {
+  if (nx <= 0 || ny <= 2) return;  // rows 1 .. ny-2 only
   const size_t N = static_cast<size_t>(nx) * ny;
   Call c(__func__);
   auto fi = c.inout(f, 9 * N);
@@ -764,6 +779,7 @@ void d2q9_apply_zou_he_outflow_east(float f[], float rho[], float ux[], float uy
   c.end();
 }
 void d2q9_apply_convective_outflow_east(float f[], const float f_prev[], float u_conv, int nx, int ny) {
+  if (nx <= 0 || ny <= 2) return;  // rows 1 .. ny-2 only
   const size_t N = static_cast<size_t>(nx) * ny;
   Call c(__func__);
   auto fi = c.inout(f, 9 * N);
@@ -793,6 +809,7 @@ void d3q19_compute_macroscopic_moments_forced(const float f[], float rho[], floa
 }
 void d3q19_compute_velocity_magnitude(const float ux[], const float uy[], const float uz[], float speed[],
                                       int count) {
+  if (count <= 0) return;
   Call c(__func__);
   auto a = c.in(ux, count);
   auto b = c.in(uy, count);
@@ -831,8 +848,21 @@ void d3q19_compute_smagorinsky_tau(const float f[], const float rho[], const flo
 void d3q19_apply_curved_bfl(float f_dst[], const float f_src[], const int fluid_indices[], const int q_dirs[],
                             const float delta_fractions[], const float uw_x[], const float uw_y[],
                             const float uw_z[], int nx, int ny, int nz, int num_boundary_links) {
+  if (nx <= 0 || ny <= 0 || nz <= 0 || num_boundary_links <= 0) return;
   const size_t N = static_cast<size_t>(nx) * ny * nz;
-  const size_t L = static_cast<size_t>(num_boundary_links > 0 ? num_boundary_links : 0);
+  const size_t L = static_cast<size_t>(num_boundary_links);
+  // The kernel addresses f by the caller's node indices and directions:
+  // check them here (the reference reads out of its arrays on a bad link).
+  if (!fluid_indices || !q_dirs || !delta_fractions || !uw_x || !uw_y || !uw_z) {
+    record_error("wake: d3q19_apply_curved_bfl: null link array");
+    return;
+  }
+  for (size_t k = 0; k < L; ++k)
+    if (fluid_indices[k] < 0 || static_cast<size_t>(fluid_indices[k]) >= N || q_dirs[k] < 0 || q_dirs[k] >= 19) {
+      record_error("wake: d3q19_apply_curved_bfl: link " + std::to_string(k) + " has a node index outside [0, " +
+                   std::to_string(N) + ") or a direction outside [0, 19)");
+      return;
+    }
   Call c(__func__);
   auto d = c.inout(f_dst, 19 * N);
   auto s = c.in(f_src, 19 * N);
@@ -843,8 +873,7 @@ void d3q19_apply_curved_bfl(float f_dst[], const float f_src[], const int fluid_
   auto ay = c.in(uw_y, L);
   auto az = c.in(uw_z, L);
   if (!c.begin()) return;
-  if (L)
-    k_bfl_d3q19<<<blocks(static_cast<long long>(L)), 256>>>(c.dev(d), c.dev(s), c.dev(i), c.dev(q), c.dev(dl),
+  k_bfl_d3q19<<<blocks(static_cast<long long>(L)), 256>>>(c.dev(d), c.dev(s), c.dev(i), c.dev(q), c.dev(dl),
                                                             c.dev(ax), c.dev(ay), c.dev(az), nx, ny, nz,
                                                             num_boundary_links);
   c.end();
@@ -872,8 +901,9 @@ void d3q27_compute_macroscopic_moments(const float f[], float rho[], float ux[],
 // thermal_kernels.ispc:4-44
 void thermal_collide_d2q9(const float g_in[], float g_out[], const uint8_t cell_type[], const float ux[],
                           const float uy[], float temperature[], float tau_T, int count) {
+  if (count <= 0) return;
   Call c(__func__);
-  const size_t n = static_cast<size_t>(count > 0 ? count : 0);
+  const size_t n = static_cast<size_t>(count);
   auto gi = c.in(g_in, 9 * n);
   auto go = c.out(g_out, 9 * n);
   auto t = c.in(cell_type, n);
@@ -889,8 +919,9 @@ void thermal_collide_d2q9(const float g_in[], float g_out[], const uint8_t cell_
 // thermal_kernels.ispc:48-60
 void thermal_compute_buoyancy_force_2d(const float temperature[], float gy_body[], float t_ref, float beta,
                                        float g_gravity, int count) {
+  if (count <= 0) return;
   Call c(__func__);
-  const size_t n = static_cast<size_t>(count > 0 ? count : 0);
+  const size_t n = static_cast<size_t>(count);
   auto T = c.in(temperature, n);
   auto gy = c.inout(gy_body, n);
   if (!c.begin()) return;
