@@ -2,6 +2,7 @@
 // SimulationRunner, compute_aerodynamic_coefficients) the way the
 // reference's own validation tests use it, on the GPU.  Exit status 0 means
 // every check passed.
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -12,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "wake_hip.h"
 #include "wake/analytics/field_analytics.hpp"
 #include "wake/core/error.hpp"
 #include "wake/export/csv_telemetry.hpp"
@@ -109,6 +111,29 @@ static void sphere_forces() {
   EXPECT(std::fabs(c.cl) < 0.05f * c.cd, "C_L %g vs C_D %g", c.cl, c.cd);
   grid.check_finite();
   std::printf("sphere_forces: C_D %g C_L %g\n", c.cd, c.cl);
+
+  // The reference's scenarios call the kernel export on the grid's own
+  // arrays (sphere_3d_wake.cpp:98-105).  The grid's f_src mirror is reduced
+  // from its device copy; a host copy of the same bytes is uploaded.  Both
+  // must give the same bits.
+  const size_t n19 = static_cast<size_t>(19) * nx * ny * nz;
+  float f1[3], f2[3];
+  const real_t* mirror = grid.f_src_data();  // the readback happens here, outside the timing
+  const std::vector<float> copy(mirror, mirror + n19);
+  auto t0 = std::chrono::steady_clock::now();
+  d3q19_compute_momentum_exchange_forces(mirror, grid.cell_type_data(), nx, ny, nz, cx, cy, cz, 2.0f * R, &f1[0],
+                                         &f1[1], &f1[2]);
+  auto t1 = std::chrono::steady_clock::now();
+  auto t2 = std::chrono::steady_clock::now();
+  d3q19_compute_momentum_exchange_forces(copy.data(), grid.cell_type_data(), nx, ny, nz, cx, cy, cz, 2.0f * R, &f2[0],
+                                         &f2[1], &f2[2]);
+  auto t3 = std::chrono::steady_clock::now();
+  EXPECT(std::memcmp(f1, f2, sizeof(f1)) == 0, "MEM export on the grid mirror (%g %g %g) vs a host copy (%g %g %g)",
+         f1[0], f1[1], f1[2], f2[0], f2[1], f2[2]);
+  EXPECT(f1[0] > 0.0f, "MEM export drag %g", f1[0]);
+  std::printf("MEM export: grid mirror %.2f ms (resident), host copy %.2f ms (uploaded)\n",
+              std::chrono::duration<double, std::milli>(t1 - t0).count(),
+              std::chrono::duration<double, std::milli>(t3 - t2).count());
 }
 
 // Runner: callbacks once per step, MLUPS average, max_steps respected.

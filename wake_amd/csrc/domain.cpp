@@ -6,6 +6,7 @@
 #include <cstring>
 #include <functional>
 #include <limits>
+#include <mutex>
 
 #include "lattice.hpp"
 
@@ -69,6 +70,22 @@ size_t elem_size(int storage) { return storage == kStoreF32 ? 4 : 2; }
 
 }  // namespace
 
+// Live domains, for device_copy_of_host_populations().
+namespace {
+std::mutex g_live_mutex;
+std::vector<const Domain*> g_live;
+}  // namespace
+
+const float* device_copy_of_host_populations(const float* host, size_t count) {
+  if (!host) return nullptr;
+  std::lock_guard<std::mutex> lock(g_live_mutex);
+  for (const Domain* d : g_live)
+    if (d->fsrc_valid_ && d->sout_is_fsrc_ && d->d_sout_ && host == d->h_fsrc_.data() &&
+        count == static_cast<size_t>(d->Q_) * d->N_)
+      return d->d_sout_;
+  return nullptr;
+}
+
 Domain::Domain(int dim, int lattice, int nx, int ny, int nz, float tau, int storage, long long plane_pad)
     : dim_(dim), lattice_(lattice), nx_(nx), ny_(ny), nz_(dim == 2 ? 1 : nz), storage_(storage), tau_(tau) {
   if (dim != 2 && dim != 3) throw HipError("dimension must be 2 or 3");
@@ -92,6 +109,8 @@ Domain::Domain(int dim, int lattice, int nx, int ny, int nz, float tau, int stor
     release();  // the destructor does not run for a constructor that throws
     throw;
   }
+  std::lock_guard<std::mutex> lock(g_live_mutex);
+  g_live.push_back(this);
 }
 
 void Domain::allocate_and_initialize() {
@@ -120,6 +139,10 @@ void Domain::allocate_and_initialize() {
 Domain::~Domain() { release(); }
 
 void Domain::release() noexcept {
+  {
+    std::lock_guard<std::mutex> lock(g_live_mutex);
+    g_live.erase(std::remove(g_live.begin(), g_live.end(), this), g_live.end());
+  }
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (hipEvent_t e : kev_) (void)hipEventDestroy(e);
   kev_.clear();
@@ -206,6 +229,7 @@ void Domain::initialize(float rho, float ux, float uy, float uz) {
   }
   fields_valid_ = true;
   fsrc_valid_ = fdst_valid_ = false;
+  sout_is_fsrc_ = false;
   check(hipStreamSynchronize(stream_), "initialize");
 }
 
@@ -233,6 +257,7 @@ void Domain::upload_state_from_host(const std::vector<float>& f_ref) {
   a_buf_ = -1;
   h_fsrc_ = f_ref;
   fsrc_valid_ = (storage_ == kStoreF32);
+  sout_is_fsrc_ = false;
   fdst_valid_ = false;
 }
 
@@ -898,6 +923,7 @@ const float* Domain::f_src() {
   }
   check(hipStreamSynchronize(stream_), "readback");
   fsrc_valid_ = true;
+  sout_is_fsrc_ = true;
   return h_fsrc_.data();
 }
 
@@ -920,6 +946,7 @@ const float* Domain::f_dst() {
   check(hipStreamSynchronize(stream_), "readback");
   fdst_valid_ = true;
   fsrc_valid_ = false;  // d_sout_ now holds f_dst
+  sout_is_fsrc_ = false;
   return h_fdst_.data();
 }
 

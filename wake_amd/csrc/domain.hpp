@@ -188,9 +188,13 @@ class Domain {
   void sync_device_config();
   void ensure_fields();
   void upload_state_from_host(const std::vector<float>& f_ref);
-  void invalidate_mirrors() { fields_valid_ = dfields_valid_ = fsrc_valid_ = fdst_valid_ = an_valid_ = false; }
+  void invalidate_mirrors() {
+    fields_valid_ = dfields_valid_ = fsrc_valid_ = fdst_valid_ = an_valid_ = false;
+    sout_is_fsrc_ = false;
+  }
   void ensure_device_fields();
   void* alloc(size_t bytes);
+  friend const float* device_copy_of_host_populations(const float* host, size_t count);
   void allocate_and_initialize();  // constructor body after validation
   void release() noexcept;         // wait for the stream, free everything
   void release(void* p);
@@ -271,6 +275,7 @@ class Domain {
   std::vector<float> h_rho_, h_ux_, h_uy_, h_uz_, h_speed_, h_fsrc_, h_fdst_;
   std::vector<float> h_prev_;  // staging of the outlet store (kept until the upload completes)
   bool fields_valid_ = false, fsrc_valid_ = false, fdst_valid_ = false;
+  bool sout_is_fsrc_ = false;  // d_sout_ holds the bytes of the current f_src() mirror
   bool dfields_valid_ = false;  // device moment arrays current (implied by fields_valid_)
 };
 
@@ -289,5 +294,11 @@ inline uint64_t splitmix64(uint64_t x) {
 // written.
 bool write_vti(Domain& d, const std::string& path, bool binary, const float* vort2d, const float* qcrit,
                const float* vort3d);
+
+// The device buffer that holds the same bytes as `host`, when `host` is the
+// current f_src() mirror of a live Domain with `count` floats, else null.
+// Lets the momentum-exchange export reduce a grid's own populations (as the
+// reference's scenarios pass grid.f_src_data()) without uploading them again.
+const float* device_copy_of_host_populations(const float* host, size_t count);
 
 }  // namespace wake_hip
