@@ -396,7 +396,8 @@ def main():
     ap.add_argument("--storage", choices=["fp32", "fp16", "bf16"], default="fp32")
     ap.add_argument("--plane-pad", type=int, default=-1)
     ap.add_argument("--block-x", type=int, default=0)
-    ap.add_argument("--cpu-steps", type=int, default=4, help="oracle steps timed for the CPU baseline (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=30,
+                    help="oracle steps timed for the CPU baseline, about 10 s on 16 host threads (0: skip)")
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--domains", type=int, default=512)
     ap.add_argument("--plain-probe", type=int, default=20)
