@@ -123,3 +123,16 @@ def test_serve_results_jsonl(tmp_path):
     assert rows[0]["storage"] == "f32" and rows[1]["storage"] == "f16"
     assert rows[1]["time_to_result_s"] == 1.5 and rows[1]["result"] == {"Nu": 2.1}
     assert rows[0]["semi_axes"] == [9.0, 6.0, 5.0] and rows[1]["ra"] == 1e4
+
+
+def test_serve_policy_names():
+    """The serving loop's admission policies (CPU: nothing is admitted)."""
+    from wake_amd import serve
+
+    assert serve.Server(policy="arrival").policy == "arrival"
+    assert serve.Server().policy == "least_work"
+    with pytest.raises(ValueError):
+        serve.Server(policy="random")
+    r3 = serve.SimRequest(0, "bluff3d", 64, 100)
+    r2 = serve.SimRequest(1, "cavity2d", 64, 100)
+    assert serve.node_count(r3) == 64 ** 3 and serve.node_count(r2) == 64 ** 2

@@ -89,3 +89,19 @@ def test_serve_fp16_cavity_within_1e3_of_fp64(sv):
     assert eu <= 1e-3 and et <= 1e-3
     # the served request's Nusselt number agrees with the plain run's
     assert abs(s.done[0].result["Nu"] / sv.result_of(r, g)["Nu"] - 1) < 1e-4
+
+
+def test_serve_least_work_admission(sv):
+    """Shortest-job-first admission: with room for one request at a time the
+    requests finish in order of node-steps, and each result equals the one
+    the arrival-order policy gives (a request's result does not depend on
+    when or beside what it runs)."""
+    reqs = mix(sv)
+    s = sv.Server(budget_bytes=8e9, chunk=32, policy="least_work", fill_nodes=1)
+    s.serve(reqs, realtime=False)
+    work = {r.rid: sv.node_count(r) * r.steps for r in reqs}
+    finished = [r.rid for r in sorted(s.done, key=lambda r: r.finished)]
+    assert finished == sorted(work, key=work.get)
+    a = sv.Server(budget_bytes=8e9, chunk=32, policy="arrival")
+    a.serve(mix(sv), realtime=False)
+    assert {r.rid: r.result for r in s.done} == {r.rid: r.result for r in a.done}

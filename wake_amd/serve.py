@@ -7,10 +7,19 @@ LES, temperature coupling) share a Batch that advances all of them with one
 launch per step.  Every scheduling tick launches one chunk on every
 non-empty batch (each on its own stream, so the kinds overlap on the GPU),
 waits, retires the requests that reached their step count, frees their
-memory, and admits queued requests in arrival order while the HBM budget
-allows (first fit: a request that does not fit waits; later, smaller ones
-may still enter).  A request therefore joins the running mix at the next
-chunk boundary after memory frees up.
+memory, and admits queued requests while the HBM budget allows (first fit:
+a request that does not fit waits; others may still enter).  A request
+therefore joins the running mix at the next chunk boundary after memory
+frees up.
+
+Admission order (`policy`):
+  * "least_work" (default): the queued requests with the fewest node-steps
+    (nodes x steps, the time a request needs on a full GPU) first, and only
+    while the running mix holds fewer than `fill_nodes` nodes -- enough to
+    fill the GPU -- so short requests finish early instead of sharing the
+    GPU with every long one (shortest-job-first; the mean and median time to
+    result fall, the makespan does not grow);
+  * "arrival": arrival order, everything that fits in the budget at once.
 
 Results do not depend on the mix: a domain's arithmetic is the same alone
 and batched (tests/test_gpu_serve.py checks that).
@@ -141,14 +150,24 @@ def bytes_needed(req, storage=None):
     return _lib.lib().wake_domain_bytes_needed(2, 9, req.n, req.n, 1, st, -1, 1)
 
 
+def node_count(req):
+    return req.n ** 3 if req.kind == "bluff3d" else req.n ** 2
+
+
 class Server:
     """Continuous-batching request loop under an HBM admission budget."""
 
-    def __init__(self, budget_bytes=200e9, chunk=100, use_graph=True, tune=True):
+    def __init__(self, budget_bytes=200e9, chunk=100, use_graph=True, tune=True, policy="least_work",
+                 fill_nodes=4_000_000):
+        if policy not in ("least_work", "arrival"):
+            raise ValueError(f"unknown admission policy {policy!r}")
         self.budget = budget_bytes
         self.chunk = chunk
         self.use_graph = use_graph
         self.tune = tune
+        self.policy = policy
+        self.fill_nodes = fill_nodes
+        self.active_nodes = 0
         self.batches = {}
         self.active = {}        # rid -> (req, grid, key, slot)
         self.queue = deque()
@@ -163,10 +182,13 @@ class Server:
 
     def _admit(self, now):
         kept = deque()
-        while self.queue:
-            req = self.queue.popleft()
+        order = list(self.queue)
+        if self.policy == "least_work":
+            order.sort(key=lambda r: (node_count(r) * r.steps, r.arrival))
+        for req in order:
             need = bytes_needed(req)
-            if self.used + need > self.budget:
+            full = self.policy == "least_work" and self.active and self.active_nodes >= self.fill_nodes
+            if full or self.used + need > self.budget:
                 kept.append(req)
                 continue
             g = make_domain(req)
@@ -182,6 +204,7 @@ class Server:
             slot = b.add(g, req.steps, **({"smag": CS_LES} if req.kind == "bluff3d" else {}))
             self.active[req.rid] = (req, g, key, slot)
             self.used += need
+            self.active_nodes += node_count(req)
             self.peak_used = max(self.peak_used, self.used)
             req.admitted = now
         self.queue = kept
@@ -206,6 +229,7 @@ class Server:
                 req.finished = t
                 b.remove(slot)
                 self.used -= bytes_needed(req)
+                self.active_nodes -= node_count(req)
                 self.node_steps += g.total_nodes() * req.steps
                 del self.active[rid]
                 self.done.append(req)
