@@ -489,7 +489,8 @@ void oracle_d2q9_stream_with_boundaries(const real* f_src, real* f_dst, const ui
 }
 
 /* src/ispc/d2q9_boundaries.ispc:5-39 (uncalled by the reference grid).
- * parity unpinned (restatement only; no reference caller or test). */
+ * pinned: test_d2q9_apply_bounce_back_closed_form (mass kept,
+ * j -> -j + 2 u_w, an involution at u_w = 0). */
 void oracle_d2q9_apply_bounce_back(real* f, const uint8_t* cell_type, real uw_x, real uw_y, int count) {
   size_t n = (size_t)count;
   real mom[9];
@@ -1429,7 +1430,8 @@ void oracle_grid_set_solid_naca(oracle_grid* g, int digits, real chord, real aoa
         for (int z = za; z <= zb; ++z) oracle_grid_set_cell_type(g, x, y, z, CELL_SOLID);
 }
 
-/* lattice_grid_2d.cpp:143-159 */
+/* lattice_grid_2d.cpp:143-159.  pinned: test_oracle_shape_setters_published_counts
+ * (Gauss circle counts). */
 void oracle_grid_set_solid_circle(oracle_grid* g, real cx, real cy, real radius) {
   real r_sq = radius * radius;
   int min_x = imax(0, (int)(cx - radius - R(1))), max_x = imin(g->nx - 1, (int)(cx + radius + R(1)));
@@ -1441,7 +1443,8 @@ void oracle_grid_set_solid_circle(oracle_grid* g, real cx, real cy, real radius)
     }
 }
 
-/* lattice_grid_3d.cpp:184-203 */
+/* lattice_grid_3d.cpp:184-203.  pinned: test_oracle_shape_setters_published_counts
+ * (nz x the Gauss circle count). */
 void oracle_grid_set_solid_cylinder_z(oracle_grid* g, real cx, real cy, real radius) {
   g->is_periodic = 0;
   real r_sq = radius * radius;
@@ -1455,7 +1458,9 @@ void oracle_grid_set_solid_cylinder_z(oracle_grid* g, real cx, real cy, real rad
       }
 }
 
-/* lattice_grid_3d.cpp:205-215 and lattice_grid_2d.cpp:178-184 (north wall). */
+/* lattice_grid_3d.cpp:205-215 and lattice_grid_2d.cpp:178-184 (north wall).
+ * pinned: test_oracle_grid_setters_and_diagnostics_closed_forms (the cells
+ * marked), test_oracle_moving_wall_adds_momentum (the wall term). */
 void oracle_grid_set_moving_wall_top(oracle_grid* g, real uwx, real uwz) {
   g->is_periodic = 0;
   g->has_moving_wall_top = 1;
@@ -1465,23 +1470,30 @@ void oracle_grid_set_moving_wall_top(oracle_grid* g, real uwx, real uwz) {
     for (int x = 0; x < g->nx; ++x) oracle_grid_set_cell_type(g, x, g->ny - 1, z, CELL_MOVING_WALL);
 }
 
-/* lattice_grid_2d.cpp:161-168 (3D: EXTENSION, the whole x = 0 plane). */
+/* lattice_grid_2d.cpp:161-168 (3D: EXTENSION, the whole x = 0 plane).
+ * pinned: test_oracle_grid_setters_and_diagnostics_closed_forms,
+ * test_oracle_zou_he_2d_identities, test_oracle_zou_he_3d_identities. */
 void oracle_grid_set_inflow_west(oracle_grid* g, real ux, real uy) {
   g->has_inflow = 1; g->u_in_x = ux; g->u_in_y = uy; g->inflow_parabolic = 0;
   for (int z = 0; z < g->nz; ++z)
     for (int y = 0; y < g->ny; ++y) oracle_grid_set_cell_type(g, 0, y, z, CELL_INFLOW);
 }
 /* EXTENSION: parabolic profile u(y) = 4 u_max s (H - s) / H^2, s = y - 1/2,
- * H = ny - 2 (halfway walls at y = 1/2 and ny - 3/2). */
+ * H = ny - 2 (halfway walls at y = 1/2 and ny - 3/2).  pinned:
+ * test_oracle_grid_setters_and_diagnostics_closed_forms (symmetry, the row
+ * mean u (2/3 + 1/(3 H^2)) and the peak u (1 - 1/H^2)). */
 void oracle_grid_set_inflow_parabolic(oracle_grid* g, real u_max) {
   oracle_grid_set_inflow_west(g, u_max, R(0));
   g->inflow_parabolic = 1; g->u_max = u_max;
 }
-/* EXTENSION: u_x(node, step) = u_in (1 + a (2 U - 1)), U = rng(seed, step*N + node). */
+/* EXTENSION: u_x(node, step) = u_in (1 + a (2 U - 1)), U = rng(seed, step*N + node).
+ * pinned: test_oracle_rng_splitmix64_known_values (the generator),
+ * test_oracle_regularized_inlet. */
 void oracle_grid_set_inflow_turbulence(oracle_grid* g, real amplitude, uint64_t seed) {
   g->turb_amp = amplitude; g->turb_seed = seed;
 }
-/* lattice_grid_2d.cpp:170-176 */
+/* lattice_grid_2d.cpp:170-176.  pinned: test_oracle_grid_setters_and_diagnostics_closed_forms
+ * (the cells marked), test_oracle_zou_he_2d_identities (the closure). */
 void oracle_grid_set_outflow_east(oracle_grid* g, real rho_out) {
   g->has_outflow = 1; g->has_conv = 0; g->rho_out = rho_out;
   for (int z = 0; z < g->nz; ++z)
@@ -1691,7 +1703,12 @@ static void apply_open_boundaries(oracle_grid* g, uint64_t step) {
   }
 }
 
-/* lattice_grid_2d.cpp:195-265 and lattice_grid_3d.cpp:301-380. */
+/* lattice_grid_2d.cpp:195-265 and lattice_grid_3d.cpp:301-380.  pinned:
+ * test_oracle_poiseuille_reference_validation and
+ * test_oracle_taylor_green_3d_reference_validation (the reference's own
+ * validation tests), test_oracle_taylor_green_2d_periodic_decay,
+ * test_oracle_streaming_walls_conserve_mass_in_closed_box,
+ * test_oracle_fp32_fp64_agree_on_sphere_flow. */
 /* ------------------------------------------------------------------------
  * EXTENSION (no reference counterpart; the reference's ThermalSolver2D,
  * src/thermal/thermal_solver.cpp:83-123, is a D2Q9 passive scalar whose
@@ -1959,7 +1976,9 @@ const real* oracle_grid_uz(const oracle_grid* g) { return g->uz; }
 const real* oracle_grid_speed(const oracle_grid* g) { return g->speed; }
 const uint8_t* oracle_grid_cell_type(const oracle_grid* g) { return g->cell; }
 
-/* lattice_grid_3d.cpp:382-410 (Fluid nodes only), accumulated in double. */
+/* lattice_grid_3d.cpp:382-410 (Fluid nodes only), accumulated in double.
+ * pinned: test_oracle_grid_setters_and_diagnostics_closed_forms (uniform
+ * state with a solid block), test_oracle_taylor_green_3d_reference_validation. */
 double oracle_grid_total_mass(const oracle_grid* g) {
   double t = 0;
   for (size_t i = 0; i < g->N; ++i)
@@ -1992,7 +2011,8 @@ void oracle_grid_momentum_exchange(const oracle_grid* g, real cx, real cy, real 
 }
 
 /* analytics_kernels.ispc:4-29: omega_z = d uy/dx - d ux/dy, central
- * differences, periodic wrap. */
+ * differences, periodic wrap.  pinned:
+ * test_vorticity_and_q_criterion_closed_forms (this and the 3D kernel). */
 void oracle_compute_vorticity_2d(const real* ux, const real* uy, real* w, int nx, int ny) {
   for (int y = 0; y < ny; ++y) {
     int ym = (y - 1 + ny) % ny, yp = (y + 1) % ny;
@@ -2068,7 +2088,9 @@ static int sat_axis(real a, real b, real fa, real fb, real p0, real q0, real p1,
 }
 
 /* obj_loader.cpp:209-306 (test_triangle_box_overlap).  The nine edge axes
- * are called with the reference's arguments (:277-301). */
+ * are called with the reference's arguments (:277-301).  pinned:
+ * test_overlap_against_exact_clipping (never misses an overlap; the only
+ * extra overlaps are the ones those arguments predict). */
 int oracle_tri_box_overlap(const real* tri, const real* c, const real* h) {
   real v0x = tri[0] - c[0], v0y = tri[1] - c[1], v0z = tri[2] - c[2];
   real v1x = tri[3] - c[0], v1y = tri[4] - c[1], v1z = tri[5] - c[2];
@@ -2100,7 +2122,9 @@ int oracle_tri_box_overlap(const real* tri, const real* c, const real* h) {
  * test over each triangle's floor/ceil cell box, then a breadth-first walk
  * from the non-Solid cells of the six outer faces; unreached cells become
  * Solid.  Returns how many Solid writes were made (set_cell_type clears
- * the periodic mode on any of them). */
+ * the periodic mode on any of them).  pinned: tests/test_geometry_oracle.py
+ * (the reference's OBJ cube test and its closed form, closed vs open shells,
+ * sphere between its inscribed and circumscribed balls, torus hole). */
 long long oracle_voxelize_mesh(uint8_t* flags, int nx, int ny, int nz, const real* tris, long long ntri,
                                int fill_interior) {
   const real half[3] = {R(0.5), R(0.5), R(0.5)};
@@ -2170,6 +2194,10 @@ long long oracle_voxelize_mesh(uint8_t* flags, int nx, int ny, int nz, const rea
 }
 
 /* ------------------------------------------------------------------------
+ * pinned: tests/test_thermal_ibm_oracle.py (temperature conservation and the
+ * equilibrium fixed point of the collision, the in-place wall sequence in
+ * closed form, the buoyancy term, the interpolation's partition of unity and
+ * x periodicity, the spread total in closed form).
  * The reference's D2Q9 temperature solver kernels and the IBM kernels.
  * ---------------------------------------------------------------------- */
 

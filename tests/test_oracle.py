@@ -525,6 +525,73 @@ def test_oracle_sphere_flags_integer_count():
     assert np.array_equal(ct, expect)
 
 
+# Lattice points with x^2 + y^2 <= r^2 (Gauss's circle problem, OEIS A000328)
+# and with x^2 + y^2 + z^2 <= r^2 (OEIS A000605), r = 0 .. 8.
+GAUSS_CIRCLE = [1, 5, 13, 29, 49, 81, 113, 149, 197]
+BALL_POINTS = [1, 7, 33, 123, 257, 515, 925, 1419, 2109]
+
+
+def test_oracle_shape_setters_published_counts():
+    """lattice_grid_2d.cpp:143-159 (circle), lattice_grid_3d.cpp:184-203
+    (z cylinder) and :160-182 (sphere): with an integer radius on a lattice
+    point the Solid cells are the lattice points of the disc / ball, whose
+    counts are published (an independent check of the inclusive FP32
+    distance rule and of the centre offsets)."""
+    for r in range(1, 9):
+        g2 = OracleGrid(2, 32, 32, 1, tau=0.8)
+        g2.set_solid_circle(16.0, 15.0, float(r))
+        assert int((g2.cell_type() == 1).sum()) == GAUSS_CIRCLE[r], r
+        g3 = OracleGrid(3, 24, 24, 20, tau=0.8)
+        g3.set_solid_cylinder_z(12.0, 11.0, float(r))
+        assert int((g3.cell_type() == 1).sum()) == 20 * GAUSS_CIRCLE[r], r
+        gs = OracleGrid(3, 24, 24, 24, tau=0.8)
+        gs.set_solid_sphere(12.0, 11.0, 10.0, float(r))
+        assert int((gs.cell_type() == 1).sum()) == BALL_POINTS[r], r
+
+
+def test_oracle_grid_setters_and_diagnostics_closed_forms():
+    """Which cells the grid setters mark (lattice_grid_3d.cpp:205-215,
+    lattice_grid_2d.cpp:161-184), the Fluid-only diagnostics
+    (lattice_grid_3d.cpp:382-410) on a uniform state, and the parabolic inlet
+    extension against the closed form of its row mean."""
+    g = OracleGrid(3, 12, 10, 8, tau=0.8, bits=64)
+    g.initialize(1.1, 0.03, 0.04, 0.0)
+    g.set_solid_box(2, 2, 2, 4, 4, 4)  # 27 cells leave the sums
+    nf = 12 * 10 * 8 - 27
+    assert abs(g.total_mass() - 1.1 * nf) < 1e-9 * nf
+    assert abs(g.total_kinetic_energy() - 0.5 * 1.1 * 0.05 ** 2 * nf) < 1e-12 * nf
+    assert abs(g.max_speed() - 0.05) < 1e-12
+    g.set_moving_wall_top(0.1)  # the whole y = ny - 1 plane
+    ct = g.cell_type().reshape(8, 10, 12)
+    assert (ct[:, 9, :] == 6).all() and not (ct[:, :9, :] == 6).any()
+    g2 = OracleGrid(2, 9, 7, 1, tau=0.8)
+    g2.set_inflow_west(0.05, 0.0)  # every row of x = 0
+    g2.set_outflow_east(1.0)  # every row of x = nx - 1
+    c2 = g2.cell_type().reshape(7, 9)
+    assert (c2[:, 0] == 2).all() and (c2[:, 8] == 3).all() and (c2[:, 1:8] == 0).all()
+    g3 = OracleGrid(2, 9, 7, 1, tau=0.8)
+    g3.set_moving_wall_top(0.1)  # the north row
+    c3 = g3.cell_type().reshape(7, 9)
+    assert (c3[6] == 6).all() and not (c3[:6] == 6).any()
+    # parabolic inlet: u(s) = 4 u s (H - s) / H^2 at s = y - 1/2 on rows
+    # 1 .. ny-2 (H = ny - 2 rows between halfway walls) is symmetric, and its
+    # row mean is u (2/3 + 1 / (3 H^2)) -- an off-by-half row offset fails both
+    nx, ny, umax = 12, 18, 0.06
+    H = ny - 2
+    o = OracleGrid(2, nx, ny, 1, tau=0.8, bits=64)
+    o.initialize(1.0, 0.0, 0.0)
+    for x in range(nx):
+        o.set_cell_type(x, 0, 0, 1)
+        o.set_cell_type(x, ny - 1, 0, 1)
+    o.set_inflow_parabolic(umax)
+    o.step("BGK")
+    u_in = o.ux().reshape(ny, nx)[1:ny - 1, 0]
+    assert np.allclose(u_in, u_in[::-1], rtol=0, atol=1e-14)
+    assert abs(u_in.mean() - umax * (2 / 3 + 1 / (3 * H * H))) < 1e-12
+    # the two rows next to the centre, s = H/2 -+ 1/2: 4 u (H^2/4 - 1/4) / H^2
+    assert abs(u_in.max() - umax * (1 - 1 / (H * H))) < 1e-12
+
+
 def test_oracle_links_brute_force():
     nx = ny = nz = 20
     cx, cy, cz, R = 9.3, 10.1, 9.7, 4.6
