@@ -1355,7 +1355,9 @@ void oracle_grid_set_solid_sphere(oracle_grid* g, real cx, real cy, real cz, rea
 /* primitives.cpp:73-100 (axis-aligned: the reference's arithmetic); with
  * axes != NULL the rows of the row-major 3x3 `axes` are the body axes in
  * world coordinates, the point's body coordinates are b_k = axes[k] . d and
- * the box is bounded by the largest semi-axis (no reference counterpart). */
+ * the box is bounded by the largest semi-axis (no reference counterpart).
+ * pinned: test_ellipsoid_voxelizer_axes_conventions (identity axes change
+ * nothing, swapped axes equal swapped semi-axes, volume within 3%). */
 void oracle_grid_set_solid_ellipsoid(oracle_grid* g, real cx, real cy, real cz, real rx, real ry, real rz,
                                      const real* axes) {
   real ex = rx, ey = ry, ez = rz;
@@ -1414,7 +1416,10 @@ static int naca_contains(real camber, real camber_pos, real thickness, real chor
   return RABS(y_chord - yc) <= yt;
 }
 
-/* naca_airfoil.cpp:152-190: rasterize (2D) / voxelize_extruded over z0..z1. */
+/* naca_airfoil.cpp:152-190: rasterize (2D) / voxelize_extruded over z0..z1.
+ * pinned: test_naca_voxelizer_shape (symmetry at 0 deg, section area
+ * 0.685 t c^2, thickest section 0.12 c, the trailing edge below the leading
+ * edge at +8 deg, extrusion equal to the 2D section). */
 void oracle_grid_set_solid_naca(oracle_grid* g, int digits, real chord, real aoa_deg, real lead_x, real lead_y,
                                 int z0, int z1) {
   real camber = (real)(digits / 1000) / R(100), camber_pos = (real)((digits / 100) % 10) / R(10);
