@@ -515,7 +515,7 @@ def main():
                                                "k_step_d3q19_sphere256_fp16_pmc.md") +
                                 " (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, copy-calibrated)")),
             "bytes_per_node": bpn,
-            "kernel": "k_step (fused pull + MRT/LES collide), HIP events per launch",
+            "kernel": "k_step (fused pull + MRT/LES collide), HIP events around every 16th launch of the timed steps",
             "kernel_ms": round(kernel_ms, 5),
             "step_ms_events": round(ms / args.steps, 5),
             "practical_ceiling": {

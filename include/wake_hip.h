@@ -281,8 +281,9 @@ int wake_domain_run(wake_domain* d, int steps, int collision, float gx, float gy
                     float smag, float* ms);
 int wake_domain_synchronize(wake_domain* d);
 /* Average device duration (ms) of the fused step kernel per launch over the
- * last wake_domain_run with ms != NULL (HIP events around each launch on the
- * domain's stream; the open-boundary plane kernel is not included). */
+ * last wake_domain_run with ms != NULL: HIP events on the domain's stream
+ * around the 1st, 17th, 33rd, ... launch of that run (a pair around every
+ * launch would add about 8 us to each step). */
 int wake_domain_kernel_time(wake_domain* d, float* avg_ms);
 /* lattice_grid_3d.hpp:72-78 rho_data ... f_src_data, copied into out */
 int wake_domain_read(wake_domain* d, int field, float* out);

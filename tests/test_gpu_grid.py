@@ -594,3 +594,21 @@ def test_failed_allocation_releases_memory(wa):
     g.initialize(1.0, 0.02, 0.0, 0.0)
     g.step("BGK", 0.0, 0.0, 0.0, False, 0.14)
     assert np.isfinite(g.ux_data()).all()
+
+
+def test_kernel_time_sampled_in_timed_runs(wa):
+    """run(timed=True) returns the device time of the whole run and
+    kernel_time() the average step-kernel duration, sampled on every 16th
+    launch: positive, and close to a step's share of the run.  The grid is
+    large enough (1.2 M nodes, tens of microseconds a step) that the events
+    bracketing a sampled launch do not dominate its duration."""
+    g = wa.LatticeGrid3D(128, 96, 96, 0.6)
+    g.set_periodic_axes(True, True)
+    g.initialize(1.0, 0.03, 0.0, 0.0)
+    g.set_solid_sphere(40.0, 48.0, 48.0, 10.0)
+    g.run(3, "MRT")
+    for steps in (1, 40):
+        ms = g.run(steps, "MRT", timed=True)
+        k = g.kernel_time()
+        print(f"{steps} steps: {ms / steps * 1e3:.1f} us per step, sampled kernel {k * 1e3:.1f} us")
+        assert ms > 0 and 0.3 * ms / steps < k <= 1.5 * ms / steps, (steps, ms, k)

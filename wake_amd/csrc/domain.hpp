@@ -213,7 +213,10 @@ class Domain {
   bool time_kernels_ = false;
   bool buffer_addressing_ = true;
   size_t kev_used_ = 0;
+  uint64_t timed_launches_ = 0;            // pull launches in the current timed run
+  static constexpr int kTimingStride = 16;  // launches per timed sample
   std::vector<hipEvent_t> kev_;
+  hipEvent_t kev_marker_ = nullptr;  // recorded just before a sample's start event
   float last_kernel_ms_ = 0.0f;
 
   // reference grid configuration

@@ -167,7 +167,8 @@ class _Domain:
         return ms.value
 
     def kernel_time(self):
-        """Average device ms of the fused step kernel per launch in the last timed run()."""
+        """Average device ms of the fused step kernel per launch in the last
+        timed run() (HIP events around every 16th launch of that run)."""
         t = ctypes.c_float()
         self._c(self._L.wake_domain_kernel_time(self._h, ctypes.byref(t)), "kernel_time")
         return t.value
