@@ -156,17 +156,26 @@ def main_batched(args):
     nodes = sum(g.total_nodes() for g in grids)
     log(f"setup done in {time.perf_counter() - t0:.1f}s, {nodes / 1e6:.1f} M nodes")
 
-    # plain path: each domain alone, timed per domain, scaled to S steps
-    t_plain = 0.0
+    # plain path: each domain alone, timed per domain, scaled to S steps;
+    # once the steps alone (device time), once with the force the reference's
+    # scenario takes after every step (karman_vortex.cpp:104-123) through the
+    # single-domain call (wall time: the call returns the force to the host)
+    t_plain = t_plain_f = 0.0
     probe = args.plain_probe
-    for g in grids:
+    for g, (cx, cy, D, U, re) in zip(grids, geo):
         g.run(2, "MRT", use_les=True, smag=0.17)
         ms = g.run(probe, "MRT", use_les=True, smag=0.17, timed=True)
         t_plain += ms / probe * S * 1e-3
-    log(f"plain path: {t_plain:.2f} s projected for {S} steps of every domain ({probe}-step probe each)")
+        t0 = time.perf_counter()
+        for _ in range(probe):
+            g.step("MRT", 0.0, 0.0, True, 0.17)
+            g.momentum_exchange_2d(cx, cy, 1.5 * D)
+        t_plain_f += (time.perf_counter() - t0) / probe * S
+    log(f"plain path: {t_plain:.2f} s projected for {S} steps of every domain ({probe}-step probe each); "
+        f"with the per-step force: {t_plain_f:.2f} s")
 
     b = wake_amd.Batch(lattice=9, storage=storage, collision="MRT", use_les=True, chunk=100)
-    slots = [b.add(g, S - 2 - probe, smag=0.17, forces=(cx, cy, 0.0, 1.5 * D))
+    slots = [b.add(g, S - 2 - 2 * probe, smag=0.17, forces=(cx, cy, 0.0, 1.5 * D))
              for g, (cx, cy, D, U, re) in zip(grids, geo)]
     t0 = time.perf_counter()
     shape = b.tune(probe_steps=4)
@@ -199,7 +208,7 @@ def main_batched(args):
         "unit": "sims/s",
         "n_gpus": 1,
         "steps": S,
-        "warmup": 2 + probe,
+        "warmup": 2 + 2 * probe,
         "ms_per_step": round(wall * 1e3 / left, 4),
         "higher_is_better": True,
         "scaling": "weak",
@@ -213,6 +222,8 @@ def main_batched(args):
             "nodes": nodes, "launch_shape": {"block": shape[0], "grid_factor": shape[1]},
             "batched_mlups": round(mlups, 1), "plain_sims_per_s": round(ndom / t_plain, 4),
             "speedup_vs_plain": round(t_plain / t_batch, 3),
+            "plain_with_forces_sims_per_s": round(ndom / t_plain_f, 4),
+            "speedup_vs_plain_with_forces": round(t_plain_f / t_batch, 3),
             "C_D_median": round(float(np.median(cds)), 4) if cds else None,
             "St_median": round(float(np.median(sts)), 4) if sts else None,
             "non_finite_domains": bad, "parallelism": "single GPU",
