@@ -136,3 +136,16 @@ def test_serve_policy_names():
     r3 = serve.SimRequest(0, "bluff3d", 64, 100)
     r2 = serve.SimRequest(1, "cavity2d", 64, 100)
     assert serve.node_count(r3) == 64 ** 3 and serve.node_count(r2) == 64 ** 2
+
+
+def test_serve_rejects_requests_larger_than_the_budget():
+    """A request that could never fit the HBM budget is rejected at submit
+    (CPU: sizes only, nothing is allocated), so the loop cannot wait on it
+    forever."""
+    from wake_amd import serve
+
+    s = serve.Server(budget_bytes=1e6)
+    big = serve.SimRequest(0, "bluff3d", 128, 100)
+    s.submit(big)
+    assert not s.queue and s.rejected == [big] and "budget" in big.result["error"]
+    assert s.serve([], realtime=False) >= 0.0

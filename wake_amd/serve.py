@@ -173,11 +173,19 @@ class Server:
         self.queue = deque()
         self.used = 0
         self.done = []
+        self.rejected = []      # requests that need more HBM than the whole budget
         self.peak_used = 0
         self.ticks = 0
         self.node_steps = 0
 
     def submit(self, req):
+        """Queue a request; one that could never fit the budget is rejected
+        (kept in `rejected` with the reason) instead of waiting forever."""
+        need = bytes_needed(req)
+        if need > self.budget:
+            req.result = {"error": f"needs {need / 1e9:.2f} GB, more than the {self.budget / 1e9:.2f} GB budget"}
+            self.rejected.append(req)
+            return
         self.queue.append(req)
 
     def _admit(self, now):
