@@ -10,8 +10,11 @@
 
 namespace wake_hip {
 
+// Same per-instantiation occupancy floor as the single-domain step kernel
+// (StepWaves: the D3Q27 kernels with 16-bit storage).
 template <class L, class St, int COLL, bool LES, bool THERMAL>
-__global__ __launch_bounds__(256) void k_step_batch(const BatchSlot* __restrict__ slots,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(StepWaves<L, St>::value))) void k_step_batch(
+    const BatchSlot* __restrict__ slots,
                                                     const int* __restrict__ tile_slot, int ntiles, int i) {
   using T = typename St::T;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
