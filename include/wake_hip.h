@@ -393,6 +393,9 @@ int wake_batch_forces(wake_batch* b, int slot, double* out, long long capacity, 
  * grid_factor x compute units) and keeps the fastest.  The steps count as
  * steps of the members. */
 int wake_batch_tune(wake_batch* b, int probe_steps, int* block, int* grid_factor);
+/* Fix the launch shape.  Until this or wake_batch_tune is called, the
+ * work-group width is the one of 256 / 128 / 64 that leaves the fewest idle
+ * lanes over the members' rows, with one work-group per tile. */
 int wake_batch_set_launch(wake_batch* b, int block, int grid_factor);
 /* out = {members, total nodes, block, grid_factor} */
 int wake_batch_info(wake_batch* b, long long out[4]);
