@@ -151,7 +151,7 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(
-            f"{LIB_PATH} is not built; run `python -m wake_amd.build` (HIP gfx950 build of the LBM kernels)")
+            f"{LIB_PATH} is not built; run `python wake_amd/build.py` (HIP gfx950 build of the LBM kernels)")
     L = ctypes.CDLL(LIB_PATH)
     for name, sig in KERNEL_SIGS.items():
         fn = getattr(L, name)

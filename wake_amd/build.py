@@ -4,7 +4,10 @@ Each source compiles to its own object under wake_amd/_build/ (in parallel;
 an object is rebuilt when its source or any header in csrc/ or include/ is
 newer), then everything links into wake_amd/libwake_hip.so.
 
-    python -m wake_amd.build [--force] [-j N]
+    python wake_amd/build.py [--force] [-j N]
+
+Run it as a script: `python -m wake_amd.build` would import the package,
+which refuses to load until the library exists.
 """
 import argparse
 import concurrent.futures as cf
