@@ -555,7 +555,10 @@ void oracle_d2q9_apply_zou_he_outflow_east(real* f, real* rho, real* ux, real* u
 }
 
 /* src/ispc/d2q9_boundaries.ispc:125-146 (uncalled by the reference grid).
- * pinned: test_oracle_zou_he (uniform fields are a fixed point). */
+ * pinned: test_convective_outlet_is_implicit_upwind_advection (zero residual
+ * of the implicit upwind advection equation on random states, the alpha and
+ * beta limits, the x = nx - 2 upstream column on a ramp, untouched corner
+ * rows), test_oracle_convective_fixed_point. */
 void oracle_d2q9_apply_convective_outflow_east(real* f, const real* f_prev, real u_conv, int nx, int ny) {
   size_t total = (size_t)nx * ny;
   real alpha = u_conv / (R(1) + u_conv);
@@ -566,8 +569,10 @@ void oracle_d2q9_apply_convective_outflow_east(real* f, const real* f_prev, real
   }
 }
 
-/* src/ispc/d2q9_les.ispc:6-57.  pinned: test_oracle_les (equilibrium gives
- * tau_0 exactly; scalar formula on a brute-force Pi_neq). */
+/* src/ispc/d2q9_les.ispc:6-57.  pinned:
+ * test_d2q9_smagorinsky_tau_equilibrium_and_brute_force (equilibrium gives
+ * tau_0 exactly; the closed form on a brute-force einsum Pi_neq; monotone in
+ * the stress). */
 void oracle_d2q9_compute_smagorinsky_tau(const real* f, const real* rho, const real* ux, const real* uy,
                                          real* tau_eff, real tau_0, real smagorinsky_constant, int count) {
   size_t n = (size_t)count;
@@ -1065,8 +1070,13 @@ void oracle_smagorinsky_tau_from_f(int lattice, const real* f, real* tau_eff, re
 }
 
 /* src/ispc/d3q19_curved.ispc:7-71; per_y/per_z wrap x_ff (EXTENSION) the way
- * the streaming does.  pinned: test_oracle_bfl (delta = 1/2 reduces both
- * branches to halfway bounce-back). */
+ * the streaming does.  pinned: test_bfl_reproduces_uniform_equilibrium_of_the
+ * _wall_velocity (Galilean consistency of both wall terms, delta 0.01..0.99),
+ * test_bfl_interpolates_a_linear_field_at_the_departure_point (interpolation
+ * weights and the upstream neighbour in both branches),
+ * test_bfl_limits_and_upstream_neighbour (delta -> 0 and 1, the y-edge
+ * fallback, the x wrap), test_oracle_bfl_half_delta_is_halfway_bounce_back;
+ * tests/test_oracle_mutants.py shows planted mistakes fail them. */
 static void bfl_general(const lat_t* L, real* f_dst, const real* f_src, const int* fluid_indices,
                         const int* q_dirs, const real* delta_fractions, const real* uw_x, const real* uw_y,
                         const real* uw_z, int nx, int ny, int nz, int per_y, int per_z, int nlinks) {
@@ -1113,7 +1123,10 @@ void oracle_d3q19_apply_curved_bfl(real* f_dst, const real* f_src, const int* fl
 
 /* src/ispc/forces_momentum_exchange.ispc:71-138, double accumulation (see the
  * 2D note).  per_y/per_z (EXTENSION) wrap the neighbour instead of skipping.
- * pinned: test_oracle_mem (rest gives zero on a symmetric body; impulse). */
+ * pinned: test_mem3d_single_population_impulses (2 delta c_q for five
+ * directions at linked nodes, nothing for the opposite population, for a link
+ * outside r_max or across the non-periodic z edge),
+ * test_oracle_mem_3d_symmetric_body_at_rest. */
 static void mem3d(const lat_t* L, const real* f, const uint8_t* cell_type, int nx, int ny, int nz, int per_y,
                   int per_z, real cxx, real cyy, real czz, real max_radius, double* out) {
   size_t total = (size_t)nx * ny * nz;
