@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <functional>
 #include <memory>
+#include <vector>
 
 #include "wake/core/constants.hpp"
 #include "wake/core/types.hpp"
@@ -76,6 +77,11 @@ class LatticeGrid2D {
   void add_noise(real_t amplitude, uint64_t seed);
   // (Fx, Fy, torque_z) on the post-stream state (karman_vortex.cpp:107-114).
   Vec3 momentum_exchange(real_t cx, real_t cy, real_t r_max) const;
+  // From the current step on, keep momentum_exchange(cx, cy, r_max) of every
+  // step, computed inside the step launch itself (no per-step call).
+  void enable_force_history(real_t cx, real_t cy, real_t r_max);
+  // Forces of steps *first_step .. current_step() (Fx, Fy, torque_z).
+  std::vector<Vec3> force_history(uint64_t* first_step = nullptr) const;
   float run(int steps, CollisionType collision = CollisionType::MRT, real_t gx = 0.0f, real_t gy = 0.0f,
             bool use_les = false, real_t smag_const = DEFAULT_SMAGORINSKY_CONSTANT);
   void check_finite() const;

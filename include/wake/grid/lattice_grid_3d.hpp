@@ -15,6 +15,7 @@
 #include <cstdint>
 #include <functional>
 #include <memory>
+#include <vector>
 
 #include "wake/core/constants.hpp"
 #include "wake/core/types.hpp"
@@ -98,6 +99,11 @@ class LatticeGrid3D {
   void add_noise(real_t amplitude, uint64_t seed);
   // (Fx, Fy, Fz) on the post-stream state, links within r_max of the centre.
   Vec3 momentum_exchange(real_t cx, real_t cy, real_t cz, real_t r_max) const;
+  // From the current step on, keep momentum_exchange(cx, cy, cz, r_max) of
+  // every step, computed inside the step launch itself (no per-step call).
+  void enable_force_history(real_t cx, real_t cy, real_t cz, real_t r_max);
+  // Forces of steps *first_step .. current_step() (3D: Fx, Fy, Fz).
+  std::vector<Vec3> force_history(uint64_t* first_step = nullptr) const;
   // `steps` steps; returns the device milliseconds (hip events) of the launches.
   float run(int steps, CollisionType collision = CollisionType::MRT, real_t gx = 0.0f, real_t gy = 0.0f,
             real_t gz = 0.0f, bool use_les = false, real_t smag_const = DEFAULT_SMAGORINSKY_CONSTANT);

@@ -297,6 +297,20 @@ int wake_domain_diagnostics(wake_domain* d, double out[3]);
  * scenarios make, sphere_3d_wake.cpp:98-105): out = (Fx, Fy, Fz) in 3D and
  * (Fx, Fy, torque_z) in 2D. */
 int wake_domain_momentum_exchange(wake_domain* d, float cx, float cy, float cz, float r_max, double out[3]);
+/* Per-step momentum exchange (the reference's scenarios take it after every
+ * step, sphere_3d_wake.cpp:94-105, karman_vortex.cpp:104-114), computed
+ * inside the launch that forms each post-stream state: from the current step
+ * on, every step's forces (as wake_domain_momentum_exchange would return them
+ * right after that step) are kept on the device.  Setting a probe again
+ * restarts the history; so does initialising the state.  r_max <= 0 takes
+ * every link.  Not for temperature-coupled domains. */
+int wake_domain_set_force_probe(wake_domain* d, float cx, float cy, float cz, float r_max);
+int wake_domain_clear_force_probe(wake_domain* d);
+/* The history so far: *first_step = step of row 0, *count = rows (steps
+ * first_step .. current step); min(*count, max_rows) rows of 3 doubles are
+ * copied to out (out may be NULL with max_rows 0 to ask for the count). */
+int wake_domain_force_history(wake_domain* d, double* out, long long max_rows, long long* first_step,
+                              long long* count);
 /* out: nx, ny, nz, Q, plane stride, current step, storage, device bytes */
 int wake_domain_info(wake_domain* d, long long out[8]);
 /* src/export/vtk_exporter.cpp:8-191: the current fields as a VTK XML

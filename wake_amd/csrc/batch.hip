@@ -68,18 +68,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(StepWaves<L
   }
 }
 
-// Momentum exchange on S_{cur+1} = gather of the buffer step i just wrote.
+// Momentum exchange on S_{cur+1} = gather of the buffer step i just wrote,
+// one work-group per member: the lanes walk the member's linked rows and form
+// S only at the Fluid nodes next to a wall; lane sums in a fixed order, then
+// a fixed tree over the work-group.  A member with forces but no link
+// records zeros.
 template <class L, class St>
 __global__ __launch_bounds__(256) void k_mem_batch(const BatchSlot* __restrict__ slots, int i, int chunk,
                                                    double* __restrict__ hist) {
   const BatchSlot& sl = slots[blockIdx.x];
   const unsigned long long cur = sl.step + static_cast<unsigned long long>(i);
-  if (cur >= sl.stop_at || sl.mem_count == 0) return;
+  if (!sl.mem_on || cur >= sl.stop_at) return;
+  const StepArgs& a = sl.a;
   const int par = ((sl.parity + i) & 1) ^ 1;
   const auto* buf = static_cast<const typename St::T*>(sl.buf[par]);
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  mem_links<L, St, false>(sl.a, buf, cur + 1, sl.mem_node, sl.mem_q, sl.mem_count, threadIdx.x, blockDim.x,
-                          sl.mem_cx, sl.mem_cy, s0, s1, s2);
+  for (int r = 0; r < sl.mem_nrows; ++r) {
+    const int row = sl.mem_list[r];
+    const int y = row % a.ny, z = row / a.ny;
+    for (int x = threadIdx.x; x < a.nx; x += blockDim.x) {
+      const int n = row * a.nx + x;
+      const uint8_t c = a.cls[n];
+      if (!(c & kClsSlow) || a.flags[n] != kFluid) continue;
+      float f[L::Q];
+      form_s_at<L, St, false>(a, buf, x, y, z, n, c, false, cur + 1, f);
+      mem_node<L>(a, x, y, z, f, a.mem_c[0], a.mem_c[1], a.mem_c[2], a.mem_rmax, s0, s1, s2);
+    }
+  }
   block_sum3(s0, s1, s2);
   if (threadIdx.x == 0) {
     double* o = hist + (static_cast<long long>(blockIdx.x) * chunk + i) * 3;

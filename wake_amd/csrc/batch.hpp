@@ -31,10 +31,10 @@ struct BatchSlot {
   int nodes;                  // nx * ny * nz
   int tile0, ntiles;          // tiles [tile0, tile0 + ntiles) belong to this slot
   int tiles_per_row;          // ceil(nx / block)
-  const int* mem_node;        // momentum-exchange links (node, q), or null
-  const int* mem_q;
-  int mem_count;
-  float mem_cx, mem_cy;       // 2D torque centre
+  // momentum exchange (a.mem_c, a.mem_rmax): the member's rows holding links
+  const int* mem_list;
+  int mem_nrows;
+  int mem_on;                 // forces recorded for this member
 };
 
 // One launch = step i of the chunk for every slot.  grid_x work-groups of

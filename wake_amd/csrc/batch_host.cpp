@@ -348,9 +348,12 @@ class Batch {
       b.ntiles = m.d->ny() * m.d->nz() * b.tiles_per_row;
       tile_slot.insert(tile_slot.end(), b.ntiles, s);
       if (m.forces) {
-        b.mem_count = m.d->batch_mem_links(m.c[0], m.c[1], m.c[2], m.r_max, &b.mem_node, &b.mem_q);
-        b.mem_cx = m.c[0];
-        b.mem_cy = m.c[1];
+        b.mem_list = m.d->batch_mem_rows(m.c[0], m.c[1], m.c[2], m.r_max, &b.mem_nrows);
+        b.mem_on = 1;
+        b.a.mem_c[0] = m.c[0];
+        b.a.mem_c[1] = m.c[1];
+        b.a.mem_c[2] = m.c[2];
+        b.a.mem_rmax = m.r_max;
         any_forces_ = true;
       }
       m.step = b.step;

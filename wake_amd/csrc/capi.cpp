@@ -243,6 +243,25 @@ int wake_domain_diagnostics(wake_domain* d, double out[3]) {
 int wake_domain_momentum_exchange(wake_domain* d, float cx, float cy, float cz, float r_max, double out[3]) {
   return guard(d, [&](wake_hip::Domain& g) { g.momentum_exchange(cx, cy, cz, r_max, out); });
 }
+int wake_domain_set_force_probe(wake_domain* d, float cx, float cy, float cz, float r_max) {
+  return guard(d, [&](wake_hip::Domain& g) { g.set_force_probe(cx, cy, cz, r_max); });
+}
+int wake_domain_clear_force_probe(wake_domain* d) {
+  return guard(d, [&](wake_hip::Domain& g) { g.clear_force_probe(); });
+}
+int wake_domain_force_history(wake_domain* d, double* out, long long max_rows, long long* first_step,
+                              long long* count) {
+  return guard(d, [&](wake_hip::Domain& g) {
+    if (max_rows > 0 && !out) throw wake_hip::HipError("force_history: null output");
+    uint64_t first = 0;
+    const std::vector<double>& h = g.force_history(&first);
+    const long long rows = static_cast<long long>(h.size() / 3);
+    if (first_step) *first_step = static_cast<long long>(first);
+    if (count) *count = rows;
+    const long long n = rows < max_rows ? rows : max_rows;
+    if (n > 0) std::memcpy(out, h.data(), static_cast<size_t>(n) * 3 * sizeof(double));
+  });
+}
 int wake_domain_info(wake_domain* d, long long out[8]) {
   return guard(d, [&](wake_hip::Domain& g) {
     out[0] = g.nx();

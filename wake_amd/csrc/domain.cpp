@@ -233,6 +233,7 @@ void Domain::initialize(float rho, float ux, float uy, float uz) {
   fsrc_valid_ = fdst_valid_ = false;
   sout_is_fsrc_ = false;
   check(hipStreamSynchronize(stream_), "initialize");
+  if (probe_) probe_restart();
 }
 
 void Domain::upload_state_from_host(const std::vector<float>& f_ref) {
@@ -261,6 +262,7 @@ void Domain::upload_state_from_host(const std::vector<float>& f_ref) {
   fsrc_valid_ = (storage_ == kStoreF32);
   sout_is_fsrc_ = false;
   fdst_valid_ = false;
+  if (probe_) probe_restart();  // the state the samples were taken of is gone
 }
 
 // lattice_grid_3d.cpp:95-131 with the per-node values as arrays; cell types
@@ -314,6 +316,7 @@ void Domain::add_noise(float amplitude, uint64_t seed) {
   const bool fv = fields_valid_, dfv = dfields_valid_;
   upload_state_from_host(f);
   step_count_ = keep;
+  if (probe_) probe_restart();
   fields_valid_ = fv;
   dfields_valid_ = dfv;
 }
@@ -713,11 +716,18 @@ void Domain::sync_device_config() {
     check(launch_build_class(lattice_, a, d_cls_, stream_), "build class");
     check(launch_mark_bfl(d_bfl_nodes_, n_bfl_nodes_, d_cls_, d_aux_, stream_), "mark bfl");
     cls_dirty_ = false;
+    ++cls_epoch_;
   }
 }
 
 void Domain::materialize() {
   sync_device_config();
+  if (probe_) {
+    // the force sample of S_k is taken from the unrounded S, as a pull
+    // launch would take it, before the state is stored
+    probe_sync();
+    probe_sample_current();
+  }
   const StepArgs a = args(true);
   const int out = 1 - a_buf_;
   check(launch_materialize(lattice_, storage_, a, buf_[a_buf_], buf_[out], stream_), "materialize");
@@ -729,6 +739,7 @@ void Domain::materialize() {
 void Domain::step(int collision, float gx, float gy, float gz, bool use_les, float smag) {
   ++version_;
   sync_device_config();
+  if (probe_) probe_sync();
   if (dim_ == 2) gz = 0.0f;
   StepArgs a = args(true);
   a.tau = tau_;
@@ -747,6 +758,9 @@ void Domain::step(int collision, float gx, float gy, float gz, bool use_les, flo
     a_buf_ = out;
     s_buf_ = -1;
   } else if (s_buf_ >= 0) {
+    // S_k is materialised: the collide-only launch forms no S, so the force
+    // sample of S_k is taken from the materialised state
+    if (probe_) probe_sample_current();
     const int out = 1 - s_buf_;
     check(launch_step(lattice_, storage_, collision, les, false, a, buf_[s_buf_], buf_[out], block_x_, stream_),
           "step");
@@ -754,6 +768,12 @@ void Domain::step(int collision, float gx, float gy, float gz, bool use_les, flo
     s_buf_ = -1;
   } else {
     const int out = 1 - a_buf_;
+    // the pull forms S_k: its force sample is taken inside the launch
+    if (probe_ && fh_next_ == step_count_) {
+      a = with_mem(a, probe_rows_, probe_slot(step_count_), probe_c_, probe_rmax_);
+      if (probe_rows_.nrows == 0) a.mem_rows = nullptr;
+      fh_next_ = step_count_ + 1;
+    }
     // A timed run brackets every kTimingStride-th pull launch with events:
     // a pair around every launch costs about 8 us (1.7% of a 256^3 FP32
     // step), which the sampled average no longer pays.
@@ -989,67 +1009,213 @@ double Domain::max_speed() {
   return d[2];
 }
 
-// forces_momentum_exchange.ispc:6-138: links from Fluid nodes to in-range
-// Solid or MovingWall neighbours whose midpoint lies within r_max.
-// Fluid nodes x and directions q (1..Q-1) whose in-range neighbour x + c_q
-// is Solid or MovingWall, with the link midpoint within r_max of the centre
-// (forces_momentum_exchange.ispc:71-138); q-major order as the reference's
-// loops.  Cached per geometry version, centre and radius.
-const std::pair<std::vector<int>, std::vector<int>>& Domain::mem_link_list(float ccx, float ccy, float ccz,
-                                                                            float r_max) {
-  const auto key = std::make_tuple(ccx, ccy, ccz, r_max, flags_version_ * 4 + (ext_axes_ ? (per_y_ ? 2 : 0) + (per_z_ ? 1 : 0) : 0));
-  auto it = mem_links_.find(key);
-  if (it == mem_links_.end()) {
-    const LatTab L = tab(lattice_);
-    const bool py = ext_axes_ && per_y_, pz = ext_axes_ && per_z_;
-    const float r_max_sq = r_max * r_max;
-    std::vector<int> nodes, qs;
-    for (int q = 1; q < L.Q; ++q)
-      for (int z = 0; z < nz_; ++z) {
-        int nzz = z + L.cz[q];
-        if (pz) nzz = (nzz + nz_) % nz_;
-        if (nzz < 0 || nzz >= nz_) continue;
-        for (int y = 0; y < ny_; ++y) {
-          int nyy = y + L.cy[q];
-          if (py) nyy = (nyy + ny_) % ny_;
-          if (nyy < 0 || nyy >= ny_) continue;
-          for (int x = 0; x < nx_; ++x) {
-            const size_t idx = (static_cast<size_t>(z) * ny_ + y) * nx_ + x;
-            if (flags_[idx] != kFluid) continue;
-            const uint8_t nt = flags_[(static_cast<size_t>(nzz) * ny_ + nyy) * nx_ + (x + L.cx[q] + nx_) % nx_];
-            if (nt != kSolid && nt != kMovingWall) continue;
-            const float rx = (static_cast<float>(x) + 0.5f * static_cast<float>(L.cx[q])) - ccx;
-            const float ry = (static_cast<float>(y) + 0.5f * static_cast<float>(L.cy[q])) - ccy;
-            const float rz = (static_cast<float>(z) + 0.5f * static_cast<float>(L.cz[q])) - ccz;
-            const float d2 = dim_ == 2 ? rx * rx + ry * ry : rx * rx + ry * ry + rz * rz;
-            if (r_max <= 0.0f || d2 <= r_max_sq) {
-              nodes.push_back(static_cast<int>(idx));
-              qs.push_back(q);
-            }
-          }
-        }
-      }
-    if (mem_links_.size() > 8) mem_links_.clear();
-    it = mem_links_.emplace(key, std::make_pair(std::move(nodes), std::move(qs))).first;
+// ------------------------------------------------------ momentum exchange
+// Linked rows of (centre, r_max) from the device cell types: the rows are
+// marked on the device, numbered here in (z, y) order, each given
+// mem_waves_per_row(nx) partial slots.
+Domain::MemRows Domain::build_mem_rows(float ccx, float ccy, float ccz, float r_max) {
+  const size_t rows = static_cast<size_t>(ny_) * nz_;
+  MemRows r;
+  r.d_rows = static_cast<int*>(alloc(rows * sizeof(int)));
+  StepArgs a = args(false);
+  a.mem_c[0] = ccx;
+  a.mem_c[1] = ccy;
+  a.mem_c[2] = ccz;
+  a.mem_rmax = r_max;
+  check(hipMemsetAsync(r.d_rows, 0, rows * sizeof(int), stream_), "mark rows");
+  check(launch_mem_mark_rows(lattice_, a, r.d_rows, stream_), "mark rows");
+  std::vector<int> has(rows);
+  check(hipMemcpyAsync(has.data(), r.d_rows, rows * sizeof(int), hipMemcpyDeviceToHost, stream_), "mark rows");
+  check(hipStreamSynchronize(stream_), "mark rows");
+  const int W = mem_waves_per_row(nx_);
+  std::vector<int> list;
+  for (size_t i = 0; i < rows; ++i) {
+    if (has[i]) {
+      has[i] = static_cast<int>(list.size()) * W;
+      list.push_back(static_cast<int>(i));
+    } else {
+      has[i] = -1;
+    }
   }
-  return it->second;
+  r.nrows = static_cast<int>(list.size());
+  r.nslots = r.nrows * W;
+  r.d_list = static_cast<int*>(alloc(std::max<size_t>(1, list.size()) * sizeof(int)));
+  check(hipMemcpyAsync(r.d_rows, has.data(), rows * sizeof(int), hipMemcpyHostToDevice, stream_), "rows");
+  if (!list.empty())
+    check(hipMemcpyAsync(r.d_list, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice, stream_), "rows");
+  check(hipStreamSynchronize(stream_), "rows");  // the host vectors go out of scope
+  return r;
 }
 
+void Domain::free_mem_rows(MemRows& r) {
+  if (r.d_rows || r.d_list) check(hipStreamSynchronize(stream_), "free rows");
+  release(r.d_rows);
+  release(r.d_list);
+  r = MemRows{};
+}
+
+const Domain::MemRows& Domain::cached_mem_rows(float ccx, float ccy, float ccz, float r_max) {
+  const auto key = std::make_tuple(ccx, ccy, ccz, r_max, cls_epoch_);
+  auto it = mem_rows_cache_.find(key);
+  if (it != mem_rows_cache_.end()) return it->second;
+  if (mem_rows_cache_.size() >= 4) {
+    for (auto& kv : mem_rows_cache_) free_mem_rows(kv.second);
+    mem_rows_cache_.clear();
+  }
+  return mem_rows_cache_.emplace(key, build_mem_rows(ccx, ccy, ccz, r_max)).first->second;
+}
+
+StepArgs Domain::with_mem(StepArgs a, const MemRows& r, double* part, const float c[3], float r_max) const {
+  a.mem_rows = r.d_rows;
+  a.mem_part = part;
+  a.mem_c[0] = c[0];
+  a.mem_c[1] = c[1];
+  a.mem_c[2] = c[2];
+  a.mem_rmax = r_max;
+  return a;
+}
+
+// Forces on the current S_k: the partials of the linked rows (S_k formed at
+// the nodes next to a wall, or read from a materialised state), summed in
+// slot order.  The same kernels and order as the per-step history.
 void Domain::momentum_exchange(float ccx, float ccy, float ccz, float r_max, double out[3]) {
   sync_device_config();
-  const StepArgs a = args(false);
+  const MemRows& r = cached_mem_rows(ccx, ccy, ccz, r_max);
+  if (r.nrows == 0) {
+    out[0] = out[1] = out[2] = 0.0;
+    return;
+  }
+  const size_t need = 3 * static_cast<size_t>(r.nslots);
+  if (need > mempart_cap_) {
+    if (d_mempart_) {
+      check(hipStreamSynchronize(stream_), "momentum exchange");
+      release(d_mempart_);
+    }
+    d_mempart_ = static_cast<double*>(alloc(need * sizeof(double)));
+    mempart_cap_ = need;
+  }
+  const float c[3] = {ccx, ccy, ccz};
+  const StepArgs a = with_mem(args(false), r, d_mempart_, c, r_max);
   const bool from_s = s_buf_ >= 0;
-  const int nblocks = static_cast<int>(std::min<size_t>(1024, (N_ + 255) / 256));
-  check(launch_mem_scan(lattice_, storage_, from_s, a, buf_[from_s ? s_buf_ : a_buf_], ccx, ccy, ccz, r_max,
-                        d_partial_, nblocks, d_out_, stream_),
+  check(launch_mem_rows(lattice_, storage_, from_s, a, buf_[from_s ? s_buf_ : a_buf_], r.d_list, r.nrows, stream_),
         "momentum exchange");
+  check(launch_mem_fold(d_mempart_, r.nslots, 1, 0, 1, d_out_, 0, stream_), "momentum exchange");
   check(hipMemcpyAsync(out, d_out_, 3 * sizeof(double), hipMemcpyDeviceToHost, stream_), "d2h");
   check(hipStreamSynchronize(stream_), "momentum exchange");
+}
+
+// ------------------------------------------------------------- force probe
+void Domain::set_force_probe(float ccx, float ccy, float ccz, float r_max) {
+  if (thermal_) throw HipError("the force history is not available on temperature-coupled domains");
+  probe_ = true;
+  probe_c_[0] = ccx;
+  probe_c_[1] = ccy;
+  probe_c_[2] = ccz;
+  probe_rmax_ = r_max;
+  if (probe_rows_.d_rows) free_mem_rows(probe_rows_);
+  probe_epoch_ = ~0ull;
+  if (!d_fhist_) d_fhist_ = static_cast<double*>(alloc(3 * static_cast<size_t>(kForceHistCap) * sizeof(double)));
+  probe_restart();
+}
+
+void Domain::clear_force_probe() {
+  probe_ = false;
+  if (probe_rows_.d_rows) free_mem_rows(probe_rows_);
+  probe_epoch_ = ~0ull;
+  h_fhist_.clear();
+}
+
+void Domain::probe_restart() {
+  fh_first_ = fh_next_ = fh_folded_ = fh_dbase_ = fh_host_ = step_count_;
+  h_fhist_.clear();
+}
+
+// Rows for the current class bytes; samples written with the previous rows
+// are folded first (their slot layout goes with those rows).
+void Domain::probe_sync() {
+  if (probe_epoch_ == cls_epoch_) return;
+  probe_fold(fh_next_);
+  if (probe_rows_.d_rows) free_mem_rows(probe_rows_);
+  probe_rows_ = build_mem_rows(probe_c_[0], probe_c_[1], probe_c_[2], probe_rmax_);
+  probe_epoch_ = cls_epoch_;
+  // one slot at least: a geometry without links has all-zero samples
+  const size_t need = 3 * static_cast<size_t>(kForceRing) * std::max(1, probe_rows_.nslots);
+  if (need > ring_cap_) {
+    if (d_ring_) {
+      check(hipStreamSynchronize(stream_), "force ring");
+      release(d_ring_);
+    }
+    d_ring_ = static_cast<double*>(alloc(need * sizeof(double)));
+    ring_cap_ = need;
+  }
+  if (probe_rows_.nrows == 0) check(hipMemsetAsync(d_ring_, 0, need * sizeof(double), stream_), "force ring");
+}
+
+void Domain::probe_fold(uint64_t upto) {
+  if (upto <= fh_folded_) return;
+  if (upto - fh_dbase_ > static_cast<uint64_t>(kForceHistCap)) {
+    // the device history is full: move what is folded to the host
+    const size_t n = static_cast<size_t>(fh_folded_ - fh_host_);
+    const size_t at = h_fhist_.size();
+    h_fhist_.resize(at + 3 * n);
+    if (n)
+      check(hipMemcpyAsync(h_fhist_.data() + at, d_fhist_ + 3 * (fh_host_ - fh_dbase_), 3 * n * sizeof(double),
+                           hipMemcpyDeviceToHost, stream_),
+            "force history");
+    check(hipStreamSynchronize(stream_), "force history");
+    fh_host_ = fh_dbase_ = fh_folded_;
+  }
+  const int count = static_cast<int>(upto - fh_folded_);
+  check(launch_mem_fold(d_ring_, std::max(1, probe_rows_.nslots), kForceRing, static_cast<long long>(fh_folded_),
+                        count, d_fhist_, static_cast<long long>(fh_folded_ - fh_dbase_), stream_),
+        "force fold");
+  fh_folded_ = upto;
+}
+
+double* Domain::probe_slot(uint64_t k) {
+  if (k - fh_folded_ >= static_cast<uint64_t>(kForceRing)) probe_fold(k);
+  return d_ring_ + 3 * static_cast<size_t>(std::max(1, probe_rows_.nslots)) * (k % kForceRing);
+}
+
+// The sample of S_{step_count_} when no launch has formed it yet: read from
+// a materialised state or formed from the post-collision buffer.
+void Domain::probe_sample_current() {
+  if (fh_next_ != step_count_) return;
+  double* part = probe_slot(step_count_);
+  if (probe_rows_.nrows > 0) {
+    const StepArgs a = with_mem(args(false), probe_rows_, part, probe_c_, probe_rmax_);
+    const bool from_s = s_buf_ >= 0;
+    check(launch_mem_rows(lattice_, storage_, from_s, a, buf_[from_s ? s_buf_ : a_buf_], probe_rows_.d_list,
+                          probe_rows_.nrows, stream_),
+          "force sample");
+  }
+  fh_next_ = step_count_ + 1;
+}
+
+const std::vector<double>& Domain::force_history(uint64_t* first_step) {
+  if (!probe_) throw HipError("no force probe is set on this domain");
+  sync_device_config();
+  probe_sync();
+  probe_sample_current();
+  probe_fold(fh_next_);
+  const size_t n = static_cast<size_t>(fh_folded_ - fh_host_);
+  if (n) {
+    const size_t at = h_fhist_.size();
+    h_fhist_.resize(at + 3 * n);
+    check(hipMemcpyAsync(h_fhist_.data() + at, d_fhist_ + 3 * (fh_host_ - fh_dbase_), 3 * n * sizeof(double),
+                         hipMemcpyDeviceToHost, stream_),
+          "force history");
+    check(hipStreamSynchronize(stream_), "force history");
+    fh_host_ = fh_folded_;
+  }
+  if (first_step) *first_step = fh_first_;
+  return h_fhist_;
 }
 
 // ------------------------------------------------------------- temperature
 void Domain::enable_thermal(float tau_g, float gbeta, float t_ref, float t_init) {
   if (dim_ != 2) throw HipError("the temperature coupling is 2D (D2Q9 + D2Q5) only");
+  if (probe_) throw HipError("clear the force probe before enabling the temperature coupling");
   if (!(tau_g > 0.5f)) throw HipError("temperature relaxation time must exceed 1/2");
   before_config_change();
   if (!thermal_) {
@@ -1126,26 +1292,16 @@ void Domain::batch_advance(uint64_t k, int parity_after, float gx, float gy, flo
   invalidate_mirrors();
 }
 
-int Domain::batch_mem_links(float ccx, float ccy, float ccz, float r_max, const int** node, const int** q) {
+const int* Domain::batch_mem_rows(float ccx, float ccy, float ccz, float r_max, int* nrows) {
   sync_device_config();
-  const auto& lists = mem_link_list(ccx, ccy, ccz, r_max);
-  const int n = static_cast<int>(lists.first.size());
-  if (d_bmem_node_) {
-    check(hipStreamSynchronize(stream_), "batch links");
-    release(d_bmem_node_);
-    release(d_bmem_q_);
-    d_bmem_node_ = d_bmem_q_ = nullptr;
+  const auto key = std::make_tuple(ccx, ccy, ccz, r_max, cls_epoch_);
+  if (key != batch_rows_key_ || !batch_rows_.d_list) {
+    if (batch_rows_.d_rows) free_mem_rows(batch_rows_);
+    batch_rows_ = build_mem_rows(ccx, ccy, ccz, r_max);
+    batch_rows_key_ = key;
   }
-  if (n > 0) {
-    d_bmem_node_ = static_cast<int*>(alloc(n * sizeof(int)));
-    d_bmem_q_ = static_cast<int*>(alloc(n * sizeof(int)));
-    check(hipMemcpyAsync(d_bmem_node_, lists.first.data(), n * sizeof(int), hipMemcpyHostToDevice, stream_), "links");
-    check(hipMemcpyAsync(d_bmem_q_, lists.second.data(), n * sizeof(int), hipMemcpyHostToDevice, stream_), "links");
-    check(hipStreamSynchronize(stream_), "batch links");
-  }
-  *node = d_bmem_node_;
-  *q = d_bmem_q_;
-  return n;
+  *nrows = batch_rows_.nrows;
+  return batch_rows_.d_list;
 }
 
 size_t Domain::bytes_needed(int dim, int lattice, int nx, int ny, int nz, int storage, long long plane_pad,

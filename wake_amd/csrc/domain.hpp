@@ -126,6 +126,18 @@ class Domain {
   double max_speed();
   // F on S_k by momentum exchange (FP64): 3D (Fx, Fy, Fz); 2D (Fx, Fy, Tz).
   void momentum_exchange(float cx, float cy, float cz, float r_max, double out[3]);
+  // Per-step force history: the momentum exchange of every S_k from step
+  // current_step() on (links within r_max of the centre, r_max <= 0: all),
+  // computed inside the launch that forms S_k and folded per step in a
+  // fixed order, so the history holds the same values as calling
+  // momentum_exchange() after every step.  Setting it again restarts the
+  // history; initialisation restarts it at step 0.  Not for temperature-
+  // coupled domains.
+  void set_force_probe(float cx, float cy, float cz, float r_max);
+  void clear_force_probe();
+  bool force_probe() const { return probe_; }
+  // Samples of steps first .. current_step(), 3 doubles each.
+  const std::vector<double>& force_history(uint64_t* first_step);
 
   // --- accessors
   int dim() const { return dim_; }
@@ -174,9 +186,10 @@ class Domain {
   // A batch took k >= 1 steps; the state is now post-collision in buffer
   // `parity_after`.
   void batch_advance(uint64_t k, int parity_after, float gx, float gy, float gz);
-  // Momentum-exchange links around (cx, cy, cz) within r_max, uploaded to
-  // device arrays the domain owns (valid until the next call).
-  int batch_mem_links(float cx, float cy, float cz, float r_max, const int** node, const int** q);
+  // The rows holding the momentum-exchange links around (cx, cy, cz) within
+  // r_max: a device list the domain owns (valid until the next call or a
+  // geometry change), *nrows entries.
+  const int* batch_mem_rows(float cx, float cy, float cz, float r_max, int* nrows);
   // Device bytes a domain of this shape allocates.
   static size_t bytes_needed(int dim, int lattice, int nx, int ny, int nz, int storage, long long plane_pad = -1,
                              bool thermal = false);
@@ -199,7 +212,23 @@ class Domain {
   void release() noexcept;         // wait for the stream, free everything
   void release(void* p);
   void upload_prev(const std::function<float(size_t, int)>& value);
-  const std::pair<std::vector<int>, std::vector<int>>& mem_link_list(float cx, float cy, float cz, float r_max);
+  // Rows holding the momentum-exchange links of one (centre, r_max): per
+  // (y, z) row its first partial slot or -1, and the list of linked rows.
+  struct MemRows {
+    int* d_rows = nullptr;
+    int* d_list = nullptr;
+    int nrows = 0;
+    int nslots = 0;
+  };
+  MemRows build_mem_rows(float cx, float cy, float cz, float r_max);
+  void free_mem_rows(MemRows& r);
+  const MemRows& cached_mem_rows(float cx, float cy, float cz, float r_max);
+  StepArgs with_mem(StepArgs a, const MemRows& r, double* part, const float c[3], float r_max) const;
+  void probe_sync();                // probe rows current for the geometry
+  void probe_fold(uint64_t upto);   // fold the samples of steps [fh_folded_, upto)
+  void probe_sample_current();      // sample S_{step_count_} if no launch has
+  double* probe_slot(uint64_t k);   // ring entry of step k (folding first if it is taken)
+  void probe_restart();
 
   int dim_, lattice_, Q_, nx_, ny_, nz_;
   size_t N_;
@@ -250,16 +279,34 @@ class Domain {
   float* d_sout_ = nullptr;
   double* d_partial_ = nullptr;
   double* d_out_ = nullptr;
-  int* d_mem_node_ = nullptr;
-  int* d_mem_q_ = nullptr;
-  int mem_cap_ = 0;
-  int* d_bmem_node_ = nullptr;
-  int* d_bmem_q_ = nullptr;
   std::vector<void*> allocations_;
   size_t device_bytes_ = 0;
   bool flags_dirty_ = true, cls_dirty_ = true, bfl_dirty_ = true;
   uint64_t flags_version_ = 0;
-  std::map<std::tuple<float, float, float, float, uint64_t>, std::pair<std::vector<int>, std::vector<int>>> mem_links_;
+  uint64_t cls_epoch_ = 0;  // counts rebuilds of the class bytes (geometry or boundary changes)
+  std::map<std::tuple<float, float, float, float, uint64_t>, MemRows> mem_rows_cache_;
+  MemRows batch_rows_;  // the rows of batch_mem_rows(), with their key
+  std::tuple<float, float, float, float, uint64_t> batch_rows_key_{0.0f, 0.0f, 0.0f, 0.0f, ~0ull};
+  double* d_mempart_ = nullptr;  // partials of one momentum_exchange() call
+  size_t mempart_cap_ = 0;       // in doubles
+
+  // force probe: ring of per-step partials, folded into a device history
+  static constexpr int kForceRing = 64;           // steps between folds at most
+  static constexpr long long kForceHistCap = 1 << 16;  // device history rows before a download
+  bool probe_ = false;
+  float probe_c_[3] = {0.0f, 0.0f, 0.0f};
+  float probe_rmax_ = 0.0f;
+  MemRows probe_rows_;
+  uint64_t probe_epoch_ = ~0ull;
+  double* d_ring_ = nullptr;
+  size_t ring_cap_ = 0;  // in doubles
+  double* d_fhist_ = nullptr;
+  uint64_t fh_first_ = 0;   // step of the first sample
+  uint64_t fh_next_ = 0;    // first step whose sample is not written yet
+  uint64_t fh_folded_ = 0;  // samples before this step are folded into d_fhist_
+  uint64_t fh_dbase_ = 0;   // step of d_fhist_ row 0
+  uint64_t fh_host_ = 0;    // samples before this step are in h_fhist_
+  std::vector<double> h_fhist_;
 
   // temperature (D2Q5), ping-pong with buf_
   bool thermal_ = false;

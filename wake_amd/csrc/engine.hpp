@@ -75,7 +75,20 @@ struct StepArgs {
   float* speed;
   float* s_out;               // optional: S_k planes, fp32, stride nx*ny*nz
   float* temp;                // read-back temperature (thermal domains)
+  // Momentum exchange on the S_k a launch forms (mem_rows null: none).
+  // mem_rows[z * ny + y] is the first partial slot of row (y, z) or -1 for a
+  // row without links; wave w of a linked row (nodes x = 64 w .. 64 w + 63)
+  // writes its FP64 sum (3 doubles) to mem_part[3 (slot + w) ..].  Links run
+  // from Fluid nodes to in-range Solid / MovingWall neighbours whose midpoint
+  // lies within mem_rmax of mem_c (mem_rmax <= 0: every link).
+  const int* mem_rows;
+  double* mem_part;
+  float mem_c[3];
+  float mem_rmax;
 };
+
+// Partial slots of one force sample: rows with links x waves per row.
+inline int mem_waves_per_row(int nx) { return (nx + 63) / 64; }
 
 // Launchers (step_kernels.hip).  `lattice` is 9, 19 or 27.
 hipError_t launch_build_class(int lattice, const StepArgs& a, uint8_t* cls, hipStream_t s);
@@ -103,19 +116,20 @@ hipError_t launch_materialize(int lattice, int storage, const StepArgs& a, const
 hipError_t launch_prev_from_s(int lattice, int storage, const StepArgs& a, const void* s_state, hipStream_t s);
 hipError_t launch_readback(int lattice, int storage, bool from_s, const StepArgs& a, const void* buf,
                            hipStream_t s);
-// Momentum exchange over a link list (node, q) on S_k, accumulated in FP64
-// in a fixed order (up to max_blocks work-group partials of 3 doubles, then
-// their sum in block order): out[0..2] = F (3D) or (Fx, Fy, torque about
-// (cx, cy)) (2D).
-hipError_t launch_mem(int lattice, int storage, bool from_s, const StepArgs& a, const void* buf,
-                      const int* link_node, const int* link_q, int nlinks, float cx, float cy, double* partial,
-                      int max_blocks, double* out, hipStream_t s);
-// The same forces without a link list: one pass over the class bytes, S
-// formed only at nodes next to a wall; nblocks work-group partials summed in
-// block order.  r_max <= 0 takes every wall link.
-hipError_t launch_mem_scan(int lattice, int storage, bool from_s, const StepArgs& a, const void* buf, float cx,
-                           float cy, float cz, float r_max, double* partial, int nblocks, double* out,
-                           hipStream_t s);
+// Row marks of the momentum-exchange links of (a.mem_c, a.mem_rmax):
+// row_has[z * ny + y] = 1 for rows holding a link (row_has cleared by the
+// caller).
+hipError_t launch_mem_mark_rows(int lattice, const StepArgs& a, int* row_has, hipStream_t s);
+// The per-wave partials of a.mem_rows / a.mem_part for one S_k without a
+// step: over the `nrows` linked rows listed in row_list, from the
+// materialised S state (from_s) or formed from the post-collision buffer.
+// Same values and slots as a pull launch of k_step writes.
+hipError_t launch_mem_rows(int lattice, int storage, bool from_s, const StepArgs& a, const void* buf,
+                           const int* row_list, int nrows, hipStream_t s);
+// Sums of force samples: for s < count, out[3 (out_first + s) ..] = sum over
+// slots of part[3 (((first + s) % ring) nslots + slot) ..], slots in order.
+hipError_t launch_mem_fold(const double* part, int nslots, int ring, long long first, int count, double* out,
+                           long long out_first, hipStream_t s);
 // Vorticity (2D: omega_z) and, in 3D, the curl and the Q-criterion of a
 // velocity field, central differences with periodic wrap (analytics.hip).
 // Null outputs are skipped.

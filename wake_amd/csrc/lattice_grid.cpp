@@ -26,6 +26,13 @@ void finite_or_throw(const float* rho, const float* ux, std::size_t n) {
     if (!std::isfinite(rho[i]) || !std::isfinite(ux[i]))
       throw NumericalDivergenceException("non-finite density or velocity at node " + std::to_string(i));
 }
+std::vector<Vec3> to_vec3(const std::vector<double>& rows) {
+  std::vector<Vec3> v(rows.size() / 3);
+  for (std::size_t i = 0; i < v.size(); ++i)
+    v[i] = Vec3(static_cast<real_t>(rows[3 * i]), static_cast<real_t>(rows[3 * i + 1]),
+                static_cast<real_t>(rows[3 * i + 2]));
+  return v;
+}
 }  // namespace
 
 // ------------------------------------------------------------------ 3D
@@ -131,6 +138,12 @@ Vec3 LatticeGrid3D::momentum_exchange(real_t cx, real_t cy, real_t cz, real_t r_
   guarded([&] { domain_->momentum_exchange(cx, cy, cz, r_max, f); });
   return Vec3(static_cast<real_t>(f[0]), static_cast<real_t>(f[1]), static_cast<real_t>(f[2]));
 }
+void LatticeGrid3D::enable_force_history(real_t cx, real_t cy, real_t cz, real_t r_max) {
+  guarded([&] { domain_->set_force_probe(cx, cy, cz, r_max); });
+}
+std::vector<Vec3> LatticeGrid3D::force_history(uint64_t* first_step) const {
+  return guarded([&] { return to_vec3(domain_->force_history(first_step)); });
+}
 void LatticeGrid3D::check_finite() const { finite_or_throw(rho_data(), ux_data(), total_nodes()); }
 
 // ------------------------------------------------------------------ 2D
@@ -216,6 +229,12 @@ Vec3 LatticeGrid2D::momentum_exchange(real_t cx, real_t cy, real_t r_max) const 
   double f[3];
   guarded([&] { domain_->momentum_exchange(cx, cy, 0.0f, r_max, f); });
   return Vec3(static_cast<real_t>(f[0]), static_cast<real_t>(f[1]), static_cast<real_t>(f[2]));
+}
+void LatticeGrid2D::enable_force_history(real_t cx, real_t cy, real_t r_max) {
+  guarded([&] { domain_->set_force_probe(cx, cy, 0.0f, r_max); });
+}
+std::vector<Vec3> LatticeGrid2D::force_history(uint64_t* first_step) const {
+  return guarded([&] { return to_vec3(domain_->force_history(first_step)); });
 }
 void LatticeGrid2D::check_finite() const { finite_or_throw(rho_data(), ux_data(), total_nodes()); }
 

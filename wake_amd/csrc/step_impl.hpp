@@ -466,6 +466,86 @@ __device__ __forceinline__ void form_s(const StepArgs& a, const typename St::T* 
   form_s_at<L, St>(a, A, x, y, z, n, c, commit, a.s_step, f);
 }
 
+// ------------------------------------------------------ momentum exchange
+// The momentum-exchange links of Fluid node (x, y, z): every q whose
+// in-range neighbour x + c_q is Solid or MovingWall and whose link midpoint
+// lies within r_max of the centre (r_max <= 0: all), as
+// forces_momentum_exchange.ispc:71-138 walks them.  on_link(qc, rx, ry) is
+// called per link with the midpoint offset from the centre.  The midpoint
+// test rounds as the host's link builder (no fused multiply-add).
+template <class L, class OnLink>
+__device__ __forceinline__ void for_each_mem_link(const StepArgs& a, int x, int y, int z, float ccx, float ccy,
+                                                  float ccz, float r_max, OnLink&& on_link) {
+  const float r_max_sq = r_max * r_max;
+  static_for<L::Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    if constexpr (q > 0) {
+      constexpr int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
+      int ny_ = y + cy, nz_ = z + cz;
+      bool in = true;
+      if (a.per_y) ny_ = wrap(ny_, a.ny);
+      else in = in && ny_ >= 0 && ny_ < a.ny;
+      if (a.per_z) nz_ = wrap(nz_, a.nz);
+      else in = in && nz_ >= 0 && nz_ < a.nz;
+      const int nx_ = wrap(x + cx, a.nx);
+      const uint8_t t = in ? a.flags[(nz_ * a.ny + ny_) * a.nx + nx_] : static_cast<uint8_t>(kFluid);
+      if (t == kSolid || t == kMovingWall) {
+        const float rx = __fsub_rn(__fadd_rn(static_cast<float>(x), 0.5f * static_cast<float>(cx)), ccx);
+        const float ry = __fsub_rn(__fadd_rn(static_cast<float>(y), 0.5f * static_cast<float>(cy)), ccy);
+        const float rz = __fsub_rn(__fadd_rn(static_cast<float>(z), 0.5f * static_cast<float>(cz)), ccz);
+        float d2 = __fadd_rn(__fmul_rn(rx, rx), __fmul_rn(ry, ry));
+        if constexpr (L::D == 3) d2 = __fadd_rn(d2, __fmul_rn(rz, rz));
+        if (r_max <= 0.0f || d2 <= r_max_sq) on_link(qc, rx, ry);
+      }
+    }
+  });
+}
+
+// F += 2 f_q c_q over the links of one Fluid node on its S_k values f; in 2D
+// the third sum is the torque about the centre at the link midpoint.
+template <class L>
+__device__ __forceinline__ void mem_node(const StepArgs& a, int x, int y, int z, const float (&f)[L::Q], float ccx,
+                                         float ccy, float ccz, float r_max, double& s0, double& s1, double& s2) {
+  for_each_mem_link<L>(a, x, y, z, ccx, ccy, ccz, r_max, [&](auto qc, float rx, float ry) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
+    const float dfx = 2.0f * f[q] * static_cast<float>(cx);
+    const float dfy = 2.0f * f[q] * static_cast<float>(cy);
+    s0 += dfx;
+    s1 += dfy;
+    if constexpr (L::D == 2) s2 += static_cast<double>(rx * dfy - ry * dfx);
+    else s2 += static_cast<double>(2.0f * f[q] * static_cast<float>(cz));
+  });
+}
+
+// Sum over the 64 lanes of a wave in a fixed butterfly order; lane 0's
+// result is the same on every run.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+// One wave's share of a force sample: lanes with want = true add their
+// node's links (f = the node's S_k), lane 0 writes the wave's FP64 sums to
+// the row's slot.  Every lane of the wave must take part (the sum is a
+// cross-lane exchange); x is the lane's node column (lane 0: 64 w).
+template <class L>
+__device__ __forceinline__ void mem_wave(const StepArgs& a, bool want, int x, int y, int z, const float (&f)[L::Q],
+                                         int slot) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  if (want) mem_node<L>(a, x, y, z, f, a.mem_c[0], a.mem_c[1], a.mem_c[2], a.mem_rmax, s0, s1, s2);
+  s0 = wave_sum(s0);
+  s1 = wave_sum(s1);
+  s2 = wave_sum(s2);
+  if (__lane_id() == 0 && x < a.nx) {
+    double* p = a.mem_part + 3ll * (slot + x / 64);
+    p[0] = s0;
+    p[1] = s1;
+    p[2] = s2;
+  }
+}
+
 // ----------------------------------------------------------------- kernels
 template <class L>
 __global__ void k_build_class(const StepArgs a, uint8_t* __restrict__ cls) {
@@ -542,7 +622,14 @@ __global__ __launch_bounds__(256) WAKE_STEP_ATTR void k_step(const StepArgs a, c
                                                              typename St::T* __restrict__ aout) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y, z0 = blockIdx.z * WAKE_ZK;
-  if (x >= a.nx) return;
+  // Force sample of the S_k this launch forms: the slot of this row, or -1
+  // (rows without momentum-exchange links, launches without a sample).
+  int mem_slot = -1;
+  if constexpr (PULL && WAKE_ZK == 1) {
+    if (a.mem_rows) mem_slot = a.mem_rows[z0 * a.ny + y];
+  }
+  const bool live = x < a.nx;
+  if (!live && mem_slot < 0) return;
   const long long S = a.stride;
   auto fetch = [&](int z, float (&f)[L::Q], uint8_t& c, int& n) {
     n = (z * a.ny + y) * a.nx + x;
@@ -557,9 +644,16 @@ __global__ __launch_bounds__(256) WAKE_STEP_ATTR void k_step(const StepArgs a, c
     }
   };
   float fn[L::Q];
-  uint8_t cn;
-  int nn;
-  fetch(z0, fn, cn, nn);
+  uint8_t cn = 0;
+  int nn = 0;
+  if (live) fetch(z0, fn, cn, nn);
+  if constexpr (PULL && WAKE_ZK == 1) {
+    if (mem_slot >= 0) {
+      // every lane of the row's waves takes part in the sums
+      mem_wave<L>(a, live && (cn & kClsSlow) && a.flags[nn] == kFluid, x, y, z0, fn, mem_slot);
+      if (!live) return;
+    }
+  }
 #pragma unroll
   for (int k = 0; k < WAKE_ZK; ++k) {
     const int z = z0 + k;
@@ -663,57 +757,6 @@ __global__ __launch_bounds__(256) void k_readback(const StepArgs a, const typena
   }
 }
 
-// Momentum exchange contributions of this lane's links (i = first, first +
-// stride, ...) on S_k: F += 2 f_q c_q, and in 2D the torque about (ccx, ccy)
-// at the link midpoint (forces_momentum_exchange.ispc:6-138, restricted to
-// the precomputed links).
-template <class L, class St, bool FROM_S>
-__device__ __forceinline__ void mem_links(const StepArgs& a, const typename St::T* __restrict__ buf,
-                                          unsigned long long s_step, const int* __restrict__ link_node,
-                                          const int* __restrict__ link_q, int nlinks, int first, int stride,
-                                          float ccx, float ccy, double& s0, double& s1, double& s2) {
-  for (int i = first; i < nlinks; i += stride) {
-    const int n = link_node[i];
-    const int qd = link_q[i];
-    const int x = n % a.nx;
-    const int y = (n / a.nx) % a.ny;
-    const int z = n / (a.nx * a.ny);
-    float f[L::Q];
-    if constexpr (FROM_S) {
-      static_for<L::Q>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        f[q] = St::template ld<L, q>(buf, q * a.stride + n);
-      });
-    } else {
-      form_s_at<L, St, false>(a, buf, x, y, z, n, a.cls[n], false, s_step, f);
-    }
-    float fv = 0.0f;
-    int cx = 0, cy = 0, cz = 0;
-    static_for<L::Q>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      if (qd == q) {
-        fv = f[q];
-        cx = L::cx[q];
-        cy = L::cy[q];
-        cz = L::cz[q];
-      }
-    });
-    const float dfx = 2.0f * fv * static_cast<float>(cx);
-    const float dfy = 2.0f * fv * static_cast<float>(cy);
-    if constexpr (L::D == 2) {
-      const float rx = (static_cast<float>(x) + 0.5f * static_cast<float>(cx)) - ccx;
-      const float ry = (static_cast<float>(y) + 0.5f * static_cast<float>(cy)) - ccy;
-      s0 += dfx;
-      s1 += dfy;
-      s2 += static_cast<double>(rx * dfy - ry * dfx);
-    } else {
-      s0 += dfx;
-      s1 += dfy;
-      s2 += static_cast<double>(2.0f * fv * static_cast<float>(cz));
-    }
-  }
-}
-
 // Sum of three per-lane values over the work-group in a fixed tree order
 // (blockDim.x a power of two, at most 256); the result is valid in lane 0.
 __device__ __forceinline__ void block_sum3(double& s0, double& s1, double& s2) {
@@ -732,88 +775,49 @@ __device__ __forceinline__ void block_sum3(double& s0, double& s1, double& s2) {
   s2 = red[2][0];
 }
 
-// Momentum exchange without a link list: every Fluid node whose pull is slow
-// (so it has a solid or moving-wall neighbour) forms S_k and adds 2 f_q c_q
-// for each q whose in-range neighbour x + c_q is Solid or MovingWall and
-// whose link midpoint lies within r_max of the centre (r_max <= 0: all),
-// forces_momentum_exchange.ispc:71-138.  The midpoint test rounds as the
-// host's link builder (no fused multiply-add).  Work-group partials as k_mem.
-template <class L, class St, bool FROM_S>
-__global__ __launch_bounds__(256) void k_mem_scan(const StepArgs a, const typename St::T* __restrict__ buf, float ccx,
-                                                  float ccy, float ccz, float r_max, double* __restrict__ partial) {
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  const long long N = static_cast<long long>(a.nx) * a.ny * a.nz;
-  const float r_max_sq = r_max * r_max;
-  for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < N;
-       i += static_cast<long long>(gridDim.x) * blockDim.x) {
-    const int n = static_cast<int>(i);
-    const uint8_t c = a.cls[n];
-    if (!(c & kClsSlow) || a.flags[n] != kFluid) continue;
-    const int x = n % a.nx;
-    const int y = (n / a.nx) % a.ny;
-    const int z = n / (a.nx * a.ny);
-    float f[L::Q];
-    if constexpr (FROM_S) {
-      static_for<L::Q>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        f[q] = St::template ld<L, q>(buf, q * a.stride + n);
-      });
-    } else {
-      form_s_at<L, St, false>(a, buf, x, y, z, n, c, false, a.s_step, f);
-    }
-    static_for<L::Q>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      if constexpr (q > 0) {
-        constexpr int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
-        int ny_ = y + cy, nz_ = z + cz;
-        bool in = true;
-        if (a.per_y) ny_ = wrap(ny_, a.ny);
-        else in = in && ny_ >= 0 && ny_ < a.ny;
-        if (a.per_z) nz_ = wrap(nz_, a.nz);
-        else in = in && nz_ >= 0 && nz_ < a.nz;
-        const int nx_ = wrap(x + cx, a.nx);
-        const uint8_t t = in ? a.flags[(nz_ * a.ny + ny_) * a.nx + nx_] : static_cast<uint8_t>(kFluid);
-        if (t == kSolid || t == kMovingWall) {
-          const float rx = __fsub_rn(__fadd_rn(static_cast<float>(x), 0.5f * static_cast<float>(cx)), ccx);
-          const float ry = __fsub_rn(__fadd_rn(static_cast<float>(y), 0.5f * static_cast<float>(cy)), ccy);
-          const float rz = __fsub_rn(__fadd_rn(static_cast<float>(z), 0.5f * static_cast<float>(cz)), ccz);
-          float d2 = __fadd_rn(__fmul_rn(rx, rx), __fmul_rn(ry, ry));
-          if constexpr (L::D == 3) d2 = __fadd_rn(d2, __fmul_rn(rz, rz));
-          if (r_max <= 0.0f || d2 <= r_max_sq) {
-            const float dfx = 2.0f * f[q] * static_cast<float>(cx);
-            const float dfy = 2.0f * f[q] * static_cast<float>(cy);
-            s0 += dfx;
-            s1 += dfy;
-            if constexpr (L::D == 2) s2 += static_cast<double>(rx * dfy - ry * dfx);
-            else s2 += static_cast<double>(2.0f * f[q] * static_cast<float>(cz));
-          }
-        }
-      }
-    });
-  }
-  block_sum3(s0, s1, s2);
-  if (threadIdx.x == 0) {
-    partial[3 * blockIdx.x] = s0;
-    partial[3 * blockIdx.x + 1] = s1;
-    partial[3 * blockIdx.x + 2] = s2;
-  }
+// Rows holding a momentum-exchange link of (a.mem_c, a.mem_rmax): one lane
+// per node, row_has[row] = 1 (an idempotent store; cleared by the caller).
+template <class L>
+__global__ __launch_bounds__(256) void k_mem_mark_rows(const StepArgs a, int* __restrict__ row_has) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y, z = blockIdx.z;
+  if (x >= a.nx) return;
+  const int n = (z * a.ny + y) * a.nx + x;
+  if (!(a.cls[n] & kClsSlow) || a.flags[n] != kFluid) return;
+  bool any = false;
+  for_each_mem_link<L>(a, x, y, z, a.mem_c[0], a.mem_c[1], a.mem_c[2], a.mem_rmax,
+                       [&](auto, float, float) { any = true; });
+  if (any) row_has[z * a.ny + y] = 1;
 }
 
-// Momentum exchange on S_k over a link list, FP64: each work-group sums a
-// grid-strided share of the links in a fixed tree order into partial[3 b];
-// k_sum3 adds the partials in block order, so the result is reproducible.
+// The partials of one force sample without a step: one work-group row per
+// linked row (row_list), S_k read from a materialised state or formed from
+// the post-collision buffer without committing the outlet store.  The S_k
+// values, slots and sums are those a pull launch of k_step produces.
 template <class L, class St, bool FROM_S>
-__global__ __launch_bounds__(256) void k_mem(const StepArgs a, const typename St::T* __restrict__ buf,
-                                             const int* __restrict__ link_node, const int* __restrict__ link_q,
-                                             int nlinks, float ccx, float ccy, double* __restrict__ partial) {
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  mem_links<L, St, FROM_S>(a, buf, a.s_step, link_node, link_q, nlinks, blockIdx.x * blockDim.x + threadIdx.x,
-                           gridDim.x * blockDim.x, ccx, ccy, s0, s1, s2);
-  block_sum3(s0, s1, s2);
-  if (threadIdx.x == 0) {
-    partial[3 * blockIdx.x] = s0;
-    partial[3 * blockIdx.x + 1] = s1;
-    partial[3 * blockIdx.x + 2] = s2;
+__global__ __launch_bounds__(256) void k_mem_rows(const StepArgs a, const typename St::T* __restrict__ buf,
+                                                  const int* __restrict__ row_list, int nrows) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = x < a.nx;
+  for (int r = blockIdx.y; r < nrows; r += gridDim.y) {  // uniform over the work-group
+    const int row = row_list[r];
+    const int y = row % a.ny, z = row / a.ny;
+    const int n = (z * a.ny + y) * a.nx + (live ? x : 0);
+    const uint8_t c = live ? a.cls[n] : static_cast<uint8_t>(0);
+    const bool want = live && (c & kClsSlow) && a.flags[n] == kFluid;
+    float f[L::Q];
+    static_for<L::Q>([&](auto qc) { f[decltype(qc)::value] = 0.0f; });
+    if (want) {
+      if constexpr (FROM_S) {
+        static_for<L::Q>([&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          f[q] = St::template ld<L, q>(buf, q * a.stride + n);
+        });
+      } else {
+        form_s_at<L, St, false>(a, buf, x, y, z, n, c, false, a.s_step, f);
+      }
+    }
+    mem_wave<L>(a, want, x, y, z, f, a.mem_rows[row]);
   }
 }
 

@@ -82,35 +82,6 @@ hipError_t launch_step(int lattice, int storage, int collision, bool les, bool p
   }
 }
 
-// Sum of nblocks (x, y, z) partials in block order.
-__global__ void k_sum3(const double* __restrict__ partial, int nblocks, double* __restrict__ out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  for (int b = 0; b < nblocks; ++b) {
-    s0 += partial[3 * b];
-    s1 += partial[3 * b + 1];
-    s2 += partial[3 * b + 2];
-  }
-  out[0] = s0;
-  out[1] = s1;
-  out[2] = s2;
-}
-
-hipError_t launch_mem_scan(int lattice, int storage, bool from_s, const StepArgs& a, const void* buf, float cx,
-                           float cy, float cz, float r_max, double* partial, int nblocks, double* out,
-                           hipStream_t s) {
-  hipError_t e;
-  switch (lattice) {
-    case 9: e = detail::mem_scan_d2q9(storage, from_s, a, buf, cx, cy, cz, r_max, partial, nblocks, s); break;
-    case 19: e = detail::mem_scan_d3q19(storage, from_s, a, buf, cx, cy, cz, r_max, partial, nblocks, s); break;
-    case 27: e = detail::mem_scan_d3q27(storage, from_s, a, buf, cx, cy, cz, r_max, partial, nblocks, s); break;
-    default: return hipErrorInvalidValue;
-  }
-  if (e != hipSuccess) return e;
-  k_sum3<<<1, 64, 0, s>>>(partial, nblocks, out);
-  return hipGetLastError();
-}
-
 hipError_t launch_prev_from_s(int lattice, int storage, const StepArgs& a, const void* s_state, hipStream_t s) {
   switch (lattice) {
     case 9: return detail::prev_from_s_d2q9(storage, a, s_state, s);
@@ -140,19 +111,51 @@ hipError_t launch_readback(int lattice, int storage, bool from_s, const StepArgs
   }
 }
 
-hipError_t launch_mem(int lattice, int storage, bool from_s, const StepArgs& a, const void* buf,
-                      const int* link_node, const int* link_q, int nlinks, float cx, float cy, double* partial,
-                      int max_blocks, double* out, hipStream_t s) {
-  const int nb = std::max(1, std::min(max_blocks, (nlinks + 255) / 256));
-  hipError_t e;
+hipError_t launch_mem_mark_rows(int lattice, const StepArgs& a, int* row_has, hipStream_t s) {
   switch (lattice) {
-    case 9: e = detail::mem_d2q9(storage, from_s, a, buf, link_node, link_q, nlinks, cx, cy, partial, nb, s); break;
-    case 19: e = detail::mem_d3q19(storage, from_s, a, buf, link_node, link_q, nlinks, cx, cy, partial, nb, s); break;
-    case 27: e = detail::mem_d3q27(storage, from_s, a, buf, link_node, link_q, nlinks, cx, cy, partial, nb, s); break;
+    case 9: return detail::mem_mark_rows_d2q9(a, row_has, s);
+    case 19: return detail::mem_mark_rows_d3q19(a, row_has, s);
+    case 27: return detail::mem_mark_rows_d3q27(a, row_has, s);
     default: return hipErrorInvalidValue;
   }
-  if (e != hipSuccess) return e;
-  k_sum3<<<1, 64, 0, s>>>(partial, nb, out);
+}
+
+hipError_t launch_mem_rows(int lattice, int storage, bool from_s, const StepArgs& a, const void* buf,
+                           const int* row_list, int nrows, hipStream_t s) {
+  switch (lattice) {
+    case 9: return detail::mem_rows_d2q9(storage, from_s, a, buf, row_list, nrows, s);
+    case 19: return detail::mem_rows_d3q19(storage, from_s, a, buf, row_list, nrows, s);
+    case 27: return detail::mem_rows_d3q27(storage, from_s, a, buf, row_list, nrows, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// One work-group per force sample: the slots summed in a fixed order
+// (lane-strided, then a tree over the work-group).
+__global__ __launch_bounds__(256) void k_mem_fold(const double* __restrict__ part, int nslots, int ring,
+                                                  long long first, double* __restrict__ out, long long out_first) {
+  const long long k = first + blockIdx.x;
+  const double* p = part + 3ll * nslots * (k % ring);
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int i = threadIdx.x; i < nslots; i += 256) {
+    s0 += p[3 * i];
+    s1 += p[3 * i + 1];
+    s2 += p[3 * i + 2];
+  }
+  block_sum3(s0, s1, s2);
+  if (threadIdx.x == 0) {
+    double* o = out + 3 * (out_first + blockIdx.x);
+    o[0] = s0;
+    o[1] = s1;
+    o[2] = s2;
+  }
+}
+
+hipError_t launch_mem_fold(const double* part, int nslots, int ring, long long first, int count, double* out,
+                           long long out_first, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  if (nslots <= 0 || ring <= 0 || count > ring) return hipErrorInvalidValue;
+  k_mem_fold<<<count, 256, 0, s>>>(part, nslots, ring, first, out, out_first);
   return hipGetLastError();
 }
 

@@ -96,15 +96,22 @@ struct Launch {
     });
   }
 
-  static hipError_t mem_scan(int storage, bool from_s, const StepArgs& a, const void* buf, float cx, float cy,
-                             float cz, float r_max, double* partial, int nblocks, hipStream_t s) {
+  static hipError_t mem_mark_rows(const StepArgs& a, int* row_has, hipStream_t s) {
+    const int bx = pick_block(a.nx, 0);
+    k_mem_mark_rows<L><<<grid_of(a, bx), bx, 0, s>>>(a, row_has);
+    return hipGetLastError();
+  }
+
+  static hipError_t mem_rows(int storage, bool from_s, const StepArgs& a, const void* buf, const int* row_list,
+                             int nrows, hipStream_t s) {
+    if (nrows <= 0) return hipSuccess;
+    const int bx = pick_block(a.nx, 0);
+    const dim3 g((a.nx + bx - 1) / bx, nrows < 65535 ? nrows : 65535, 1);
     return with_storage(storage, [&](auto st) {
       using St = decltype(st);
       using T = typename St::T;
-      if (from_s)
-        k_mem_scan<L, St, true><<<nblocks, 256, 0, s>>>(a, static_cast<const T*>(buf), cx, cy, cz, r_max, partial);
-      else
-        k_mem_scan<L, St, false><<<nblocks, 256, 0, s>>>(a, static_cast<const T*>(buf), cx, cy, cz, r_max, partial);
+      if (from_s) k_mem_rows<L, St, true><<<g, bx, 0, s>>>(a, static_cast<const T*>(buf), row_list, nrows);
+      else k_mem_rows<L, St, false><<<g, bx, 0, s>>>(a, static_cast<const T*>(buf), row_list, nrows);
       return hipGetLastError();
     });
   }
@@ -140,21 +147,6 @@ struct Launch {
     });
   }
 
-  static hipError_t mem(int storage, bool from_s, const StepArgs& a, const void* buf, const int* link_node,
-                        const int* link_q, int nlinks, float cx, float cy, double* partial, int nblocks,
-                        hipStream_t s) {
-    return with_storage(storage, [&](auto st) {
-      using St = decltype(st);
-      using T = typename St::T;
-      if (from_s)
-        k_mem<L, St, true><<<nblocks, 256, 0, s>>>(a, static_cast<const T*>(buf), link_node, link_q, nlinks, cx, cy,
-                                                   partial);
-      else
-        k_mem<L, St, false><<<nblocks, 256, 0, s>>>(a, static_cast<const T*>(buf), link_node, link_q, nlinks, cx,
-                                                    cy, partial);
-      return hipGetLastError();
-    });
-  }
 };
 
 // Per-lattice entry points, defined in step_d2q9.hip / step_d3q19.hip / step_d3q27.hip.
@@ -164,12 +156,10 @@ struct Launch {
                          void* a_out, int block_x, hipStream_t s);                                             \
   hipError_t materialize_##NAME(int storage, const StepArgs& a, const void* a_in, void* s_state, hipStream_t s); \
   hipError_t prev_from_s_##NAME(int storage, const StepArgs& a, const void* s_state, hipStream_t s);             \
-  hipError_t mem_scan_##NAME(int storage, bool from_s, const StepArgs& a, const void* buf, float cx, float cy,    \
-                             float cz, float r_max, double* partial, int nblocks, hipStream_t s);               \
   hipError_t readback_##NAME(int storage, bool from_s, const StepArgs& a, const void* buf, hipStream_t s);      \
-  hipError_t mem_##NAME(int storage, bool from_s, const StepArgs& a, const void* buf, const int* link_node,    \
-                        const int* link_q, int nlinks, float cx, float cy, double* partial, int nblocks,        \
-                        hipStream_t s);
+  hipError_t mem_mark_rows_##NAME(const StepArgs& a, int* row_has, hipStream_t s);                             \
+  hipError_t mem_rows_##NAME(int storage, bool from_s, const StepArgs& a, const void* buf, const int* row_list, \
+                             int nrows, hipStream_t s);
 
 #define WAKE_DEFINE_LATTICE(NAME, LAT)                                                                          \
   hipError_t build_class_##NAME(const StepArgs& a, uint8_t* cls, hipStream_t s) {                               \
@@ -185,17 +175,15 @@ struct Launch {
   hipError_t prev_from_s_##NAME(int storage, const StepArgs& a, const void* s_state, hipStream_t s) {            \
     return Launch<LAT>::prev_from_s(storage, a, s_state, s);                                                    \
   }                                                                                                             \
-  hipError_t mem_scan_##NAME(int storage, bool from_s, const StepArgs& a, const void* buf, float cx, float cy,    \
-                             float cz, float r_max, double* partial, int nblocks, hipStream_t s) {              \
-    return Launch<LAT>::mem_scan(storage, from_s, a, buf, cx, cy, cz, r_max, partial, nblocks, s);              \
-  }                                                                                                             \
   hipError_t readback_##NAME(int storage, bool from_s, const StepArgs& a, const void* buf, hipStream_t s) {      \
     return Launch<LAT>::readback(storage, from_s, a, buf, s);                                                   \
   }                                                                                                             \
-  hipError_t mem_##NAME(int storage, bool from_s, const StepArgs& a, const void* buf, const int* link_node,     \
-                        const int* link_q, int nlinks, float cx, float cy, double* partial, int nblocks,         \
-                        hipStream_t s) {                                                                         \
-    return Launch<LAT>::mem(storage, from_s, a, buf, link_node, link_q, nlinks, cx, cy, partial, nblocks, s);    \
+  hipError_t mem_mark_rows_##NAME(const StepArgs& a, int* row_has, hipStream_t s) {                            \
+    return Launch<LAT>::mem_mark_rows(a, row_has, s);                                                           \
+  }                                                                                                             \
+  hipError_t mem_rows_##NAME(int storage, bool from_s, const StepArgs& a, const void* buf, const int* row_list, \
+                             int nrows, hipStream_t s) {                                                        \
+    return Launch<LAT>::mem_rows(storage, from_s, a, buf, row_list, nrows, s);                                  \
   }
 
 WAKE_DECLARE_LATTICE(d2q9)

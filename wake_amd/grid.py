@@ -237,6 +237,26 @@ class _Domain:
                 "momentum_exchange")
         return out
 
+    def set_force_probe(self, cx, cy, cz, r_max):
+        """Keep the momentum exchange of every step from now on (computed in
+        the step launch itself); read it with force_history()."""
+        self._c(self._L.wake_domain_set_force_probe(self._h, cx, cy, cz, r_max), "set_force_probe")
+
+    def clear_force_probe(self):
+        self._c(self._L.wake_domain_clear_force_probe(self._h), "clear_force_probe")
+
+    def force_history(self):
+        """(first_step, F): F[i] = forces of step first_step + i, shape (K, 3)
+        float64, up to the current step."""
+        first = ctypes.c_longlong(0)
+        count = ctypes.c_longlong(0)
+        self._c(self._L.wake_domain_force_history(self._h, None, 0, ctypes.byref(first), ctypes.byref(count)),
+                "force_history")
+        out = np.zeros((count.value, 3), np.float64)
+        self._c(self._L.wake_domain_force_history(self._h, _lib.ptr(out), count.value, ctypes.byref(first),
+                                                  ctypes.byref(count)), "force_history")
+        return first.value, out
+
 
 class LatticeGrid2D(_Domain):
     """D2Q9 grid (lattice_grid_2d.hpp:14-113).  step() streams with walls in
