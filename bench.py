@@ -7,7 +7,12 @@ convective outflow, periodic y/z.  One step = one fused pull-stream-collide
 launch over the whole domain; MLUPS counts every node, as the reference does
 (sphere_3d_wake.cpp:81-92).
 
-`--workload batched-wakes` runs the batched multi-domain path instead: 512
+By default the line also nests the metric's batched-serve half: a
+`batched` record (the 512-domain batched wakes below at 2000 steps) and a
+`serve` record (the request stream below at 0.2x its step counts), each
+against its plain one-at-a-time path (`--sub-records none` skips them).
+
+`--workload batched-wakes` runs the batched multi-domain path alone: 512
 D2Q9 MRT cylinder-wake domains of uneven size, one launch per step over all
 of them, and compares simulations per second with the plain
 one-domain-at-a-time path.  `--workload serve` replays a Poisson stream of
@@ -74,27 +79,68 @@ def sphere_workload(grid_cls, n=256, R=24.0, storage=0, lattice=19, plane_pad=-1
     return g, nl
 
 
-def cpu_baseline(n, steps, R=24.0):
-    """Oracle (unfused restatement of the reference step) on the host cores."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n, steps, R=24.0, n_1t=128, steps_1t=10):
+    """The oracle's unfused restatement of the reference step sequence on the
+    host cores, built here for this host (-O3 -march=native with fused
+    multiply-adds, as the reference's ISPC path is built with
+    --opt=fast-math), timed on every thread on the n^3 sphere workload and on
+    one thread on an n_1t^3 one.  Falls back to the prebuilt x86-64-v3 FMA
+    build if the host build fails."""
+    import subprocess
+    import tempfile
+
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind as ob
 
-    L = ob.lib(32)
-    o = ob.OracleGrid(3, n, n, n, tau=0.505, bits=32)
-    o.set_periodic_axes(True, True)
-    o.initialize(1.0, 0.05, 0.0, 0.0)
-    o.set_les_coupled(True)
-    o.set_inflow_west(0.05, 0.0)
-    o.set_inflow_turbulence(0.01, 2)
-    o.set_convective_outflow_east(0.05)
-    c = 64.0 * n / 256
-    o.set_solid_sphere(c, n / 2, n / 2, R * n / 256)
-    o.build_curved_sphere_links(c, n / 2, n / 2, R * n / 256)
-    o.step("MRT", use_les=True, smag=0.17)  # warm-up
-    t0 = time.perf_counter()
-    o.run(steps, "MRT", use_les=True, smag=0.17)
-    dt = time.perf_counter() - t0
-    return n ** 3 * steps / dt / 1e6, L.oracle_threads(), dt
+    out_dir = tempfile.mkdtemp(prefix="wake_oracle_native_")
+    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native", f"NATIVE_DIR={out_dir}"],
+                       capture_output=True, text=True)
+    if r.returncode == 0:
+        key, build = "32native", "gcc -O3 -march=native -ffp-contract=fast -fopenmp (built on this host)"
+        L = ob.lib(key, path=os.path.join(out_dir, "liboracle32native.so"))
+    else:
+        log(f"host-native oracle build failed, timing the x86-64-v3 FMA build: {r.stderr.strip()[:200]}")
+        key, build = "32fma", "gcc -O3 -march=x86-64-v3 -ffp-contract=fast -fopenmp (prebuilt)"
+        L = ob.lib(key)
+    all_threads = L.oracle_threads()
+
+    def timed(m, k, threads):
+        L.oracle_set_threads(threads)
+        o = ob.OracleGrid(3, m, m, m, tau=0.505, bits=key)
+        o.set_periodic_axes(True, True)
+        o.initialize(1.0, 0.05, 0.0, 0.0)
+        o.set_les_coupled(True)
+        o.set_inflow_west(0.05, 0.0)
+        o.set_inflow_turbulence(0.01, 2)
+        o.set_convective_outflow_east(0.05)
+        c = 64.0 * m / 256
+        o.set_solid_sphere(c, m / 2, m / 2, R * m / 256)
+        o.build_curved_sphere_links(c, m / 2, m / 2, R * m / 256)
+        o.step("MRT", use_les=True, smag=0.17)  # warm-up
+        t0 = time.perf_counter()
+        o.run(k, "MRT", use_les=True, smag=0.17)
+        dt = time.perf_counter() - t0
+        return m ** 3 * k / dt / 1e6, dt
+
+    v_all, dt_all = timed(n, steps, all_threads)
+    v_one, dt_one = timed(n_1t, steps_1t, 1)
+    L.oracle_set_threads(all_threads)
+    return {"value": round(v_all, 2), "unit": "MLUPS", "cores": all_threads, "kind": "port",
+            "sample": f"oracle (unfused reference step sequence, FP32) on the {n}^3 sphere workload, {steps} timed "
+                      f"steps after 1 warm-up on {all_threads} threads ({dt_all:.1f} s); one thread: {n_1t}^3, "
+                      f"{steps_1t} steps ({dt_one:.1f} s)",
+            "single_thread_mlups": round(v_one, 3), "build": build, "host": _cpu_model()}
 
 
 def batched_wakes(wa, ndom, seed=4, umax=0.05, storage=0):
@@ -137,12 +183,13 @@ def batched_wakes(wa, ndom, seed=4, umax=0.05, storage=0):
     return grids, geo
 
 
-def main_batched(args):
-    """Batched cylinder wakes: every domain stepped `--steps` times by one
+def batched_record(args):
+    """Batched cylinder wakes: every domain stepped `steps` times by one
     launch per step (graph-captured chunks of 100, autotuned launch shape),
     force history per step, C_D and Strouhal per domain; the plain
-    one-domain-at-a-time path is timed per domain on a short run and scaled
-    to the same step count."""
+    one-domain-at-a-time path (the single-domain step with the same per-step
+    force, taken inside its step launch) is timed per domain on a short run
+    and scaled to the same step count.  Returns the record as a dict."""
     import numpy as np
 
     import wake_amd
@@ -156,26 +203,24 @@ def main_batched(args):
     nodes = sum(g.total_nodes() for g in grids)
     log(f"setup done in {time.perf_counter() - t0:.1f}s, {nodes / 1e6:.1f} M nodes")
 
-    # plain path: each domain alone, timed per domain, scaled to S steps;
-    # once the steps alone (device time), once with the force the reference's
-    # scenario takes after every step (karman_vortex.cpp:104-123) through the
-    # single-domain call (wall time: the call returns the force to the host)
-    t_plain = t_plain_f = 0.0
+    # plain path: each domain alone with its per-step force, wall time of a
+    # short run scaled to S steps (small domains are launch-bound there)
+    t_plain = 0.0
     probe = args.plain_probe
     for g, (cx, cy, D, U, re) in zip(grids, geo):
+        g.set_force_probe(cx, cy, 0.0, 1.5 * D)
         g.run(2, "MRT", use_les=True, smag=0.17)
-        ms = g.run(probe, "MRT", use_les=True, smag=0.17, timed=True)
-        t_plain += ms / probe * S * 1e-3
+        g.synchronize()
         t0 = time.perf_counter()
-        for _ in range(probe):
-            g.step("MRT", 0.0, 0.0, True, 0.17)
-            g.momentum_exchange_2d(cx, cy, 1.5 * D)
-        t_plain_f += (time.perf_counter() - t0) / probe * S
-    log(f"plain path: {t_plain:.2f} s projected for {S} steps of every domain ({probe}-step probe each); "
-        f"with the per-step force: {t_plain_f:.2f} s")
+        g.run(probe, "MRT", use_les=True, smag=0.17)
+        g.force_history()
+        t_plain += (time.perf_counter() - t0) / probe * S
+        g.clear_force_probe()
+    log(f"plain path with the per-step force: {t_plain:.2f} s projected for {S} steps of every domain "
+        f"({probe}-step probe each)")
 
     b = wake_amd.Batch(lattice=9, storage=storage, collision="MRT", use_les=True, chunk=100)
-    slots = [b.add(g, S - 2 - 2 * probe, smag=0.17, forces=(cx, cy, 0.0, 1.5 * D))
+    slots = [b.add(g, S - 2 - probe, smag=0.17, forces=(cx, cy, 0.0, 1.5 * D))
              for g, (cx, cy, D, U, re) in zip(grids, geo)]
     t0 = time.perf_counter()
     shape = b.tune(probe_steps=4)
@@ -201,41 +246,40 @@ def main_batched(args):
         f = analysis.analyze_frequency((F[-4096:, 1] / q).astype(np.float32), 1.0)
         sts.append(analysis.strouhal_number(f, D, U))
     log(f"C_D median {np.median(cds):.3f} [{np.min(cds):.3f}, {np.max(cds):.3f}], "
-        f"St median {np.median(sts):.3f}, non-finite domains {bad}")
-    out = {
-        "metric": METRIC,
-        "value": round(ndom / t_batch, 4),
-        "unit": "sims/s",
-        "n_gpus": 1,
-        "steps": S,
-        "warmup": 2 + 2 * probe,
-        "ms_per_step": round(wall * 1e3 / left, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
+        f"St median {np.median(sts):.3f}, non-finite domains {bad}; {ndom / t_batch:.3f} sims/s batched vs "
+        f"{ndom / t_plain:.3f} plain ({t_plain / t_batch:.2f}x)")
+    return {
+        "metric": METRIC, "value": round(ndom / t_batch, 4), "unit": "sims/s", "n_gpus": 1, "steps": S,
+        "warmup": 2 + probe, "ms_per_step": round(wall * 1e3 / left, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32" if storage == 0 else ("fp16-storage/fp32" if storage == 1 else "bf16-storage/fp32"),
         "data": "synthetic",
         "config": {
-            "workload": f"{ndom} D2Q9 MRT+Smagorinsky(Cs=0.17) cylinder wakes (NX 128-1024 step 32, NY=NX/4, D=NY/8, parabolic "
-                        f"Zou-He inlet umax=0.05, Re 50-300 log-uniform, seed 4), one launch per step over all "
-                        f"domains, graph chunks of 100 steps",
+            "workload": f"{ndom} D2Q9 MRT+Smagorinsky(Cs=0.17) cylinder wakes (NX 128-1024 step 32, NY=NX/4, D=NY/8, "
+                        f"parabolic Zou-He inlet umax=0.05, Re 50-300 log-uniform, seed 4), {S} steps each with the "
+                        f"force of every step, one launch per step over all domains, graph chunks of 100 steps",
             "nodes": nodes, "launch_shape": {"block": shape[0], "grid_factor": shape[1]},
             "batched_mlups": round(mlups, 1), "plain_sims_per_s": round(ndom / t_plain, 4),
             "speedup_vs_plain": round(t_plain / t_batch, 3),
-            "plain_with_forces_sims_per_s": round(ndom / t_plain_f, 4),
-            "speedup_vs_plain_with_forces": round(t_plain_f / t_batch, 3),
+            "plain": "single-domain step, one domain at a time, per-step force inside its step launch, wall time "
+                     f"of a {probe}-step run per domain scaled to {S} steps",
+            "error": "arithmetic identical to the plain path per node (bit-identical batch of one); members vs "
+                     "the FP32 oracle within the FP32-floor criterion (tests/test_gpu_batch.py, test_gpu_configs.py)",
             "C_D_median": round(float(np.median(cds)), 4) if cds else None,
             "St_median": round(float(np.median(sts)), 4) if sts else None,
             "non_finite_domains": bad, "parallelism": "single GPU",
         },
     }
-    print(json.dumps(out), flush=True)
 
 
-def main_serve(args):
+def main_batched(args):
+    print(json.dumps(batched_record(args)), flush=True)
+
+
+def serve_record(args):
     """Serving stream: Poisson arrivals of mixed requests under an HBM
     budget, served by the continuous-batching loop, then the same stream
-    through the plain one-request-at-a-time FP32 loop."""
+    through the plain one-request-at-a-time FP32 loop.  Returns the record."""
     import numpy as np
 
     import wake_amd  # noqa: F401
@@ -256,29 +300,23 @@ def main_serve(args):
         log(f"per-request results written to {args.results}")
     plain, t_plain = serve.plain_loop(reqs, realtime=True)
     lat_p = serve.latency_stats(plain)
-    log(f"plain:  {t_plain:.2f} s makespan, p50 {lat_p['p50_s']:.2f} s, p99 {lat_p['p99_s']:.2f} s")
+    log(f"plain:  {t_plain:.2f} s makespan, p50 {lat_p['p50_s']:.2f} s, p99 {lat_p['p99_s']:.2f} s; "
+        f"{t_plain / t_srv:.2f}x")
     # agreement of served (FP16 where allowed) and plain FP32 results
     by = {r.rid: r for r in plain}
     dev = []
     for r in s.done:
         for k, v in r.result.items():
-            ref = by[r.rid].result[k]
-            if k == "C_L" or not (math.isfinite(v) and math.isfinite(ref)):
+            ref = by[r.rid].result.get(k)
+            if k == "C_L" or not isinstance(v, float) or not isinstance(ref, float):
+                continue
+            if not (math.isfinite(v) and math.isfinite(ref)):
                 continue
             dev.append(abs(v / ref - 1) if ref != 0 else abs(v))
-    out = {
-        "metric": METRIC,
-        "value": round(len(reqs) / t_srv, 4),
-        "unit": "sims/s",
-        "n_gpus": 1,
-        "steps": int(sum(r.steps for r in reqs)),
-        "warmup": 0,
-        "ms_per_step": None,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp16/fp32-storage, fp32 arithmetic",
-        "data": "synthetic",
+    return {
+        "metric": METRIC, "value": round(len(reqs) / t_srv, 4), "unit": "sims/s", "n_gpus": 1,
+        "steps": int(sum(r.steps for r in reqs)), "warmup": 0, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp16/fp32-storage, fp32 arithmetic", "data": "synthetic",
         "config": {
             "workload": f"serving stream: {len(reqs)} Poisson arrivals at {args.rate}/s (seed 5), 60% D3Q19 MRT-LES "
                         f"rotated-ellipsoid bluff bodies 64^3-192^3, 40% D2Q9+D2Q5 Boussinesq cavities 128^2-512^2, "
@@ -289,12 +327,16 @@ def main_serve(args):
             "plain_p50_s": round(lat_p["p50_s"], 3), "plain_p99_s": round(lat_p["p99_s"], 3),
             "aggregate_mlups": round(work / t_srv / 1e6, 1), "plain_mlups": round(work / t_plain / 1e6, 1),
             "peak_hbm_gb": round(s.peak_used / 1e9, 2),
+            "diverged_requests": len(s.diverged),
             "result_rel_dev_vs_plain_fp32": {"median": float(np.median(dev)) if dev else None,
                                              "max": float(np.max(dev)) if dev else None},
             "parallelism": "single GPU",
         },
     }
-    print(json.dumps(out), flush=True)
+
+
+def main_serve(args):
+    print(json.dumps(serve_record(args)), flush=True)
 
 
 def wing_setup(g, nx, ny, chord, u=0.04, aoa=8.0):
@@ -417,6 +459,12 @@ def main():
     ap.add_argument("--step-scale", type=float, default=1.0)
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--results", default=None, help="serve: write per-request results (JSON lines) here")
+    ap.add_argument("--sub-records", default="batched,serve",
+                    help="sphere workload: also run these (comma-separated: batched, serve; 'none') and nest their "
+                         "records in the line -- the metric's batched-serve half")
+    ap.add_argument("--batched-steps", type=int, default=2000, help="steps per domain of the nested batched record")
+    ap.add_argument("--serve-step-scale", type=float, default=0.2,
+                    help="step scale of the nested serve record's request stream")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -507,11 +555,9 @@ def main():
     cpu = None
     if args.cpu_steps > 0:
         log(f"CPU baseline: oracle {args.cpu_n}^3, {args.cpu_steps} steps")
-        v, cores, dt = cpu_baseline(args.cpu_n, args.cpu_steps, 24.0)
-        cpu = {"value": round(v, 2), "unit": "MLUPS", "cores": cores, "kind": "port",
-               "sample": f"oracle (unfused reference step sequence, FP32, OpenMP) on the same {args.cpu_n}^3 "
-                         f"sphere workload, {args.cpu_steps} timed steps after 1 warm-up ({dt:.1f} s)"}
-        log(f"CPU baseline {v:.1f} MLUPS on {cores} threads")
+        cpu = cpu_baseline(args.cpu_n, args.cpu_steps, 24.0)
+        log(f"CPU baseline {cpu['value']:.1f} MLUPS on {cpu['cores']} threads, {cpu['single_thread_mlups']:.2f} on one "
+            f"({cpu['host']}, {cpu['build']})")
 
     out = {
         "metric": METRIC,
@@ -565,6 +611,16 @@ def main():
         },
         "cpu_baseline": cpu,
     }
+    del g
+    subs = [x for x in args.sub_records.split(",") if x and x != "none"]
+    if "batched" in subs:
+        sub = argparse.Namespace(**vars(args))
+        sub.steps, sub.storage = args.batched_steps, "fp32"
+        out["batched"] = batched_record(sub)
+    if "serve" in subs:
+        sub = argparse.Namespace(**vars(args))
+        sub.step_scale = args.serve_step_scale
+        out["serve"] = serve_record(sub)
     print(json.dumps(out), flush=True)
 
 

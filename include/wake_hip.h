@@ -311,6 +311,16 @@ int wake_domain_clear_force_probe(wake_domain* d);
  * copied to out (out may be NULL with max_rows 0 to ask for the count). */
 int wake_domain_force_history(wake_domain* d, double* out, long long max_rows, long long* first_step,
                               long long* count);
+/* Divergence (no counterpart in the reference, whose scenarios throw
+ * NumericalDivergenceException, include/wake/core/error.hpp:14-18, after
+ * reading every density on the host).  *seen = 1 if a step since the state
+ * was last set produced a non-finite density (a flag the step kernels raise;
+ * one 4-byte read). */
+int wake_domain_nonfinite_seen(wake_domain* d, int* seen);
+/* *count = nodes whose current density or speed is not finite, *first = the
+ * first of them (-1 if none), counted on the device without downloading the
+ * fields. */
+int wake_domain_count_nonfinite(wake_domain* d, long long* count, long long* first);
 /* out: nx, ny, nz, Q, plane stride, current step, storage, device bytes */
 int wake_domain_info(wake_domain* d, long long out[8]);
 /* src/export/vtk_exporter.cpp:8-191: the current fields as a VTK XML

@@ -47,8 +47,10 @@ int oracle_real_bytes(void) { return (int)sizeof(real); }
 #ifdef _OPENMP
 #include <omp.h>
 int oracle_threads(void) { return omp_get_max_threads(); }
+void oracle_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
 #else
 int oracle_threads(void) { return 1; }
+void oracle_set_threads(int n) { (void)n; }
 #endif
 
 /* ------------------------------------------------------------------------

@@ -34,6 +34,7 @@ extern "C" {
 /* ---- tables ---- */
 int oracle_real_bytes(void);
 int oracle_threads(void); /* OpenMP threads the node loops use */
+void oracle_set_threads(int n); /* threads of the following calls (timing) */
 void oracle_d3q19_minv(real out[19 * 19]);
 void oracle_d3q19_m(real out[19 * 19]);
 uint64_t oracle_rng_splitmix64(uint64_t x);

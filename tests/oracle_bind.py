@@ -19,14 +19,17 @@ def build_oracle():
     subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
 
 
-def lib(bits=32):
+def lib(bits=32, path=None):
     """Load liboracle32.so (float), liboracle64.so (double) or, for bits =
-    "32fma", the float build with fused multiply-adds."""
+    "32fma", the float build with fused multiply-adds; `path` loads another
+    float build (the host-native one the CPU baseline times) under the key
+    `bits`."""
     if bits in _libs:
         return _libs[bits]
-    path = os.path.join(ORACLE_DIR, f"liboracle{bits}.so")
-    if not os.path.exists(path):
-        build_oracle()
+    if path is None:
+        path = os.path.join(ORACLE_DIR, f"liboracle{bits}.so")
+        if not os.path.exists(path):
+            build_oracle()
     L = ctypes.CDLL(path)
     real = ctypes.c_double if bits == 64 else ctypes.c_float
     P = ctypes.c_void_p
@@ -74,6 +77,7 @@ def lib(bits=32):
         fn.restype = ctypes.c_double
         fn.argtypes = [P]
     L.oracle_grid_momentum_exchange.argtypes = [P, real, real, real, real, P]
+    L.oracle_set_threads.argtypes = [I]
     L.oracle_rng_splitmix64.restype = ctypes.c_uint64
     L.oracle_rng_splitmix64.argtypes = [ctypes.c_uint64]
     L.oracle_rng_u01.restype = ctypes.c_double

@@ -108,6 +108,8 @@ DOMAIN_SIGS = {
     "wake_domain_set_force_probe": ([P, F, F, F, F], I),
     "wake_domain_clear_force_probe": ([P], I),
     "wake_domain_force_history": ([P, P, LL, P, P], I),
+    "wake_domain_nonfinite_seen": ([P, P], I),
+    "wake_domain_count_nonfinite": ([P, P, P], I),
     "wake_domain_info": ([P, P], I),
     "wake_domain_set_block_x": ([P, I], I),
     "wake_hip_last_error": ([], ctypes.c_char_p),

@@ -48,8 +48,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(StepWaves<L
       form_s_at<L, St>(a, in, x, y, z, n, c, true, cur, f);
     }
     if (!(c & kClsSolid)) {
-      if constexpr (LES) collide_les<L, COLL, St::kFast>(f, a.tau, a.cs, a.gx, a.gy, a.gz);
-      else collide<L, COLL, false, St::kFast>(f, a.tau, a.gx, a.gy, a.gz);
+      Macro m;
+      if constexpr (LES) m = collide_les<L, COLL, St::kFast>(f, a.tau, a.cs, a.gx, a.gy, a.gz);
+      else m = collide<L, COLL, false, St::kFast>(f, a.tau, a.gx, a.gy, a.gz);
+      flag_nonfinite(a, m.rho);
     }
     if constexpr (St::kBuf) {
       const auto r = St::rsrc(out, static_cast<unsigned>(L::Q * a.stride * sizeof(T)));

@@ -114,7 +114,12 @@ class Batch {
     shape_explicit_ = true;
     layout_dirty_ = true;
   }
-  LaunchShape shape() const { return shape_; }
+  // The shape the next launch uses: until one is set or tuned, the
+  // row-length default (resolved at launch time from the current members).
+  LaunchShape shape() const {
+    if (shape_explicit_) return shape_;
+    return LaunchShape{members_.empty() ? shape_.block : default_block(), shape_.grid_factor};
+  }
   int slots() const { return static_cast<int>(members_.size()); }
   int tiles() const { return ntiles_; }
   uint64_t nodes() const {
@@ -189,9 +194,13 @@ class Batch {
   LaunchShape tune(int probe) {
     if (pending_) throw HipError("a chunk of this batch is in flight: wait first");
     prepare();
+    // the row-length class (the width that wastes the fewest lanes on the
+    // members' rows) is part of the key: a shape tuned for long rows is not
+    // reused for a batch of short ones of similar total size
     const auto key = std::make_tuple(lattice_ + (thermal_ ? 100 : 0), storage_, collision_, les_,
                                      static_cast<int>(std::log2(static_cast<double>(std::max<uint64_t>(nodes(), 1)))),
-                                     static_cast<int>(std::log2(static_cast<double>(std::max(slots(), 1)))));
+                                     static_cast<int>(std::log2(static_cast<double>(std::max(slots(), 1)))),
+                                     default_block());
     {
       std::lock_guard<std::mutex> lk(cache_mutex());
       auto it = cache().find(key);
@@ -255,8 +264,8 @@ class Batch {
     return false;
   }
 
-  static std::map<std::tuple<int, int, int, bool, int, int>, LaunchShape>& cache() {
-    static std::map<std::tuple<int, int, int, bool, int, int>, LaunchShape> c;
+  static std::map<std::tuple<int, int, int, bool, int, int, int>, LaunchShape>& cache() {
+    static std::map<std::tuple<int, int, int, bool, int, int, int>, LaunchShape> c;
     return c;
   }
   static std::mutex& cache_mutex() {

@@ -262,6 +262,18 @@ int wake_domain_force_history(wake_domain* d, double* out, long long max_rows, l
     if (n > 0) std::memcpy(out, h.data(), static_cast<size_t>(n) * 3 * sizeof(double));
   });
 }
+int wake_domain_nonfinite_seen(wake_domain* d, int* seen) {
+  return guard(d, [&](wake_hip::Domain& g) {
+    if (!seen) throw wake_hip::HipError("nonfinite_seen: null output");
+    *seen = g.nonfinite_seen() ? 1 : 0;
+  });
+}
+int wake_domain_count_nonfinite(wake_domain* d, long long* count, long long* first) {
+  return guard(d, [&](wake_hip::Domain& g) {
+    if (!count) throw wake_hip::HipError("count_nonfinite: null output");
+    *count = g.count_nonfinite(first);
+  });
+}
 int wake_domain_info(wake_domain* d, long long out[8]) {
   return guard(d, [&](wake_hip::Domain& g) {
     out[0] = g.nx();

@@ -127,6 +127,8 @@ void Domain::allocate_and_initialize() {
   d_speed_ = static_cast<float*>(alloc(N_ * 4));
   d_partial_ = static_cast<double*>(alloc(3 * 1024 * sizeof(double)));
   d_out_ = static_cast<double*>(alloc(8 * sizeof(double)));
+  d_nonfinite_ = static_cast<int*>(alloc(sizeof(int)));
+  d_nfcount_ = static_cast<unsigned long long*>(alloc(2 * sizeof(unsigned long long)));
   flags_.assign(N_, kFluid);
   h_rho_.resize(N_);
   h_ux_.resize(N_);
@@ -232,6 +234,7 @@ void Domain::initialize(float rho, float ux, float uy, float uz) {
   fields_valid_ = true;
   fsrc_valid_ = fdst_valid_ = false;
   sout_is_fsrc_ = false;
+  clear_nonfinite();
   check(hipStreamSynchronize(stream_), "initialize");
   if (probe_) probe_restart();
 }
@@ -253,6 +256,7 @@ void Domain::upload_state_from_host(const std::vector<float>& f_ref) {
     }
   }
   check(hipMemcpyAsync(buf_[0], host.data(), host.size(), hipMemcpyHostToDevice, stream_), "upload");
+  clear_nonfinite();
   // convective outlet store = S_0 on the x = nx - 1 plane
   upload_prev([&](size_t row, int q) { return f_ref[static_cast<size_t>(q) * N_ + row * nx_ + (nx_ - 1)]; });
   check(hipStreamSynchronize(stream_), "upload");
@@ -654,6 +658,7 @@ StepArgs Domain::args(bool commit) const {
   a.gbeta = gbeta_;
   a.t_ref = t_ref_;
   a.t_wall = d_twall_;
+  a.nonfinite = d_nonfinite_;
   return a;
 }
 
@@ -1102,6 +1107,28 @@ void Domain::momentum_exchange(float ccx, float ccy, float ccz, float r_max, dou
   check(launch_mem_fold(d_mempart_, r.nslots, 1, 0, 1, d_out_, 0, stream_), "momentum exchange");
   check(hipMemcpyAsync(out, d_out_, 3 * sizeof(double), hipMemcpyDeviceToHost, stream_), "d2h");
   check(hipStreamSynchronize(stream_), "momentum exchange");
+}
+
+// -------------------------------------------------------------- divergence
+void Domain::clear_nonfinite() { check(hipMemsetAsync(d_nonfinite_, 0, sizeof(int), stream_), "clear flag"); }
+
+bool Domain::nonfinite_seen() {
+  int v = 0;
+  check(hipMemcpyAsync(&v, d_nonfinite_, sizeof(int), hipMemcpyDeviceToHost, stream_), "d2h");
+  check(hipStreamSynchronize(stream_), "divergence flag");
+  return v != 0;
+}
+
+long long Domain::count_nonfinite(long long* first) {
+  ensure_device_fields();
+  unsigned long long h[2] = {0ull, ~0ull};
+  check(hipMemcpyAsync(d_nfcount_, h, sizeof(h), hipMemcpyHostToDevice, stream_), "non-finite count");
+  check(launch_count_nonfinite(d_rho_, d_speed_, static_cast<long long>(N_), d_nfcount_, stream_),
+        "non-finite count");
+  check(hipMemcpyAsync(h, d_nfcount_, sizeof(h), hipMemcpyDeviceToHost, stream_), "d2h");
+  check(hipStreamSynchronize(stream_), "non-finite count");
+  if (first) *first = h[0] ? static_cast<long long>(h[1]) : -1;
+  return static_cast<long long>(h[0]);
 }
 
 // ------------------------------------------------------------- force probe

@@ -138,6 +138,12 @@ class Domain {
   bool force_probe() const { return probe_; }
   // Samples of steps first .. current_step(), 3 doubles each.
   const std::vector<double>& force_history(uint64_t* first_step);
+  // Divergence: whether a step since the state was last set produced a
+  // non-finite density (the flag the step kernels raise; one 4-byte read).
+  bool nonfinite_seen();
+  // Nodes whose current density or speed is not finite, counted on the
+  // device (no field download); *first = the first such node.
+  long long count_nonfinite(long long* first);
 
   // --- accessors
   int dim() const { return dim_; }
@@ -279,6 +285,9 @@ class Domain {
   float* d_sout_ = nullptr;
   double* d_partial_ = nullptr;
   double* d_out_ = nullptr;
+  int* d_nonfinite_ = nullptr;            // divergence flag of the step kernels
+  unsigned long long* d_nfcount_ = nullptr;  // count_nonfinite(): count, first node
+  void clear_nonfinite();
   std::vector<void*> allocations_;
   size_t device_bytes_ = 0;
   bool flags_dirty_ = true, cls_dirty_ = true, bfl_dirty_ = true;

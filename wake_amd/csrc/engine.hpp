@@ -85,6 +85,10 @@ struct StepArgs {
   double* mem_part;
   float mem_c[3];
   float mem_rmax;
+  // Divergence flag (null: off): a step launch sets *nonfinite = 1 when a
+  // node's density (or a thermal node's temperature) comes out of the
+  // collision non-finite.  Sticky until the state is re-initialised.
+  int* nonfinite;
 };
 
 // Partial slots of one force sample: rows with links x waves per row.
@@ -162,5 +166,10 @@ hipError_t launch_stream_walls_d2q9(const float* src, float* dst, const uint8_t*
 // FP64 sums over Fluid nodes of rho and rho |u|^2 / 2, and max |u|.
 hipError_t launch_diagnostics(const uint8_t* flags, const float* rho, const float* speed, long long n,
                               double* partial, int nblocks, double* out3, hipStream_t s);
+// Nodes (all cell types) whose density or speed is not finite: out[0] +=
+// their count, out[1] = min(out[1], first node index).  The caller sets
+// out = {0, ~0} first.
+hipError_t launch_count_nonfinite(const float* rho, const float* speed, long long n, unsigned long long* out,
+                                  hipStream_t s);
 
 }  // namespace wake_hip

@@ -21,10 +21,13 @@ auto guarded(F&& f) -> decltype(f()) {
   }
 }
 
-void finite_or_throw(const float* rho, const float* ux, std::size_t n) {
-  for (std::size_t i = 0; i < n; ++i)
-    if (!std::isfinite(rho[i]) || !std::isfinite(ux[i]))
-      throw NumericalDivergenceException("non-finite density or velocity at node " + std::to_string(i));
+// Counted on the device over the current density and speed (no download).
+void finite_or_throw(wake_hip::Domain& d) {
+  long long first = -1;
+  const long long n = guarded([&] { return d.count_nonfinite(&first); });
+  if (n > 0)
+    throw NumericalDivergenceException("non-finite density or velocity at node " + std::to_string(first) + " (" +
+                                       std::to_string(n) + " nodes)");
 }
 std::vector<Vec3> to_vec3(const std::vector<double>& rows) {
   std::vector<Vec3> v(rows.size() / 3);
@@ -144,7 +147,7 @@ void LatticeGrid3D::enable_force_history(real_t cx, real_t cy, real_t cz, real_t
 std::vector<Vec3> LatticeGrid3D::force_history(uint64_t* first_step) const {
   return guarded([&] { return to_vec3(domain_->force_history(first_step)); });
 }
-void LatticeGrid3D::check_finite() const { finite_or_throw(rho_data(), ux_data(), total_nodes()); }
+void LatticeGrid3D::check_finite() const { finite_or_throw(*domain_); }
 
 // ------------------------------------------------------------------ 2D
 LatticeGrid2D::LatticeGrid2D(int nx, int ny, real_t relaxation_time, PopulationStorage storage)
@@ -236,6 +239,6 @@ void LatticeGrid2D::enable_force_history(real_t cx, real_t cy, real_t r_max) {
 std::vector<Vec3> LatticeGrid2D::force_history(uint64_t* first_step) const {
   return guarded([&] { return to_vec3(domain_->force_history(first_step)); });
 }
-void LatticeGrid2D::check_finite() const { finite_or_throw(rho_data(), ux_data(), total_nodes()); }
+void LatticeGrid2D::check_finite() const { finite_or_throw(*domain_); }
 
 }  // namespace wake

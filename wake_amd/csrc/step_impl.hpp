@@ -154,6 +154,13 @@ using StF32Buf = StBuf<StF32>;
 using StF16Buf = StBuf<StF16>;
 using StBF16Buf = StBuf<StBF16>;
 
+// -------------------------------------------------------------- divergence
+// A non-finite density sets the domain's divergence flag (an idempotent
+// store; the branch is taken by no lane of a healthy flow).
+__device__ __forceinline__ void flag_nonfinite(const StepArgs& a, float rho) {
+  if (a.nonfinite && !__builtin_isfinite(rho)) *a.nonfinite = 1;
+}
+
 // ------------------------------------------------------------- RNG (inlet)
 // splitmix64 finaliser of seed ^ counter; top 24 bits as a uniform in [0,1).
 __device__ __forceinline__ float rng_sym(unsigned long long seed, unsigned long long counter) {
@@ -667,8 +674,10 @@ __global__ __launch_bounds__(256) WAKE_STEP_ATTR void k_step(const StepArgs a, c
     const int n = nn;
     if (k + 1 < WAKE_ZK && z + 1 < a.nz) fetch(z + 1, fn, cn, nn);
     if (!(c & kClsSolid)) {
-      if constexpr (LES) collide_les<L, COLL, St::kFast>(f, a.tau, a.cs, a.gx, a.gy, a.gz);
-      else collide<L, COLL, false, St::kFast>(f, a.tau, a.gx, a.gy, a.gz);
+      Macro m;
+      if constexpr (LES) m = collide_les<L, COLL, St::kFast>(f, a.tau, a.cs, a.gx, a.gy, a.gz);
+      else m = collide<L, COLL, false, St::kFast>(f, a.tau, a.gx, a.gy, a.gz);
+      flag_nonfinite(a, m.rho);
     }
     if constexpr (St::kBuf) {
       const auto r = St::rsrc(aout, static_cast<unsigned>(L::Q * S * sizeof(typename St::T)));

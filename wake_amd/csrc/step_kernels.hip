@@ -165,6 +165,25 @@ hipError_t launch_mark_bfl(const int* nodes, int count, uint8_t* cls, int* aux, 
   return hipGetLastError();
 }
 
+__global__ __launch_bounds__(256) void k_count_nonfinite(const float* __restrict__ rho,
+                                                         const float* __restrict__ speed, long long n,
+                                                         unsigned long long* __restrict__ out) {
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += 256ll * gridDim.x) {
+    if (!__builtin_isfinite(rho[i]) || !__builtin_isfinite(speed[i])) {
+      atomicAdd(&out[0], 1ull);
+      atomicMin(&out[1], static_cast<unsigned long long>(i));
+    }
+  }
+}
+
+hipError_t launch_count_nonfinite(const float* rho, const float* speed, long long n, unsigned long long* out,
+                                  hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const long long blocks = std::min<long long>(1024, (n + 255) / 256);
+  k_count_nonfinite<<<static_cast<int>(blocks), 256, 0, s>>>(rho, speed, n, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_diagnostics(const uint8_t* flags, const float* rho, const float* speed, long long n,
                               double* partial, int nblocks, double* out3, hipStream_t s) {
   k_diag_partial<<<nblocks, 256, 0, s>>>(flags, rho, speed, n, partial);

@@ -83,6 +83,7 @@ __device__ __forceinline__ void thermal_collide(const StepArgs& a, float (&f)[D2
     jx += static_cast<float>(D2Q9::cx[q]) * f[q];
     jy += static_cast<float>(D2Q9::cy[q]) * f[q];
   });
+  flag_nonfinite(a, rho + T);
   const float inv_rho = 1.0f / rho;
   const float ux = (jx + 0.5f * rho * a.gx) * inv_rho;
   const float uy = (jy + 0.5f * rho * gyn) * inv_rho;

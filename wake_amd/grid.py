@@ -237,6 +237,27 @@ class _Domain:
                 "momentum_exchange")
         return out
 
+    def nonfinite_seen(self):
+        """True if a step since the state was last set produced a non-finite
+        density (the step kernels' divergence flag)."""
+        v = ctypes.c_int(0)
+        self._c(self._L.wake_domain_nonfinite_seen(self._h, ctypes.byref(v)), "nonfinite_seen")
+        return bool(v.value)
+
+    def count_nonfinite(self):
+        """(count, first node) of nodes whose density or speed is not finite."""
+        n, first = ctypes.c_longlong(0), ctypes.c_longlong(-1)
+        self._c(self._L.wake_domain_count_nonfinite(self._h, ctypes.byref(n), ctypes.byref(first)),
+                "count_nonfinite")
+        return n.value, first.value
+
+    def check_finite(self):
+        """Raise FloatingPointError if a density or velocity is not finite
+        (the reference throws NumericalDivergenceException)."""
+        n, first = self.count_nonfinite()
+        if n:
+            raise FloatingPointError(f"non-finite density or velocity at {n} node(s), first at node {first}")
+
     def set_force_probe(self, cx, cy, cz, r_max):
         """Keep the momentum exchange of every step from now on (computed in
         the step launch itself); read it with force_history()."""

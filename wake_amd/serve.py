@@ -19,7 +19,15 @@ Admission order (`policy`):
     fill the GPU -- so short requests finish early instead of sharing the
     GPU with every long one (shortest-job-first; the mean and median time to
     result fall, the makespan does not grow);
+    A request that has waited longer than `max_wait_s` goes ahead of every
+    request that has not (oldest first), so a stream of short arrivals
+    cannot hold a long request back indefinitely;
   * "arrival": arrival order, everything that fits in the budget at once.
+
+A request whose flow diverges (the step kernels raise a per-domain flag
+when a density comes out of the collision non-finite) is retired at the next
+chunk boundary with an error result instead of running on with NaNs, and
+its slot and memory go to the queue.
 
 Results do not depend on the mix: a domain's arithmetic is the same alone
 and batched (tests/test_gpu_serve.py checks that).
@@ -154,11 +162,24 @@ def node_count(req):
     return req.n ** 3 if req.kind == "bluff3d" else req.n ** 2
 
 
+def admission_order(queue, now, policy="least_work", max_wait_s=60.0):
+    """The order in which queued requests are offered a place.  least_work:
+    requests waiting longer than max_wait_s first (by arrival), then the
+    rest by node-steps (then arrival); arrival: arrival order."""
+    order = list(queue)
+    if policy == "least_work":
+        order.sort(key=lambda r: (0, r.arrival, 0) if now - r.arrival > max_wait_s
+                   else (1, node_count(r) * r.steps, r.arrival))
+    else:
+        order.sort(key=lambda r: r.arrival)
+    return order
+
+
 class Server:
     """Continuous-batching request loop under an HBM admission budget."""
 
     def __init__(self, budget_bytes=200e9, chunk=100, use_graph=True, tune=True, policy="least_work",
-                 fill_nodes=4_000_000):
+                 fill_nodes=4_000_000, max_wait_s=60.0):
         if policy not in ("least_work", "arrival"):
             raise ValueError(f"unknown admission policy {policy!r}")
         self.budget = budget_bytes
@@ -167,6 +188,7 @@ class Server:
         self.tune = tune
         self.policy = policy
         self.fill_nodes = fill_nodes
+        self.max_wait_s = max_wait_s
         self.active_nodes = 0
         self.batches = {}
         self.active = {}        # rid -> (req, grid, key, slot)
@@ -174,6 +196,7 @@ class Server:
         self.used = 0
         self.done = []
         self.rejected = []      # requests that need more HBM than the whole budget
+        self.diverged = []      # ids of requests retired because their flow diverged
         self.peak_used = 0
         self.ticks = 0
         self.node_steps = 0
@@ -190,10 +213,7 @@ class Server:
 
     def _admit(self, now):
         kept = deque()
-        order = list(self.queue)
-        if self.policy == "least_work":
-            order.sort(key=lambda r: (node_count(r) * r.steps, r.arrival))
-        for req in order:
+        for req in admission_order(self.queue, now, self.policy, self.max_wait_s):
             need = bytes_needed(req)
             full = self.policy == "least_work" and self.active and self.active_nodes >= self.fill_nodes
             if full or self.used + need > self.budget:
@@ -232,8 +252,15 @@ class Server:
         for rid in list(self.active):
             req, g, key, slot = self.active[rid]
             b = self.batches[key]
-            if b.remaining(slot) == 0:
-                req.result = result_of(req, g)
+            # a request whose flow diverged (the step kernels flag a
+            # non-finite density) is retired with an error, freeing its slot
+            diverged = g.nonfinite_seen()
+            if diverged or b.remaining(slot) == 0:
+                if diverged:
+                    req.result = {"error": f"diverged: non-finite density by step {g.current_step()}"}
+                    self.diverged.append(req.rid)
+                else:
+                    req.result = result_of(req, g)
                 req.finished = t
                 b.remove(slot)
                 self.used -= bytes_needed(req)
@@ -271,7 +298,10 @@ def plain_loop(requests, realtime=True):
         r.admitted = now()
         g = make_domain(r, storage=Storage.F32)
         g.run(r.steps, "MRT", **step_kwargs(r))
-        r.result = result_of(r, g)
+        if g.nonfinite_seen():
+            r.result = {"error": f"diverged: non-finite density by step {g.current_step()}"}
+        else:
+            r.result = result_of(r, g)
         r.finished = now()
         out.append(r)
         del g
