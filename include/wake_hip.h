@@ -142,6 +142,22 @@ void d3q19_compute_momentum_exchange_forces(const float f_post_collision[], cons
                                             int ny, int nz, float center_x, float center_y, float center_z,
                                             float max_radius, float* fx_out, float* fy_out, float* fz_out);
 
+/* No counterpart in the reference: d3q19_compute_momentum_exchange_forces for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  All pointers are
+ * device pointers; forces_out receives fx, fy, fz (3 floats) in stream order
+ * on `stream` (a hipStream_t; NULL: the default stream); the call returns 0 at once, or an error code (message in
+ * wake_hip_last_error()) for null pointers, a non-positive size or more than
+ * 2^31 - 1 nodes.  The same kernels and sums as the export above, so the
+ * same bits.  The work-group partials live in a fixed 6 KiB device array
+ * (256 partials of 3 doubles, the number every call writes, held on each
+ * GPU until the process ends): calls that overlap on one GPU race.  The
+ * first call on a device also looks up that array's address (no
+ * allocation); make it outside any graph capture. */
+int d3q19_compute_momentum_exchange_forces_noalloc(const float* f_post_collision, const uint8_t* cell_type, int nx,
+                                                   int ny, int nz, float center_x, float center_y, float center_z,
+                                                   float max_radius, float* forces_out, void* stream);
+
 /* src/ispc/d3q27_collision_mrt.ispc:5 (TRT; pairs populations with the
  * reference's opposite table, include/wake/core/lattice_models.hpp:104-109) */
 void d3q27_collide(const float f_in[], float f_out[], const uint8_t cell_type[], float rho[], float ux[],

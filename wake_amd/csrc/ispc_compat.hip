@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -983,6 +984,62 @@ void ibm_spread_force_2d(const float marker_x[], const float marker_y[], const f
   k_ibm_spread_2d<<<1, 64>>>(c.dev(x), c.dev(y), c.dev(fx), c.dev(fy), c.dev(ds), c.dev(gx), c.dev(gy), nx, ny,
                              num_markers);
   c.end();
+}
+
+
+// ------------------------------------------------- allocation-free variant
+// d3q19_compute_momentum_exchange_forces on device pointers for callers that
+// cannot allocate (code under HIP graph capture, serving loops): the same
+// kernels, partition and fixed-order sums, so the same bits, with the
+// work-group partials in a file-scope device array instead of a staged
+// buffer.  The kernels always write kMemBlocks partials, whatever the grid
+// size, so kScratchPartials bounds every call.
+constexpr int kScratchPartials = kMemBlocks;
+static __device__ __align__(256) double scratch[3 * kScratchPartials];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The device address of `scratch` on the current device (each GPU holds its
+// own copy); looked up once per device with that device current.
+double* scratch_on_current_device(int* device) {
+  if (hipGetDevice(device) != hipSuccess || *device < 0 || *device >= kMaxScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[*device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+    g_scratch_ptr[*device] = static_cast<double*>(p);
+  }
+  return g_scratch_ptr[*device];
+}
+}  // namespace
+
+int d3q19_compute_momentum_exchange_forces_noalloc(const float* f_post_collision, const uint8_t* cell_type, int nx,
+                                                   int ny, int nz, float center_x, float center_y, float center_z,
+                                                   float max_radius, float* forces_out, void* stream_handle) {
+  static_assert(kMemBlocks <= kScratchPartials, "the partials of one call exceed the scratch array");
+  const char* name = __func__;
+  hipStream_t stream = static_cast<hipStream_t>(stream_handle);
+  auto fail = [&](const std::string& what) {
+    record_error(std::string("wake: ") + name + ": " + what);
+    return 1;
+  };
+  if (!f_post_collision || !cell_type || !forces_out) return fail("null device pointer");
+  if (nx <= 0 || ny <= 0 || nz <= 0) return fail("grid sizes must be positive");
+  const long long N = static_cast<long long>(nx) * ny * nz;
+  // the kernel's node coordinates are 32-bit
+  if (N > 0x7fffffffll) return fail("more than 2^31 - 1 nodes");
+  int device = -1;
+  double* part = scratch_on_current_device(&device);
+  if (!part) return fail("no scratch address on device " + std::to_string(device));
+  k_mem_grid<D3Q19><<<kMemBlocks, 256, 0, stream>>>(f_post_collision, cell_type, nx, ny, nz, center_x, center_y,
+                                                    center_z, max_radius, part);
+  k_sum_partials<<<1, 64, 0, stream>>>(part, kMemBlocks, forces_out);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(std::string("kernel launch: ") + hipGetErrorString(e));
+  return 0;
 }
 
 }  // extern "C"
