@@ -61,6 +61,54 @@ def test_kernel_exports_match_reference_names():
     assert set(names) == set(wl.KERNEL_SIGS)
 
 
+REF_ISPC = "/root/reference/src/ispc"
+
+
+def _reference_exports():
+    """name -> signature letters (p pointer, i int, f float) of every
+    `export void` in the reference's ISPC sources: arrays and `&` outputs are
+    pointers at the ABI level, `uniform int` / `uniform float` scalars."""
+    import glob
+
+    out = {}
+    for path in sorted(glob.glob(os.path.join(REF_ISPC, "*.ispc"))):
+        txt = re.sub(r"//[^\n]*", "", open(path).read())
+        for name, params in re.findall(r"export\s+void\s+(\w+)\s*\(([^)]*)\)", txt):
+            sig = ""
+            for prm in (x.strip() for x in params.split(",") if x.strip()):
+                if "[" in prm or "&" in prm or "*" in prm:
+                    sig += "p"
+                elif re.search(r"\bfloat\b", prm):
+                    sig += "f"
+                elif re.search(r"\b(int|int32)\b", prm):
+                    sig += "i"
+                else:
+                    raise AssertionError(f"unparsed parameter {prm!r} of {name}")
+            out[name] = sig
+    return out
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_ISPC), reason="the reference tree is not mounted here")
+def test_kernel_exports_match_reference_prototypes():
+    """Every ISPC export's parameter list (kinds and order) equals the one
+    this library binds and include/wake_hip.h declares."""
+    import wake_amd._lib as wl
+
+    ref = _reference_exports()
+    assert set(ref) == set(wl.KERNEL_SIGS), set(ref) ^ set(wl.KERNEL_SIGS)
+    bad = {n: (ref[n], wl.KERNEL_SIGS[n]) for n in ref if ref[n] != wl.KERNEL_SIGS[n]}
+    assert not bad, bad
+    # the header's parameter lists give the same kinds
+    txt = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    for name, sig in ref.items():
+        m = re.search(r"\bvoid\s+" + name + r"\s*\(([^)]*)\)", txt)
+        assert m, name
+        kinds = ""
+        for prm in (x.strip() for x in m.group(1).split(",")):
+            kinds += "p" if ("[" in prm or "*" in prm) else ("f" if prm.startswith("float") else "i")
+        assert kinds == sig, (name, kinds, sig)
+
+
 def test_compiled_mrt_table_equals_oracle():
     L = ctypes.CDLL(LIB)
     dev = np.zeros(361, np.float32)
