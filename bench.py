@@ -228,7 +228,7 @@ def batched_record(args):
     left = max(b.remaining(s) for s in slots)
     log(f"autotuned launch shape block={shape[0]} grid_factor={shape[1]} in {tune_s:.2f}s; {left} steps left")
     t0 = time.perf_counter()
-    ms = b.run(left)
+    ms = b.run(left, use_graph=not getattr(args, "no_graph", False))
     wall = time.perf_counter() - t0
     mlups = nodes * left / wall / 1e6
     log(f"batched: {left} steps in {wall:.2f} s wall ({ms / 1e3:.2f} s device), {mlups:.0f} MLUPS")
@@ -376,6 +376,12 @@ def main_wing(args):
                            "kernel_gbps": round(bpn * nx * ny * nz / (kernel_ms * 1e-3) / 1e9, 1)}
         log(f"full {nx}x{ny}x{nz} {names[st]}: {perf[names[st]]}")
         del g
+    from wake_amd.hbm_probe import ceilings_d3q27
+    ceil = ceilings_d3q27(nx, ny, nz)
+    for nm, key in (("fp32", "pull_d3q27_gbps"), ("fp16", "pull_d3q27_16bit_gbps"), ("bf16", "pull_d3q27_16bit_gbps")):
+        perf[nm]["frac_of_pull_same_width"] = round(perf[nm]["kernel_gbps"] / ceil[key], 4)
+    log(f"D3Q27 pull copy on this GPU: {ceil['pull_d3q27_gbps']:.0f} GB/s (FP32), "
+        f"{ceil['pull_d3q27_16bit_gbps']:.0f} GB/s (16-bit)")
     out = {
         "metric": METRIC, "value": perf["fp32"]["MLUPS"], "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
         "warmup": 10, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -384,6 +390,7 @@ def main_wing(args):
                                f"{nx}x{ny}x{nz}, inlet u=0.04, Re=2000",
                    "storage_perf": perf,
                    "bytes_per_node": {"fp32": 216, "fp16": 108, "bf16": 108},
+                   "pull_ceiling_gbps": {k: round(v, 1) for k, v in ceil.items()},
                    "error_study": "tests/precision_wing.py (FP64 oracle at 1/4 size)",
                    "parallelism": "single GPU"},
     }
@@ -530,6 +537,7 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--domains", type=int, default=512)
     ap.add_argument("--plain-probe", type=int, default=20)
+    ap.add_argument("--no-graph", action="store_true", help="batched-wakes: launch each step directly (no HIP graph)")
     ap.add_argument("--requests", type=int, default=40)
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--step-scale", type=float, default=1.0)

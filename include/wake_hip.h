@@ -57,8 +57,11 @@ void wake_hip_d3q19_minv(float out[361]);
  *            x - c_q, periodic), the FP32 step's memory access without
  *            arithmetic (152 bytes per node);
  *   shape 2: the same pull with 16-bit values, the FP16 / BF16 step's
- *            access (76 bytes per node).
- * Allocates and frees two buffers of 76 * nx * ny * nz bytes.  Needs
+ *            access (76 bytes per node);
+ *   shapes 3, 4: the D3Q27 pull over 27 planes, floats (216 bytes per node)
+ *            and 16-bit values (108 bytes per node).
+ * Allocates and frees two buffers of 76 (shapes 3, 4: 108) * nx * ny * nz
+ * bytes.  Needs
  * 1 <= nx <= 1024 and 1 <= nz <= 65535; returns 0 or an error code. */
 int wake_hip_hbm_probe(int shape, int nx, int ny, int nz, int reps, float* ms_per_launch);
 

@@ -178,3 +178,28 @@ def test_batch_3d_les_bfl_matches_plain(wa, storage):
         # FP16 storage rounds each population to the format, so a last-bit
         # difference in FP32 can move a stored value by one FP16 step
         assert rel_l2(g.f_src_data(), p.f_src_data()) <= (1e-6 if storage == 0 else 1e-5)
+
+
+def test_forces_member_without_links_records_zeros(wa):
+    """A member with forces but no wall link (no body, or r_max too small)
+    records zero forces, also after an earlier member left the batch and the
+    slots moved."""
+    batch = wa.Batch(lattice=9, collision="MRT", chunk=10)
+    g0, (cx, cy, D) = cylinder_channel(wa, 128, 1)
+    s0 = batch.add(g0, 40, forces=(cx, cy, 0.0, 1.5 * D))
+    empty = wa.LatticeGrid2D(96, 24, 0.7)
+    empty.initialize(1.0, 0.03, 0.0)
+    s1 = batch.add(empty, 40, forces=(48.0, 12.0, 0.0, 5.0))
+    g2, (cx2, cy2, D2) = cylinder_channel(wa, 160, -2)
+    s2 = batch.add(g2, 40, forces=(cx2, cy2, 0.0, 0.1))  # r_max below every link midpoint
+    # the default width is the one wasting the fewest lanes over the members'
+    # rows (128x32, 96x24, 160x40: 64), reported before the first launch
+    assert batch.info()["block"] == 64
+    batch.run(15)
+    assert np.all(batch.forces(s1) == 0.0) and np.all(batch.forces(s2) == 0.0)
+    assert batch.forces(s0).shape == (15, 3) and batch.forces(s0)[-1, 0] != 0.0
+    batch.remove(s0)
+    batch.run(25)
+    F1, F2 = batch.forces(s1), batch.forces(s2)
+    assert F1.shape == (40, 3) and F2.shape == (40, 3)
+    assert np.all(F1 == 0.0) and np.all(F2 == 0.0)

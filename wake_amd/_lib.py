@@ -9,7 +9,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libwake_hip.so")
+# WAKE_HIP_LIB selects another build of the library (a code-generation
+# variant of the same sources, for A/B timing); the default is the in-tree one.
+LIB_PATH = os.environ.get("WAKE_HIP_LIB") or os.path.join(HERE, "libwake_hip.so")
 
 _lib = None
 

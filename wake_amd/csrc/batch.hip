@@ -71,10 +71,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(StepWaves<L
 }
 
 // Momentum exchange on S_{cur+1} = gather of the buffer step i just wrote,
-// one work-group per member: the lanes walk the member's linked rows and form
-// S only at the Fluid nodes next to a wall; lane sums in a fixed order, then
-// a fixed tree over the work-group.  A member with forces but no link
-// records zeros.
+// one work-group per member: the lanes take the member's nodes holding links
+// (a device list in node order) in turn, form S there and add the links;
+// lane sums in a fixed order, then a fixed tree over the work-group.  A
+// member with forces but no link records zeros.
 template <class L, class St>
 __global__ __launch_bounds__(256) void k_mem_batch(const BatchSlot* __restrict__ slots, int i, int chunk,
                                                    double* __restrict__ hist) {
@@ -85,17 +85,14 @@ __global__ __launch_bounds__(256) void k_mem_batch(const BatchSlot* __restrict__
   const int par = ((sl.parity + i) & 1) ^ 1;
   const auto* buf = static_cast<const typename St::T*>(sl.buf[par]);
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  for (int r = 0; r < sl.mem_nrows; ++r) {
-    const int row = sl.mem_list[r];
-    const int y = row % a.ny, z = row / a.ny;
-    for (int x = threadIdx.x; x < a.nx; x += blockDim.x) {
-      const int n = row * a.nx + x;
-      const uint8_t c = a.cls[n];
-      if (!(c & kClsSlow) || a.flags[n] != kFluid) continue;
-      float f[L::Q];
-      form_s_at<L, St, false>(a, buf, x, y, z, n, c, false, cur + 1, f);
-      mem_node<L>(a, x, y, z, f, a.mem_c[0], a.mem_c[1], a.mem_c[2], a.mem_rmax, s0, s1, s2);
-    }
+  for (int i = threadIdx.x; i < sl.mem_count; i += blockDim.x) {
+    const int n = sl.mem_nodes[i];
+    const int x = n % a.nx;
+    const int y = (n / a.nx) % a.ny;
+    const int z = n / (a.nx * a.ny);
+    float f[L::Q];
+    form_s_at<L, St, false>(a, buf, x, y, z, n, a.cls[n], false, cur + 1, f);
+    mem_node<L>(a, x, y, z, f, a.mem_c[0], a.mem_c[1], a.mem_c[2], a.mem_rmax, s0, s1, s2);
   }
   block_sum3(s0, s1, s2);
   if (threadIdx.x == 0) {

@@ -31,9 +31,9 @@ struct BatchSlot {
   int nodes;                  // nx * ny * nz
   int tile0, ntiles;          // tiles [tile0, tile0 + ntiles) belong to this slot
   int tiles_per_row;          // ceil(nx / block)
-  // momentum exchange (a.mem_c, a.mem_rmax): the member's rows holding links
-  const int* mem_list;
-  int mem_nrows;
+  // momentum exchange (a.mem_c, a.mem_rmax): the member's nodes holding links
+  const int* mem_nodes;
+  int mem_count;
   int mem_on;                 // forces recorded for this member
 };
 

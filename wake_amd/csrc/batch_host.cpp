@@ -357,7 +357,7 @@ class Batch {
       b.ntiles = m.d->ny() * m.d->nz() * b.tiles_per_row;
       tile_slot.insert(tile_slot.end(), b.ntiles, s);
       if (m.forces) {
-        b.mem_list = m.d->batch_mem_rows(m.c[0], m.c[1], m.c[2], m.r_max, &b.mem_nrows);
+        b.mem_nodes = m.d->batch_mem_nodes(m.c[0], m.c[1], m.c[2], m.r_max, &b.mem_count);
         b.mem_on = 1;
         b.a.mem_c[0] = m.c[0];
         b.a.mem_c[1] = m.c[1];

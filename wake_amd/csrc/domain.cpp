@@ -1319,16 +1319,47 @@ void Domain::batch_advance(uint64_t k, int parity_after, float gx, float gy, flo
   invalidate_mirrors();
 }
 
-const int* Domain::batch_mem_rows(float ccx, float ccy, float ccz, float r_max, int* nrows) {
+const int* Domain::batch_mem_nodes(float ccx, float ccy, float ccz, float r_max, int* count) {
   sync_device_config();
   const auto key = std::make_tuple(ccx, ccy, ccz, r_max, cls_epoch_);
   if (key != batch_rows_key_ || !batch_rows_.d_list) {
     if (batch_rows_.d_rows) free_mem_rows(batch_rows_);
+    if (d_batch_nodes_) {
+      check(hipStreamSynchronize(stream_), "batch nodes");
+      release(d_batch_nodes_);
+      d_batch_nodes_ = nullptr;
+    }
     batch_rows_ = build_mem_rows(ccx, ccy, ccz, r_max);
     batch_rows_key_ = key;
+    n_batch_nodes_ = 0;
+    const int nr = batch_rows_.nrows;
+    if (nr > 0) {
+      StepArgs a = args(false);
+      a.mem_c[0] = ccx;
+      a.mem_c[1] = ccy;
+      a.mem_c[2] = ccz;
+      a.mem_rmax = r_max;
+      // per-row counts, their prefix on the host, then the nodes
+      int* d_cnt = static_cast<int*>(alloc(2 * static_cast<size_t>(nr) * sizeof(int)));
+      check(launch_mem_row_nodes(lattice_, a, batch_rows_.d_list, nr, d_cnt, nullptr, nullptr, stream_),
+            "batch nodes");
+      std::vector<int> cnt(nr), off(nr);
+      check(hipMemcpyAsync(cnt.data(), d_cnt, nr * sizeof(int), hipMemcpyDeviceToHost, stream_), "batch nodes");
+      check(hipStreamSynchronize(stream_), "batch nodes");
+      for (int r = 0; r < nr; ++r) {
+        off[r] = n_batch_nodes_;
+        n_batch_nodes_ += cnt[r];
+      }
+      d_batch_nodes_ = static_cast<int*>(alloc(std::max(1, n_batch_nodes_) * sizeof(int)));
+      check(hipMemcpyAsync(d_cnt + nr, off.data(), nr * sizeof(int), hipMemcpyHostToDevice, stream_), "batch nodes");
+      check(launch_mem_row_nodes(lattice_, a, batch_rows_.d_list, nr, d_cnt, d_cnt + nr, d_batch_nodes_, stream_),
+            "batch nodes");
+      check(hipStreamSynchronize(stream_), "batch nodes");
+      release(d_cnt);
+    }
   }
-  *nrows = batch_rows_.nrows;
-  return batch_rows_.d_list;
+  *count = n_batch_nodes_;
+  return d_batch_nodes_;
 }
 
 size_t Domain::bytes_needed(int dim, int lattice, int nx, int ny, int nz, int storage, long long plane_pad,

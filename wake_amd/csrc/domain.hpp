@@ -192,10 +192,10 @@ class Domain {
   // A batch took k >= 1 steps; the state is now post-collision in buffer
   // `parity_after`.
   void batch_advance(uint64_t k, int parity_after, float gx, float gy, float gz);
-  // The rows holding the momentum-exchange links around (cx, cy, cz) within
-  // r_max: a device list the domain owns (valid until the next call or a
-  // geometry change), *nrows entries.
-  const int* batch_mem_rows(float cx, float cy, float cz, float r_max, int* nrows);
+  // The nodes holding momentum-exchange links around (cx, cy, cz) within
+  // r_max, in node order: a device list the domain owns (valid until the
+  // next call or a geometry change), *count entries.
+  const int* batch_mem_nodes(float cx, float cy, float cz, float r_max, int* count);
   // Device bytes a domain of this shape allocates.
   static size_t bytes_needed(int dim, int lattice, int nx, int ny, int nz, int storage, long long plane_pad = -1,
                              bool thermal = false);
@@ -294,7 +294,9 @@ class Domain {
   uint64_t flags_version_ = 0;
   uint64_t cls_epoch_ = 0;  // counts rebuilds of the class bytes (geometry or boundary changes)
   std::map<std::tuple<float, float, float, float, uint64_t>, MemRows> mem_rows_cache_;
-  MemRows batch_rows_;  // the rows of batch_mem_rows(), with their key
+  MemRows batch_rows_;  // the rows of batch_mem_nodes(), with their key
+  int* d_batch_nodes_ = nullptr;
+  int n_batch_nodes_ = 0;
   std::tuple<float, float, float, float, uint64_t> batch_rows_key_{0.0f, 0.0f, 0.0f, 0.0f, ~0ull};
   double* d_mempart_ = nullptr;  // partials of one momentum_exchange() call
   size_t mempart_cap_ = 0;       // in doubles

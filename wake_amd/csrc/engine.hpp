@@ -124,6 +124,11 @@ hipError_t launch_readback(int lattice, int storage, bool from_s, const StepArgs
 // row_has[z * ny + y] = 1 for rows holding a link (row_has cleared by the
 // caller).
 hipError_t launch_mem_mark_rows(int lattice, const StepArgs& a, int* row_has, hipStream_t s);
+// The nodes holding links of (a.mem_c, a.mem_rmax) in the linked rows
+// row_list[0 .. nrows): nodes == null: counts[r] = the count of row r; else
+// those nodes, in node order, written from nodes[offsets[r]] on.
+hipError_t launch_mem_row_nodes(int lattice, const StepArgs& a, const int* row_list, int nrows, int* counts,
+                                const int* offsets, int* nodes, hipStream_t s);
 // The per-wave partials of a.mem_rows / a.mem_part for one S_k without a
 // step: over the `nrows` linked rows listed in row_list, from the
 // materialised S state (from_s) or formed from the post-collision buffer.

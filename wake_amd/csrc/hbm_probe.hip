@@ -11,7 +11,8 @@
 // * the same pull with 16-bit values: the FP16 / BF16 step's access.
 //
 // The first two move 19 * 4 bytes in and out per node, the FP32 step's
-// 152 B/node; the third 19 * 2, the 16-bit step's 76 B/node.
+// 152 B/node; the third 19 * 2, the 16-bit step's 76 B/node.  Shapes 3 and 4
+// are the D3Q27 pull over 27 planes (216 and 108 B/node).
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -44,10 +45,9 @@ __global__ __launch_bounds__(kCopyBlock) void k_probe_stream(const float* __rest
 
 // One work-group per (row y, z), one thread per x; nx <= 1024.  T: float
 // (the FP32 step's access) or a 16-bit value (the FP16 / BF16 step's).
-template <class T>
-__global__ __launch_bounds__(1024) void k_probe_pull19(const T* __restrict__ a, T* __restrict__ b, int nx, int ny,
-                                                       int nz) {
-  using L = D3Q19;
+template <class L, class T>
+__global__ __launch_bounds__(1024) void k_probe_pull(const T* __restrict__ a, T* __restrict__ b, int nx, int ny,
+                                                     int nz) {
   const int x = threadIdx.x, y = blockIdx.x, z = blockIdx.y;
   if (x >= nx) return;
   const long long plane = static_cast<long long>(nx) * ny * nz;
@@ -66,8 +66,8 @@ __global__ __launch_bounds__(1024) void k_probe_pull19(const T* __restrict__ a, 
 }
 
 hipError_t run_probe(int shape, int nx, int ny, int nz, int reps, float* ms_out) {
-  const long long count = 19LL * nx * ny * nz;
-  float* a = nullptr;  // count floats; shape 2 uses the first count halves
+  const long long count = (shape >= 3 ? 27LL : 19LL) * nx * ny * nz;
+  float* a = nullptr;  // count floats; the 16-bit shapes use the first count halves
   float* b = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipError_t err = hipMalloc(&a, count * sizeof(float));
@@ -80,10 +80,15 @@ hipError_t run_probe(int shape, int nx, int ny, int nz, int reps, float* ms_out)
       const long long per = kCopyBlock * kCopyPerThread;
       k_probe_stream<<<static_cast<unsigned>((count + per - 1) / per), kCopyBlock>>>(a, b, count);
     } else if (shape == 1) {
-      k_probe_pull19<float><<<dim3(ny, nz), nx>>>(a, b, nx, ny, nz);
+      k_probe_pull<D3Q19, float><<<dim3(ny, nz), nx>>>(a, b, nx, ny, nz);
+    } else if (shape == 2) {
+      k_probe_pull<D3Q19, unsigned short><<<dim3(ny, nz), nx>>>(reinterpret_cast<const unsigned short*>(a),
+                                                                reinterpret_cast<unsigned short*>(b), nx, ny, nz);
+    } else if (shape == 3) {
+      k_probe_pull<D3Q27, float><<<dim3(ny, nz), nx>>>(a, b, nx, ny, nz);
     } else {
-      k_probe_pull19<unsigned short><<<dim3(ny, nz), nx>>>(reinterpret_cast<const unsigned short*>(a),
-                                                           reinterpret_cast<unsigned short*>(b), nx, ny, nz);
+      k_probe_pull<D3Q27, unsigned short><<<dim3(ny, nz), nx>>>(reinterpret_cast<const unsigned short*>(a),
+                                                                reinterpret_cast<unsigned short*>(b), nx, ny, nz);
     }
   };
   if (err == hipSuccess) {
@@ -112,7 +117,7 @@ hipError_t run_probe(int shape, int nx, int ny, int nz, int reps, float* ms_out)
 
 extern "C" int wake_hip_hbm_probe(int shape, int nx, int ny, int nz, int reps, float* ms_per_launch) {
   if (!ms_per_launch) return wake_hip::set_last_error("wake_hip_hbm_probe: null output");
-  if (shape < 0 || shape > 2) return wake_hip::set_last_error("wake_hip_hbm_probe: shape must be 0, 1 or 2");
+  if (shape < 0 || shape > 4) return wake_hip::set_last_error("wake_hip_hbm_probe: shape must be 0 to 4");
   if (nx < 1 || ny < 1 || nz < 1 || nx > 1024 || reps < 1)
     return wake_hip::set_last_error("wake_hip_hbm_probe: need 1 <= nx <= 1024, ny, nz >= 1 and reps >= 1");
   if (nz > 65535) return wake_hip::set_last_error("wake_hip_hbm_probe: nz above the 65,535 grid limit");

@@ -450,11 +450,169 @@ __device__ __forceinline__ Macro collide_mrt_d3q19(float (&f)[19], float tau_she
   return {rho, ux, uy, uz};
 }
 
+// --------------------------------------------------------------------------
+// D3Q27 TRT for 16-bit population storage.  The storage rounds f - w_q to 11
+// (FP16) or 8 (BF16) significant bits, far above the rounding of any order
+// of FP32 operations, so this path is free to reorganise the arithmetic for
+// issue count (the 16-bit kernels are bound by VALU issue, not HBM):
+//  * the moments rho, j and sum c_a c_b f come from summing the 27
+//    populations along x, then y, then z (66 additions, against about 160
+//    for the separate sums);
+//  * each opposite pair (q, o) relaxes through its sum s and difference d:
+//      f_q' = f_q - D_s - D_a,  f_o' = f_o - D_s + D_a,
+//      D_s = omega_s (s / 2 - w rho (1 - 1.5 u^2 + 4.5 (c u)^2)),
+//      D_a = omega_a (d / 2 - 3 w rho c u),
+//    which is f - omega_s (f+ - feq+) - omega_a (f- - feq-) with the
+//    symmetric and antisymmetric equilibria formed once per pair;
+//  * the Guo force terms are compiled out when the body force is zero.
+
+// Index of velocity (a, b, c) in lattice L, or -1.
+template <class L>
+constexpr int q_of(int a, int b, int c) {
+  for (int q = 0; q < L::Q; ++q)
+    if (L::cx[q] == a && L::cy[q] == b && L::cz[q] == c) return q;
+  return -1;
+}
+
+struct Moments2 {
+  float rho, jx, jy, jz;        // sum f, sum c f
+  float sxx, syy, szz;          // sum c_a^2 f
+  float sxy, syz, sxz;          // sum c_a c_b f
+};
+
+template <class L>
+__device__ __forceinline__ Moments2 moments_d3q27(const float (&f)[L::Q]) {
+  static_assert(L::Q == 27, "the x-y-z cascade needs every velocity of {-1,0,1}^3");
+  // x lines (y = b, z = c): orders 0, 1, 2 in c_x
+  float k0[3][3], k1[3][3], k2[3][3];
+  static_for<9>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    constexpr int b = i / 3 - 1, c = i % 3 - 1;
+    constexpr int qm = q_of<L>(-1, b, c), q0 = q_of<L>(0, b, c), qp = q_of<L>(1, b, c);
+    static_assert(qm >= 0 && q0 >= 0 && qp >= 0, "missing velocity");
+    const float t = f[qp] + f[qm];
+    k2[b + 1][c + 1] = t;
+    k1[b + 1][c + 1] = f[qp] - f[qm];
+    k0[b + 1][c + 1] = t + f[q0];
+  });
+  // y: for each z, orders (x0 y0, x0 y1, x0 y2, x1 y0, x1 y1, x2 y0)
+  float a0[3], a1[3], a2[3], b0[3], b1[3], c0[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float t = k0[2][c] + k0[0][c];
+    a2[c] = t;
+    a0[c] = t + k0[1][c];
+    a1[c] = k0[2][c] - k0[0][c];
+    b0[c] = (k1[2][c] + k1[0][c]) + k1[1][c];
+    b1[c] = k1[2][c] - k1[0][c];
+    c0[c] = (k2[2][c] + k2[0][c]) + k2[1][c];
+  }
+  // z
+  Moments2 m;
+  const float t = a0[2] + a0[0];
+  m.szz = t;
+  m.rho = t + a0[1];
+  m.jz = a0[2] - a0[0];
+  m.jy = (a1[2] + a1[0]) + a1[1];
+  m.syz = a1[2] - a1[0];
+  m.syy = (a2[2] + a2[0]) + a2[1];
+  m.jx = (b0[2] + b0[0]) + b0[1];
+  m.sxz = b0[2] - b0[0];
+  m.sxy = (b1[2] + b1[0]) + b1[1];
+  m.sxx = (c0[2] + c0[0]) + c0[1];
+  return m;
+}
+
+// Smagorinsky tau from the second moments (as smagorinsky_tau_moments).
+__device__ __forceinline__ float smagorinsky_tau_m2(const Moments2& m, float ux, float uy, float uz, float inv_rho,
+                                                    float tau0, float cs) {
+  const float rho = m.rho;
+  const float r3 = rho * (1.0f / 3.0f);
+  const float pxx = m.sxx - (r3 + rho * ux * ux);
+  const float pyy = m.syy - (r3 + rho * uy * uy);
+  const float pzz = m.szz - (r3 + rho * uz * uz);
+  const float pxy = m.sxy - rho * ux * uy;
+  const float pyz = m.syz - rho * uy * uz;
+  const float pxz = m.sxz - rho * ux * uz;
+  const float tensor_sum = (pxx * pxx + pyy * pyy + pzz * pzz) + 2.0f * (pxy * pxy + pyz * pyz + pxz * pxz);
+  const float q_stress = __builtin_amdgcn_sqrtf(2.0f * tensor_sum);
+  const float prefactor = 18.0f * 1.4142135623730951f * (cs * cs);
+  const float disc = tau0 * tau0 + (prefactor * inv_rho) * q_stress;
+  return 0.5f * (tau0 + __builtin_amdgcn_sqrtf(disc));
+}
+
+template <class L, bool FORCE>
+__device__ __forceinline__ void trt_pairs(float (&f)[L::Q], float rho, float ux, float uy, float uz, float omega_s,
+                                          float omega_a, float gx, float gy, float gz) {
+  const float usq = ux * ux + uy * uy + uz * uz;
+  const float K = 1.0f - 1.5f * usq;
+  const float hs = 0.5f * omega_s, ha = 0.5f * omega_a;
+  float pre_s = 0.0f, pre_a = 0.0f, ug = 0.0f;
+  if constexpr (FORCE) {
+    pre_s = 1.0f - 0.5f * omega_s;
+    pre_a = 1.0f - 0.5f * omega_a;
+    ug = ux * gx + uy * gy + uz * gz;
+  }
+  {
+    const float wr = L::w[0] * rho;
+    float v = f[0] - omega_s * (f[0] - wr * K);
+    if constexpr (FORCE) v += pre_s * wr * (-3.0f * ug);
+    f[0] = v;
+  }
+  static_for<L::Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int o = L::opp[q];
+    if constexpr (q > 0 && q < o) {
+      const float wr = L::w[q] * rho;
+      const float cu = cdot<L, q>(ux, uy, uz);
+      const float s = f[q] + f[o];
+      const float d = f[q] - f[o];
+      const float esym = wr * ((4.5f * cu) * cu + K);
+      const float easym = (3.0f * wr) * cu;
+      const float Ds = hs * s - omega_s * esym;
+      const float Da = ha * d - omega_a * easym;
+      float fq = f[q] - Ds - Da;
+      float fo = f[o] - Ds + Da;
+      if constexpr (FORCE) {
+        const float cg = cdot<L, q>(gx, gy, gz);
+        const float Fa = pre_a * (3.0f * wr) * cg;
+        const float Fs = pre_s * wr * (9.0f * cu * cg - 3.0f * ug);
+        fq += Fs + Fa;
+        fo += Fs - Fa;
+      }
+      f[q] = fq;
+      f[o] = fo;
+    }
+  });
+}
+
+// TRT (Lambda = 1/4 pairing of tau_a) with the shear tau from the Smagorinsky
+// closure (LES) or given.
+template <class L, bool LES>
+__device__ __forceinline__ Macro collide_trt_fast27(float (&f)[L::Q], float tau, float cs, float gx, float gy,
+                                                    float gz) {
+  const Moments2 m = moments_d3q27<L>(f);
+  const float inv_rho = __builtin_amdgcn_rcpf(m.rho);
+  const float ux = (m.jx + 0.5f * m.rho * gx) * inv_rho;
+  const float uy = (m.jy + 0.5f * m.rho * gy) * inv_rho;
+  const float uz = (m.jz + 0.5f * m.rho * gz) * inv_rho;
+  const float tau_s = LES ? smagorinsky_tau_m2(m, ux, uy, uz, inv_rho, tau, cs) : tau;
+  const float omega_s = __builtin_amdgcn_rcpf(tau_s);
+  const float omega_a = __builtin_amdgcn_rcpf(trt_tau_a(tau_s));
+  if (gx == 0.0f && gy == 0.0f && gz == 0.0f)
+    trt_pairs<L, false>(f, m.rho, ux, uy, uz, omega_s, omega_a, gx, gy, gz);
+  else
+    trt_pairs<L, true>(f, m.rho, ux, uy, uz, omega_s, omega_a, gx, gy, gz);
+  return {m.rho, ux, uy, uz};
+}
+
 // Dispatch one collision on a node.  tau_eff is the shear tau (the scalar tau
 // of the step, or the per-node Smagorinsky value in LES-coupled mode).
 template <class L, int COLL, bool REF_PAIRS = false, bool FAST = false>
 __device__ __forceinline__ Macro collide(float (&f)[L::Q], float tau_eff, float gx, float gy, float gz) {
-  if constexpr (COLL == kBGK) {
+  if constexpr (L::Q == 27 && COLL != kBGK && FAST && !REF_PAIRS) {
+    return collide_trt_fast27<L, false>(f, tau_eff, 0.0f, gx, gy, gz);
+  } else if constexpr (COLL == kBGK) {
     return collide_bgk<L, FAST>(f, tau_eff, gx, gy, gz);
   } else if constexpr (COLL == kTRT || L::Q == 27) {
     return collide_trt<L, REF_PAIRS, FAST>(f, tau_eff, trt_tau_a(tau_eff), gx, gy, gz);
@@ -469,7 +627,9 @@ __device__ __forceinline__ Macro collide(float (&f)[L::Q], float tau_eff, float 
 // stress from its moments; BGK and TRT form it from feq.
 template <class L, int COLL, bool FAST = false>
 __device__ __forceinline__ Macro collide_les(float (&f)[L::Q], float tau0, float cs, float gx, float gy, float gz) {
-  if constexpr (COLL == kMRT && L::Q == 19) {
+  if constexpr (L::Q == 27 && COLL != kBGK && FAST) {
+    return collide_trt_fast27<L, true>(f, tau0, cs, gx, gy, gz);
+  } else if constexpr (COLL == kMRT && L::Q == 19) {
     return collide_mrt_d3q19<true, FAST>(f, tau0, kTauBulk3D, gx, gy, gz, cs);
   } else if constexpr (COLL == kMRT && L::Q == 9) {
     return collide_mrt_d2q9<true, FAST>(f, tau0, kTauBulk2D, gx, gy, cs);

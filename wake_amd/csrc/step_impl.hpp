@@ -73,6 +73,21 @@ struct StF32 {
   __device__ __forceinline__ static void st(T* __restrict__ p, long long i, float v) { st_pop(p, i, v); }
 };
 
+// FP16 <-> FP32 with the weight offset in one mixed-precision FMA each:
+// w + (float)h (exact conversion, one rounding: the same value as a
+// conversion followed by an add) and (_Float16)(v - w) (one rounding of the
+// exact difference).  h holds FP16 bits in its low half.
+__device__ __forceinline__ float f16_bits_plus(unsigned h, float w) {
+  float r;
+  asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "s"(w));
+  return r;
+}
+__device__ __forceinline__ unsigned short f16_bits_of_minus(float v, float w) {
+  unsigned r;
+  asm("v_fma_mixlo_f16 %0, %1, 1.0, %2" : "=v"(r) : "v"(v), "s"(-w));
+  return static_cast<unsigned short>(r);
+}
+
 // FP16 storage of the deviation f_q - w_q; all arithmetic stays FP32.
 struct StF16 {
   using T = _Float16;
@@ -86,11 +101,19 @@ struct StF16 {
   }
   template <class L, int q>
   __device__ __forceinline__ static float ld(const T* __restrict__ p, long long i) {
-    return L::w[q] + static_cast<float>(ld_pop(p, i));
+    return f16_bits_plus(__builtin_bit_cast(unsigned short, ld_pop(p, i)), L::w[q]);
   }
   template <class L, int q>
   __device__ __forceinline__ static void st(T* __restrict__ p, long long i, float v) {
-    st_pop(p, i, static_cast<_Float16>(v - L::w[q]));
+    st_pop(p, i, __builtin_bit_cast(_Float16, f16_bits_of_minus(v, L::w[q])));
+  }
+  template <class L, int q>
+  __device__ __forceinline__ static float from_dev_bits(unsigned short h) {
+    return f16_bits_plus(h, L::w[q]);
+  }
+  template <class L, int q>
+  __device__ __forceinline__ static unsigned short dev_bits(float v) {
+    return f16_bits_of_minus(v, L::w[q]);
   }
 };
 
@@ -114,6 +137,14 @@ struct StBF16 {
   __device__ __forceinline__ static void st(T* __restrict__ p, long long i, float v) {
     st_pop(p, i, static_cast<uint16_t>(to_bits(v - L::w[q])));
   }
+  template <class L, int q>
+  __device__ __forceinline__ static float from_dev_bits(unsigned short h) {
+    return L::w[q] + from_bits(h);
+  }
+  template <class L, int q>
+  __device__ __forceinline__ static unsigned short dev_bits(float v) {
+    return to_bits(v - L::w[q]);
+  }
 };
 
 // The 16-bit storages addressed through a buffer descriptor (population
@@ -136,7 +167,7 @@ struct StBuf : Base {
     } else {
       const unsigned short h =
           __builtin_amdgcn_raw_buffer_load_b16(r, static_cast<int>(voff), static_cast<int>(soff), 0);
-      return L::w[q] + Base::from_bits(h);
+      return Base::template from_dev_bits<L, q>(h);
     }
   }
   template <class L, int q>
@@ -145,7 +176,7 @@ struct StBuf : Base {
       __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, static_cast<int>(voff), static_cast<int>(soff),
                                             kNonTemporal);
     } else {
-      __builtin_amdgcn_raw_buffer_store_b16(Base::to_bits(v - L::w[q]), r, static_cast<int>(voff),
+      __builtin_amdgcn_raw_buffer_store_b16(Base::template dev_bits<L, q>(v), r, static_cast<int>(voff),
                                             static_cast<int>(soff), kNonTemporal);
     }
   }
@@ -786,17 +817,50 @@ __device__ __forceinline__ void block_sum3(double& s0, double& s1, double& s2) {
 
 // Rows holding a momentum-exchange link of (a.mem_c, a.mem_rmax): one lane
 // per node, row_has[row] = 1 (an idempotent store; cleared by the caller).
+// Whether Fluid node (x, y, z) holds a link of (a.mem_c, a.mem_rmax).
+template <class L>
+__device__ __forceinline__ bool has_mem_link(const StepArgs& a, int x, int y, int z, int n) {
+  if (!(a.cls[n] & kClsSlow) || a.flags[n] != kFluid) return false;
+  bool any = false;
+  for_each_mem_link<L>(a, x, y, z, a.mem_c[0], a.mem_c[1], a.mem_c[2], a.mem_rmax,
+                       [&](auto, float, float) { any = true; });
+  return any;
+}
+
 template <class L>
 __global__ __launch_bounds__(256) void k_mem_mark_rows(const StepArgs a, int* __restrict__ row_has) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y, z = blockIdx.z;
   if (x >= a.nx) return;
   const int n = (z * a.ny + y) * a.nx + x;
-  if (!(a.cls[n] & kClsSlow) || a.flags[n] != kFluid) return;
-  bool any = false;
-  for_each_mem_link<L>(a, x, y, z, a.mem_c[0], a.mem_c[1], a.mem_c[2], a.mem_rmax,
-                       [&](auto, float, float) { any = true; });
-  if (any) row_has[z * a.ny + y] = 1;
+  if (has_mem_link<L>(a, x, y, z, n)) row_has[z * a.ny + y] = 1;
+}
+
+// The nodes holding links, row by row in node order: one wave per linked row
+// (row_list) walks the row in 64-node segments; counts[r] = the row's nodes
+// (EMIT = false), or the nodes written from nodes[offsets[r]] on (EMIT = true)
+// at the place their ballot prefix gives them.
+template <class L, bool EMIT>
+__global__ __launch_bounds__(64) void k_mem_row_nodes(const StepArgs a, const int* __restrict__ row_list, int nrows,
+                                                      int* __restrict__ counts, const int* __restrict__ offsets,
+                                                      int* __restrict__ nodes) {
+  const int lane = __lane_id();
+  for (int r = blockIdx.x; r < nrows; r += gridDim.x) {  // uniform over the wave
+    const int row = row_list[r];
+    const int y = row % a.ny, z = row / a.ny;
+    int base = EMIT ? offsets[r] : 0;
+    for (int x0 = 0; x0 < a.nx; x0 += 64) {
+      const int x = x0 + lane;
+      const int n = row * a.nx + x;
+      const bool want = x < a.nx && has_mem_link<L>(a, x, y, z, n);
+      const unsigned long long ballot = __ballot(want);
+      if constexpr (EMIT) {
+        if (want) nodes[base + __popcll(ballot & ((1ull << lane) - 1ull))] = n;
+      }
+      base += __popcll(ballot);
+    }
+    if (!EMIT && lane == 0) counts[r] = base;
+  }
 }
 
 // The partials of one force sample without a step: one work-group row per

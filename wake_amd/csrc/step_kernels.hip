@@ -120,6 +120,16 @@ hipError_t launch_mem_mark_rows(int lattice, const StepArgs& a, int* row_has, hi
   }
 }
 
+hipError_t launch_mem_row_nodes(int lattice, const StepArgs& a, const int* row_list, int nrows, int* counts,
+                                const int* offsets, int* nodes, hipStream_t s) {
+  switch (lattice) {
+    case 9: return detail::mem_row_nodes_d2q9(a, row_list, nrows, counts, offsets, nodes, s);
+    case 19: return detail::mem_row_nodes_d3q19(a, row_list, nrows, counts, offsets, nodes, s);
+    case 27: return detail::mem_row_nodes_d3q27(a, row_list, nrows, counts, offsets, nodes, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 hipError_t launch_mem_rows(int lattice, int storage, bool from_s, const StepArgs& a, const void* buf,
                            const int* row_list, int nrows, hipStream_t s) {
   switch (lattice) {

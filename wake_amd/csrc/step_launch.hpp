@@ -102,6 +102,15 @@ struct Launch {
     return hipGetLastError();
   }
 
+  static hipError_t mem_row_nodes(const StepArgs& a, const int* row_list, int nrows, int* counts,
+                                  const int* offsets, int* nodes, hipStream_t s) {
+    if (nrows <= 0) return hipSuccess;
+    const int g = nrows < 65535 ? nrows : 65535;
+    if (nodes) k_mem_row_nodes<L, true><<<g, 64, 0, s>>>(a, row_list, nrows, counts, offsets, nodes);
+    else k_mem_row_nodes<L, false><<<g, 64, 0, s>>>(a, row_list, nrows, counts, offsets, nodes);
+    return hipGetLastError();
+  }
+
   static hipError_t mem_rows(int storage, bool from_s, const StepArgs& a, const void* buf, const int* row_list,
                              int nrows, hipStream_t s) {
     if (nrows <= 0) return hipSuccess;
@@ -158,6 +167,8 @@ struct Launch {
   hipError_t prev_from_s_##NAME(int storage, const StepArgs& a, const void* s_state, hipStream_t s);             \
   hipError_t readback_##NAME(int storage, bool from_s, const StepArgs& a, const void* buf, hipStream_t s);      \
   hipError_t mem_mark_rows_##NAME(const StepArgs& a, int* row_has, hipStream_t s);                             \
+  hipError_t mem_row_nodes_##NAME(const StepArgs& a, const int* row_list, int nrows, int* counts,               \
+                                  const int* offsets, int* nodes, hipStream_t s);                              \
   hipError_t mem_rows_##NAME(int storage, bool from_s, const StepArgs& a, const void* buf, const int* row_list, \
                              int nrows, hipStream_t s);
 
@@ -180,6 +191,10 @@ struct Launch {
   }                                                                                                             \
   hipError_t mem_mark_rows_##NAME(const StepArgs& a, int* row_has, hipStream_t s) {                            \
     return Launch<LAT>::mem_mark_rows(a, row_has, s);                                                           \
+  }                                                                                                             \
+  hipError_t mem_row_nodes_##NAME(const StepArgs& a, const int* row_list, int nrows, int* counts,               \
+                                  const int* offsets, int* nodes, hipStream_t s) {                             \
+    return Launch<LAT>::mem_row_nodes(a, row_list, nrows, counts, offsets, nodes, s);                           \
   }                                                                                                             \
   hipError_t mem_rows_##NAME(int storage, bool from_s, const StepArgs& a, const void* buf, const int* row_list, \
                              int nrows, hipStream_t s) {                                                        \
