@@ -294,7 +294,8 @@ def serve_record(args):
     t_srv = s.serve(reqs, realtime=True)
     lat = serve.latency_stats(s.done)
     log(f"served: {t_srv:.2f} s makespan, p50 {lat['p50_s']:.2f} s, p99 {lat['p99_s']:.2f} s, "
-        f"peak HBM {s.peak_used / 1e9:.1f} GB, {s.ticks} ticks")
+        f"peak HBM {s.peak_used / 1e9:.1f} GB, {s.ticks} ticks; host s waiting {s.stats['wait_s']:.2f}, "
+        f"at boundaries {s.stats['boundary_s']:.2f}, building domains {s.stats['build_s']:.2f}")
     if args.results:
         serve.write_results_jsonl(s.done, args.results)
         log(f"per-request results written to {args.results}")
@@ -328,6 +329,7 @@ def serve_record(args):
             "aggregate_mlups": round(work / t_srv / 1e6, 1), "plain_mlups": round(work / t_plain / 1e6, 1),
             "peak_hbm_gb": round(s.peak_used / 1e9, 2),
             "diverged_requests": len(s.diverged),
+            "host_seconds": {k: round(v, 3) for k, v in s.stats.items()},
             "result_rel_dev_vs_plain_fp32": {"median": float(np.median(dev)) if dev else None,
                                              "max": float(np.max(dev)) if dev else None},
             "parallelism": "single GPU",

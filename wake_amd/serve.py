@@ -4,13 +4,14 @@ Requests (3D bluff-body runs, 2D thermal cavities) arrive over time.  Each
 admitted request becomes a Domain whose populations stay in HBM until its
 last step; requests of one kind (lattice, population storage, collision,
 LES, temperature coupling) share a Batch that advances all of them with one
-launch per step.  Every scheduling tick launches one chunk on every
-non-empty batch (each on its own stream, so the kinds overlap on the GPU),
-waits, retires the requests that reached their step count, frees their
-memory, and admits queued requests while the HBM budget allows (first fit:
-a request that does not fit waits; others may still enter).  A request
+launch per step.  At every chunk boundary the loop completes the chunks in
+flight (one per batch, each on its own stream, so the kinds overlap on the
+GPU), retires the requests that reached their step count, puts in the
+domains built meanwhile and launches the next chunk; while that chunk runs
+it builds domains for queued requests while the HBM budget allows (first
+fit: a request that does not fit waits; others may still enter).  A request
 therefore joins the running mix at the next chunk boundary after memory
-frees up.
+frees up, and building its domain does not stop the GPU.
 
 Admission order (`policy`):
   * "least_work" (default): the queued requests with the fewest node-steps
@@ -200,6 +201,11 @@ class Server:
         self.peak_used = 0
         self.ticks = 0
         self.node_steps = 0
+        self.prepared = []      # (request, domain) built while a chunk ran, joining at the next boundary
+        self.inflight = []      # batches with a chunk in flight
+        # host seconds: waiting for chunks, at chunk boundaries (results,
+        # joins, launches) and building domains (overlapping the chunks)
+        self.stats = {"wait_s": 0.0, "boundary_s": 0.0, "build_s": 0.0}
 
     def submit(self, req):
         """Queue a request; one that could never fit the budget is rejected
@@ -212,43 +218,51 @@ class Server:
         self.queue.append(req)
 
     def _admit(self, now):
+        """Build domains for queued requests that fit; they join their batch
+        at the next chunk boundary."""
         kept = deque()
         for req in admission_order(self.queue, now, self.policy, self.max_wait_s):
             need = bytes_needed(req)
-            full = self.policy == "least_work" and self.active and self.active_nodes >= self.fill_nodes
+            busy = self.active or self.prepared
+            full = self.policy == "least_work" and busy and self.active_nodes >= self.fill_nodes
             if full or self.used + need > self.budget:
                 kept.append(req)
                 continue
-            g = make_domain(req)
-            key = batch_key(req)
-            b = self.batches.get(key)
-            if b is None:
-                lat, st, coll, les, thermal = key
-                b = Batch(lattice=lat, storage=st, collision=coll, use_les=les, thermal=thermal, chunk=self.chunk)
-                # row segments of 64 lanes: no idle lanes for the 64-multiple
-                # grid edges of the request mix
-                b.set_launch(64, 0)
-                self.batches[key] = b
-            slot = b.add(g, req.steps, **({"smag": CS_LES} if req.kind == "bluff3d" else {}))
-            self.active[req.rid] = (req, g, key, slot)
+            t0 = time.perf_counter()
+            self.prepared.append((req, make_domain(req)))
+            self.stats["build_s"] += time.perf_counter() - t0
             self.used += need
             self.active_nodes += node_count(req)
             self.peak_used = max(self.peak_used, self.used)
             req.admitted = now
         self.queue = kept
 
+    def _join(self, req, g):
+        key = batch_key(req)
+        b = self.batches.get(key)
+        if b is None:
+            lat, st, coll, les, thermal = key
+            b = Batch(lattice=lat, storage=st, collision=coll, use_les=les, thermal=thermal, chunk=self.chunk)
+            # row segments of 64 lanes: no idle lanes for the 64-multiple
+            # grid edges of the request mix
+            b.set_launch(64, 0)
+            self.batches[key] = b
+        slot = b.add(g, req.steps, **({"smag": CS_LES} if req.kind == "bluff3d" else {}))
+        self.active[req.rid] = (req, g, key, slot)
+
     def tick(self, now_fn):
-        """Admit, run one chunk on every batch (overlapped), retire."""
-        self._admit(now_fn())
-        live = [b for b in self.batches.values() if b.info()["members"] > 0]
-        if not live:
-            return False
-        for b in live:
-            b.launch(self.chunk, self.use_graph)
-        for b in live:
+        """One chunk boundary.  Complete the chunks in flight; take the
+        requests that reached their step count (or diverged) out of their
+        batches and compute their results; put in the domains built
+        meanwhile; launch the next chunk on every batch; then, while it runs,
+        build domains for queued requests that now fit.  Returns whether
+        anything is in flight or waiting to join."""
+        t0 = time.perf_counter()
+        for b in self.inflight:
             b.wait()
-        self.ticks += 1
-        t = now_fn()
+        self.inflight = []
+        self.stats["wait_s"] += time.perf_counter() - t0
+        t0 = time.perf_counter()
         for rid in list(self.active):
             req, g, key, slot = self.active[rid]
             b = self.batches[key]
@@ -256,28 +270,42 @@ class Server:
             # non-finite density) is retired with an error, freeing its slot
             diverged = g.nonfinite_seen()
             if diverged or b.remaining(slot) == 0:
+                b.remove(slot)
+                del self.active[rid]
                 if diverged:
                     req.result = {"error": f"diverged: non-finite density by step {g.current_step()}"}
                     self.diverged.append(req.rid)
                 else:
                     req.result = result_of(req, g)
-                req.finished = t
-                b.remove(slot)
+                req.finished = now_fn()
                 self.used -= bytes_needed(req)
                 self.active_nodes -= node_count(req)
                 self.node_steps += g.total_nodes() * req.steps
-                del self.active[rid]
                 self.done.append(req)
-        return True
+                del g
+        for req, g in self.prepared:
+            self._join(req, g)
+        self.prepared = []
+        live = [b for b in self.batches.values() if b.info()["members"] > 0]
+        for b in live:
+            b.launch(self.chunk, self.use_graph)
+        self.inflight = live
+        self.stats["boundary_s"] += time.perf_counter() - t0
+        if live:
+            self.ticks += 1
+        self._admit(now_fn())
+        return bool(live) or bool(self.prepared)
 
     def serve(self, requests, realtime=True):
         """Replay `requests` (sorted by arrival) against the wall clock."""
         pending = deque(sorted(requests, key=lambda r: r.arrival))
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
-        while pending or self.queue or self.active:
+        while pending or self.queue or self.active or self.prepared or self.inflight:
             while pending and (not realtime or pending[0].arrival <= now()):
                 self.submit(pending.popleft())
+            if not self.inflight and not self.prepared:
+                self._admit(now())
             if not self.tick(now) and pending:
                 wait = pending[0].arrival - now()
                 if wait > 0:
