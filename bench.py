@@ -460,11 +460,13 @@ def main_tg2d(args):
     err = float(np.linalg.norm(np.concatenate([g.ux_data(), g.uy_data()]) - np.concatenate([o.ux(), o.uy()]))
                 / np.linalg.norm(np.concatenate([o.ux(), o.uy()])))
     log(f"oracle on the host: {cpu:.0f} MLUPS on {ob.lib(32).oracle_threads()} threads; GPU vs oracle u rel L2 {err:.2e}")
+    best, best_wall, path = max((mlups, wall, "plain"), (mlups_b, wall_b, "batch_of_one_graph"))
     out = {
-        "metric": METRIC, "value": round(mlups_b, 1), "unit": "MLUPS", "n_gpus": 1, "steps": S, "warmup": 10,
-        "ms_per_step": round(wall_b * 1e3 / S, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "metric": METRIC, "value": round(best, 1), "unit": "MLUPS", "n_gpus": 1, "steps": S, "warmup": 10,
+        "ms_per_step": round(best_wall * 1e3 / S, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32", "data": "synthetic",
         "config": {"workload": "D2Q9 BGK Taylor-Green vortex 256^2 periodic, tau 0.8, u0 0.04, noise 1e-6 (seed 1)",
+                   "value_path": path,
                    "plain_mlups": round(mlups, 1), "batch_of_one_graph_mlups": round(mlups_b, 1),
                    "on_chip": "2 x 2.36 MB working set: cache-resident, not an HBM-roofline figure",
                    "ke_ratio": round(ratio, 6), "ke_ratio_analytic": round(expect, 6),
