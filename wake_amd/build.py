@@ -5,6 +5,12 @@ an object is rebuilt when its source or any header in csrc/ or include/ is
 newer), then everything links into wake_amd/libwake_hip.so.
 
     python wake_amd/build.py [--force] [-j N]
+    WAKE_EXTRA_DEVICE_FLAGS=-DNAME=1 python wake_amd/build.py --variant V
+
+A variant build (code-generation switches such as WAKE_MRT_MFMA, passed in
+WAKE_EXTRA_DEVICE_FLAGS) goes to wake_amd/_build_V/ and
+wake_amd/_variants/lib_V.so, and is loaded instead of the default library
+when WAKE_HIP_LIB names it.
 
 Run it as a script: `python -m wake_amd.build` would import the package,
 which refuses to load until the library exists.
@@ -50,9 +56,9 @@ def _headers_mtime():
     return max((os.path.getmtime(h) for h in hs), default=0.0)
 
 
-def _compile(src, force, hdr_mtime):
+def _compile(src, force, hdr_mtime, build_dir=BUILD):
     path = os.path.join(CSRC, src)
-    obj = os.path.join(BUILD, src + ".o")
+    obj = os.path.join(build_dir, src + ".o")
     if not force and os.path.exists(obj):
         om = os.path.getmtime(obj)
         if om >= os.path.getmtime(path) and om >= hdr_mtime:
@@ -67,14 +73,19 @@ def _compile(src, force, hdr_mtime):
     return obj, "built"
 
 
-def build(force=False, jobs=None, verbose=True):
-    os.makedirs(BUILD, exist_ok=True)
+def build(force=False, jobs=None, verbose=True, variant=None):
+    build_dir, lib = BUILD, LIB
+    if variant:
+        build_dir = os.path.join(HERE, "_build_" + variant)
+        lib = os.path.join(HERE, "_variants", f"lib_{variant}.so")
+        os.makedirs(os.path.dirname(lib), exist_ok=True)
+    os.makedirs(build_dir, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     hdr = _headers_mtime()
     jobs = jobs or min(len(srcs), os.cpu_count() or 4, 16)
     objs = []
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        futs = {ex.submit(_compile, s, force, hdr): s for s in srcs}
+        futs = {ex.submit(_compile, s, force, hdr, build_dir): s for s in srcs}
         for f in cf.as_completed(futs):
             obj, what = f.result()
             objs.append(obj)
@@ -82,15 +93,16 @@ def build(force=False, jobs=None, verbose=True):
                 print(f"[wake_amd.build] {futs[f]}: {what}", flush=True)
     objs.sort()
     newest = max(os.path.getmtime(o) for o in objs)
-    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < newest:
-        cmd = ["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", LIB, *objs]
+    if force or not os.path.exists(lib) or os.path.getmtime(lib) < newest:
+        cmd = ["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", lib, *objs]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
         if verbose:
-            print(f"[wake_amd.build] linked {LIB}", flush=True)
-    build_api_check(force=force, verbose=verbose)
-    return LIB
+            print(f"[wake_amd.build] linked {lib}", flush=True)
+    if not variant:
+        build_api_check(force=force, verbose=verbose)
+    return lib
 
 
 API_CHECK_SRC = os.path.join(ROOT, "tests", "cpp", "grid_api_check.cpp")
@@ -118,9 +130,10 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--force", action="store_true")
     ap.add_argument("-j", type=int, default=None)
+    ap.add_argument("--variant", default=None, help="build into _build_V/ and _variants/lib_V.so")
     a = ap.parse_args()
     try:
-        build(force=a.force, jobs=a.j)
+        build(force=a.force, jobs=a.j, variant=a.variant)
     except RuntimeError as e:
         print(e, file=sys.stderr)
         sys.exit(1)

@@ -119,6 +119,7 @@ void Domain::allocate_and_initialize() {
   d_flags_ = static_cast<uint8_t*>(alloc(N_));
   d_cls_ = static_cast<uint8_t*>(alloc(N_));
   d_aux_ = static_cast<int*>(alloc(N_ * sizeof(int)));
+  d_bmask_ = static_cast<uint32_t*>(alloc(N_ * sizeof(uint32_t)));
   d_prev_ = static_cast<float*>(alloc(static_cast<size_t>(prev_stride(Q_)) * ny_ * nz_ * sizeof(float)));
   d_rho_ = static_cast<float*>(alloc(N_ * 4));
   d_ux_ = static_cast<float*>(alloc(N_ * 4));
@@ -650,6 +651,7 @@ StepArgs Domain::args(bool commit) const {
   a.flags = d_flags_;
   a.cls = d_cls_;
   a.aux = d_aux_;
+  a.bmask = d_bmask_;
   a.bfl = d_bfl_;
   a.gx = g_last_[0];
   a.gy = g_last_[1];
@@ -718,7 +720,7 @@ void Domain::sync_device_config() {
   }
   if (cls_dirty_) {
     const StepArgs a = args(false);
-    check(launch_build_class(lattice_, a, d_cls_, stream_), "build class");
+    check(launch_build_class(lattice_, a, d_cls_, d_bmask_, stream_), "build class");
     check(launch_mark_bfl(d_bfl_nodes_, n_bfl_nodes_, d_cls_, d_aux_, stream_), "mark bfl");
     cls_dirty_ = false;
     ++cls_epoch_;
@@ -1370,7 +1372,7 @@ size_t Domain::bytes_needed(int dim, int lattice, int nx, int ny, int nz, int st
   const long long stride = base + (plane_pad >= 0 ? plane_pad : (N >= (1u << 20) ? 1024 : 0));
   const size_t yz = static_cast<size_t>(ny) * (dim == 2 ? 1 : nz);
   const size_t t = thermal ? 2 * 5 * static_cast<size_t>(stride) * 4 + N * 8 : 0;
-  return 2 * static_cast<size_t>(Q) * stride * elem_size(storage) + N * (1 + 1 + 4 + 5 * 4) + prev_stride(Q) * yz * 4 +
+  return 2 * static_cast<size_t>(Q) * stride * elem_size(storage) + N * (1 + 1 + 4 + 4 + 5 * 4) + prev_stride(Q) * yz * 4 +
          3 * 1024 * 8 + 64 + t;
 }
 

@@ -276,6 +276,7 @@ class Domain {
   int a_buf_ = -1;  // buffer holding A_{k-1}, or -1
   uint8_t* d_flags_ = nullptr;
   uint8_t* d_cls_ = nullptr;
+  uint32_t* d_bmask_ = nullptr;  // per node: bounce mask of the kClsSlow nodes
   int* d_aux_ = nullptr;
   BflNode* d_bfl_ = nullptr;
   int* d_bfl_nodes_ = nullptr;

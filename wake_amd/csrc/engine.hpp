@@ -30,6 +30,13 @@ enum ClassBits : uint8_t {
   kClsOutlet = 32  // Zou-He pressure or convective outlet node
 };
 
+// Per-node bounce mask of a kClsSlow node: bit q set when the pull source
+// along c_q is out of range (non-periodic axis), Solid or a MovingWall;
+// kBounceMovingWall set when one of them is a MovingWall.
+enum : uint32_t {
+  kBounceMovingWall = 1u << 31
+};
+
 // Curved-wall (BFL) links of one fluid node: bit q of mask marks a link from
 // the node along c_q into the wall, with wall fraction delta[q] and the
 // wall-velocity projection c_q . u_w in ciuw[q].
@@ -66,6 +73,7 @@ struct StepArgs {
   const uint8_t* flags;       // reference cell types
   const uint8_t* cls;         // ClassBits
   const int* aux;             // per node: index into bfl[] (valid where kClsBfl)
+  const uint32_t* bmask;      // per node: bounce mask (valid where kClsSlow); null: from the cell types
   const BflNode* bfl;
   // read-back outputs
   float* rho;
@@ -95,7 +103,7 @@ struct StepArgs {
 inline int mem_waves_per_row(int nx) { return (nx + 63) / 64; }
 
 // Launchers (step_kernels.hip).  `lattice` is 9, 19 or 27.
-hipError_t launch_build_class(int lattice, const StepArgs& a, uint8_t* cls, hipStream_t s);
+hipError_t launch_build_class(int lattice, const StepArgs& a, uint8_t* cls, uint32_t* bmask, hipStream_t s);
 hipError_t launch_mark_bfl(const int* nodes, int count, uint8_t* cls, int* aux, hipStream_t s);
 // pull=false: collide-only from the materialised S state.  block_x is the
 // work-group width along x (64, 128 or 256).

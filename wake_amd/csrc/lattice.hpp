@@ -101,32 +101,37 @@ constexpr double round7(double v) {
   return out == 0.0 ? 0.0 : out;
 }
 
+// Entry (k, q) of the d'Humieres D3Q19 moment matrix: basis polynomial k at
+// velocity q.
+constexpr double d3q19_m(int k, int q) {
+  const double cx = D3Q19::cx[q], cy = D3Q19::cy[q], cz = D3Q19::cz[q];
+  const double c2 = cx * cx + cy * cy + cz * cz;
+  const double row[19] = {1.0,
+                          19.0 * c2 - 30.0,
+                          (21.0 * c2 * c2 - 53.0 * c2 + 24.0) / 2.0,
+                          cx,
+                          (5.0 * c2 - 9.0) * cx,
+                          cy,
+                          (5.0 * c2 - 9.0) * cy,
+                          cz,
+                          (5.0 * c2 - 9.0) * cz,
+                          3.0 * cx * cx - c2,
+                          (3.0 * c2 - 5.0) * (3.0 * cx * cx - c2),
+                          cy * cy - cz * cz,
+                          (3.0 * c2 - 5.0) * (cy * cy - cz * cz),
+                          cx * cy,
+                          cy * cz,
+                          cx * cz,
+                          cx * (cy * cy - cz * cz),
+                          cy * (cz * cz - cx * cx),
+                          cz * (cy * cy - cx * cx)};
+  return row[k];
+}
+
 constexpr MrtTable make_d3q19_minv() {
   double M[19][19] = {};
-  for (int q = 0; q < 19; ++q) {
-    const double cx = D3Q19::cx[q], cy = D3Q19::cy[q], cz = D3Q19::cz[q];
-    const double c2 = cx * cx + cy * cy + cz * cz;
-    const double row[19] = {1.0,
-                            19.0 * c2 - 30.0,
-                            (21.0 * c2 * c2 - 53.0 * c2 + 24.0) / 2.0,
-                            cx,
-                            (5.0 * c2 - 9.0) * cx,
-                            cy,
-                            (5.0 * c2 - 9.0) * cy,
-                            cz,
-                            (5.0 * c2 - 9.0) * cz,
-                            3.0 * cx * cx - c2,
-                            (3.0 * c2 - 5.0) * (3.0 * cx * cx - c2),
-                            cy * cy - cz * cz,
-                            (3.0 * c2 - 5.0) * (cy * cy - cz * cz),
-                            cx * cy,
-                            cy * cz,
-                            cx * cz,
-                            cx * (cy * cy - cz * cz),
-                            cy * (cz * cz - cx * cx),
-                            cz * (cy * cy - cx * cx)};
-    for (int k = 0; k < 19; ++k) M[k][q] = row[k];
-  }
+  for (int q = 0; q < 19; ++q)
+    for (int k = 0; k < 19; ++k) M[k][q] = d3q19_m(k, q);
   double A[19][38] = {};
   for (int i = 0; i < 19; ++i)
     for (int j = 0; j < 38; ++j) A[i][j] = (j < 19) ? M[i][j] : ((j - 19) == i ? 1.0 : 0.0);

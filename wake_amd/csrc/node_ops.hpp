@@ -340,11 +340,7 @@ struct D3Q19OddMoment {
 //   Pi_yy, Pi_zz = ((dm1 / 19 - Pi_xx) +- dm11) / 2,
 //   Pi_xy, Pi_yz, Pi_xz = dm13, dm14, dm15,
 // which is sum_q c c (f - feq) of d3q19_les.ispc:139-159 without forming feq.
-template <bool LES = false, bool FAST = false>
-__device__ __forceinline__ Macro collide_mrt_d3q19(float (&f)[19], float tau_shear, float tau_bulk, float gx,
-                                                   float gy, float gz, float cs = 0.0f) {
-  const float s_e = 1.0f / tau_bulk, s_eps = 1.4f, s_q = 1.2f, s_pi = 1.4f, s_m = 1.98f;
-  float m[19];
+__device__ __forceinline__ void mrt_forward_d3q19(const float (&f)[19], float (&m)[19]) {
   m[0] = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8] + f[9] + f[10] + f[11] + f[12] + f[13] +
          f[14] + f[15] + f[16] + f[17] + f[18];
   const float faces = f[1] + f[2] + f[3] + f[4] + f[5] + f[6];
@@ -372,7 +368,13 @@ __device__ __forceinline__ Macro collide_mrt_d3q19(float (&f)[19], float tau_she
   m[16] = (f[7] - f[8] + f[9] - f[10] - f[11] + f[12] - f[13] + f[14]);
   m[17] = (-f[7] - f[8] + f[9] + f[10] + f[15] - f[16] + f[17] - f[18]);
   m[18] = (-f[11] - f[12] + f[13] + f[14] + f[15] + f[16] - f[17] - f[18]);
+}
 
+// Moment-space relaxation of one node (in place on m), returning rho and u.
+template <bool LES, bool FAST>
+__device__ __forceinline__ Macro mrt_relax_d3q19(float (&m)[19], float tau_shear, float tau_bulk, float gx, float gy,
+                                                 float gz, float cs) {
+  const float s_e = 1.0f / tau_bulk, s_eps = 1.4f, s_q = 1.2f, s_pi = 1.4f, s_m = 1.98f;
   const float rho = m[0];
   const float inv_rho = recip<FAST>(rho);
   const float ux = (m[3] + 0.5f * rho * gx) * inv_rho;
@@ -420,8 +422,12 @@ __device__ __forceinline__ Macro collide_mrt_d3q19(float (&f)[19], float tau_she
   m[16] = m[16] - s_m * m[16];
   m[17] = m[17] - s_m * m[17];
   m[18] = m[18] - s_m * m[18];
+  return {rho, ux, uy, uz};
+}
 
-  // Back transform (odd moments: see D3Q19OddMoment).
+// Back transform with the rounded inverse table (odd moments: see
+// D3Q19OddMoment).
+__device__ __forceinline__ void mrt_inverse_d3q19(const float (&m)[19], float (&f)[19]) {
   static_for<19>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     constexpr int o = D3Q19::opp[q];
@@ -447,7 +453,16 @@ __device__ __forceinline__ Macro collide_mrt_d3q19(float (&f)[19], float tau_she
       f[o] = ev - od;
     }
   });
-  return {rho, ux, uy, uz};
+}
+
+template <bool LES = false, bool FAST = false>
+__device__ __forceinline__ Macro collide_mrt_d3q19(float (&f)[19], float tau_shear, float tau_bulk, float gx,
+                                                   float gy, float gz, float cs = 0.0f) {
+  float m[19];
+  mrt_forward_d3q19(f, m);
+  const Macro r = mrt_relax_d3q19<LES, FAST>(m, tau_shear, tau_bulk, gx, gy, gz, cs);
+  mrt_inverse_d3q19(m, f);
+  return r;
 }
 
 // --------------------------------------------------------------------------

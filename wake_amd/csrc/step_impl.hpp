@@ -24,6 +24,15 @@
 #include "lattice.hpp"
 #include "node_ops.hpp"
 
+// WAKE_MRT_MFMA=1 builds the D3Q19 MRT step with 16-bit storage with its
+// moment transforms on the matrix cores (mrt_mfma.hpp); off by default.
+#ifndef WAKE_MRT_MFMA
+#define WAKE_MRT_MFMA 0
+#endif
+#if WAKE_MRT_MFMA
+#include "mrt_mfma.hpp"
+#endif
+
 namespace wake_hip {
 
 // ----------------------------------------------------------------- storage
@@ -250,40 +259,64 @@ __device__ __forceinline__ void gather(const StepArgs& a, const typename St::T* 
       f[q] = St::template ld<L, q>(A, q * S + n);
     });
   } else if (c & kClsSlow) {
-    static_for<L::Q>([&](auto qc) {
+    // Cell type of the pull source along c_q (Fluid when out of range).
+    auto source = [&](auto qc, bool& in) -> uint8_t {
       constexpr int q = decltype(qc)::value;
-      constexpr int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q], o = L::opp[q];
-      if constexpr (q > 0) {
-        const int sx = cx == 0 ? x : (cx > 0 ? xm : xp);
-        int sy = y - cy, sz = z - cz;
-        bool in = true;
-        if constexpr (cy != 0) {
-          if (a.per_y) sy = wrap(sy, ny);
-          else in = in && sy >= 0 && sy < ny;
-        }
-        if constexpr (cz != 0) {
-          if (a.per_z) sz = wrap(sz, nz);
-          else in = in && sz >= 0 && sz < nz;
-        }
-        // Halfway bounce-back (domain wall, solid or moving-wall source)
-        // replaces the pulled value with the node's own opposite population.
-        const int sn = in ? (sz * ny + sy) * nx + sx : n;
-        const uint8_t t = (in && !a.periodic_mode) ? a.flags[sn] : static_cast<uint8_t>(kFluid);
-        const bool bounce = !in || t == kSolid || t == kMovingWall;
-        float v = f[q];
-        if (bounce) {
-          v = St::template ld<L, o>(A, o * S + n);
-          if (t == kMovingWall) {
-            // 2 w_q rho0 3 (c_q . u_w); the products with c in {-1, 0, 1} are
-            // exact, so this rounds as the host's left-to-right expression
-            const float ci_uw = static_cast<float>(cx) * a.uw[0] + static_cast<float>(cy) * a.uw[1] +
-                                static_cast<float>(cz) * a.uw[2];
-            v += 2.0f * L::w[q] * 1.0f * 3.0f * ci_uw;
+      constexpr int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
+      const int sx = cx == 0 ? x : (cx > 0 ? xm : xp);
+      int sy = y - cy, sz = z - cz;
+      in = true;
+      if constexpr (cy != 0) {
+        if (a.per_y) sy = wrap(sy, ny);
+        else in = in && sy >= 0 && sy < ny;
+      }
+      if constexpr (cz != 0) {
+        if (a.per_z) sz = wrap(sz, nz);
+        else in = in && sz >= 0 && sz < nz;
+      }
+      const int sn = in ? (sz * ny + sy) * nx + sx : n;
+      return (in && !a.periodic_mode) ? a.flags[sn] : static_cast<uint8_t>(kFluid);
+    };
+    // Halfway bounce-back (domain wall, solid or moving-wall source) replaces
+    // the pulled value with the node's own opposite population; a moving wall
+    // adds 2 w_q rho0 3 (c_q . u_w), whose products with c in {-1, 0, 1} are
+    // exact, so it rounds as the host's left-to-right expression.
+    auto moving_wall_term = [&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      const float ci_uw = static_cast<float>(L::cx[q]) * a.uw[0] + static_cast<float>(L::cy[q]) * a.uw[1] +
+                          static_cast<float>(L::cz[q]) * a.uw[2];
+      return 2.0f * L::w[q] * 1.0f * 3.0f * ci_uw;
+    };
+    if (a.bmask) {
+      // the bounce mask built with the class bytes: no cell-type reads
+      const uint32_t bm = a.bmask[n];
+      static_for<L::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (q > 0) {
+          if (bm & (1u << q)) {
+            float v = St::template ld<L, L::opp[q]>(A, L::opp[q] * S + n);
+            if (bm & kBounceMovingWall) {
+              bool in;
+              if (source(qc, in) == kMovingWall) v += moving_wall_term(qc);
+            }
+            f[q] = v;
           }
         }
-        f[q] = v;
-      }
-    });
+      });
+    } else {
+      static_for<L::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (q > 0) {
+          bool in;
+          const uint8_t t = source(qc, in);
+          if (!in || t == kSolid || t == kMovingWall) {
+            float v = St::template ld<L, L::opp[q]>(A, L::opp[q] * S + n);
+            if (t == kMovingWall) v += moving_wall_term(qc);
+            f[q] = v;
+          }
+        }
+      });
+    }
   }
   if (c & kClsBfl) {
     const BflNode* r = a.bfl + a.aux[n];
@@ -322,9 +355,13 @@ __device__ __forceinline__ void gather(const StepArgs& a, const typename St::T* 
 }
 
 // --------------------------------------------------------- open boundaries
+// Multiply-adds are fused per source expression only ("contract on"): with
+// cross-statement fusion the inlet's fused pairs depended on the code around
+// the call, so kernels that differ only in addressing rounded it differently.
 template <class L>
 __device__ __forceinline__ void apply_inlet(const StepArgs& a, float (&f)[L::Q], int y, int n,
                                             unsigned long long s_step) {
+#pragma clang fp contract(on)
   float ux = a.uin_x;
   if (a.inflow_parabolic) {
     const float s = static_cast<float>(y) - 0.5f;
@@ -437,15 +474,7 @@ template <class L, class St, bool ROW_LANES = true>
 __device__ __forceinline__ void form_s_at(const StepArgs& a, const typename St::T* __restrict__ A, int x, int y,
                                           int z, int n, uint8_t c, bool commit, unsigned long long s_step,
                                           float (&f)[L::Q]) {
-  gather<L, St>(a, A, x, y, z, n, c, f);
-  if (c & kClsInlet) apply_inlet<L>(a, f, y, n, s_step);
-  if constexpr (L::Q == 9) {
-    if ((c & kClsOutlet) && a.outflow == 1) apply_zh_outflow_d2q9(a, f);
-  }
-  if (a.outflow != 2) return;
   constexpr int QP = prev_stride(L::Q);
-  const float alpha = a.conv_alpha, beta = a.conv_beta;
-  const bool is_out = (c & kClsOutlet) != 0;
   float4* rec = reinterpret_cast<float4*>(a.f_prev + (static_cast<long long>(z) * a.ny + y) * QP);
   auto load_prev = [&](float (&prev)[QP]) {
     static_for<QP / 4>([&](auto vc) {
@@ -463,14 +492,34 @@ __device__ __forceinline__ void form_s_at(const StepArgs& a, const typename St::
       rec[v] = make_float4(prev[4 * v], prev[4 * v + 1], prev[4 * v + 2], prev[4 * v + 3]);
     });
   };
+  // With ROW_LANES the convective outlet's record is loaded together with
+  // the pull: whether a wave holds an outlet lane follows from the lanes'
+  // positions (the rule that sets kClsOutlet), so the load does not wait
+  // for the class byte.  Every lane loads its own row's record (in bounds
+  // for all lanes).
+  float prev[QP];
+  bool rec_loaded = false;
+  if constexpr (ROW_LANES) {
+    const bool out_pos = a.outflow == 2 && x == a.nx - 1 && y >= a.by0 && y < a.by1 && z >= a.bz0 && z < a.bz1;
+    if (__any(out_pos)) {
+      load_prev(prev);
+      rec_loaded = true;
+    }
+  }
+  gather<L, St>(a, A, x, y, z, n, c, f);
+  if (c & kClsInlet) apply_inlet<L>(a, f, y, n, s_step);
+  if constexpr (L::Q == 9) {
+    if ((c & kClsOutlet) && a.outflow == 1) apply_zh_outflow_d2q9(a, f);
+  }
+  if (a.outflow != 2) return;
+  const float alpha = a.conv_alpha, beta = a.conv_beta;
+  const bool is_out = (c & kClsOutlet) != 0;
   bool done = false;
   if constexpr (ROW_LANES) {
-    // wave-uniform: every lane loads its own row's record (in bounds for all
-    // lanes) and takes part in the shuffles; only outlet lanes keep results
-    if (__any(is_out)) {
+    // wave-uniform: every lane takes part in the shuffles; only outlet lanes
+    // keep results
+    if (rec_loaded) {
       const bool use = is_out && __lane_id() != 0;
-      float prev[QP];
-      load_prev(prev);
       static_for<L::Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         const float v = beta * prev[q] + alpha * __shfl_up(f[q], 1);
@@ -487,8 +536,7 @@ __device__ __forceinline__ void form_s_at(const StepArgs& a, const typename St::
   const uint8_t cn = a.cls[n - 1];
   gather<L, St>(a, A, x - 1, y, z, n - 1, cn, nb);
   if (cn & kClsInlet) apply_inlet<L>(a, nb, y, n - 1, s_step);
-  float prev[QP];
-  load_prev(prev);
+  load_prev(prev);  // (the shuffle pass above, if any, overwrote prev)
   static_for<L::Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     const float v = beta * prev[q] + alpha * nb[q];
@@ -586,16 +634,16 @@ __device__ __forceinline__ void mem_wave(const StepArgs& a, bool want, int x, in
 
 // ----------------------------------------------------------------- kernels
 template <class L>
-__global__ void k_build_class(const StepArgs a, uint8_t* __restrict__ cls) {
+__global__ void k_build_class(const StepArgs a, uint8_t* __restrict__ cls, uint32_t* __restrict__ bmask) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y, z = blockIdx.z;
   if (x >= a.nx) return;
   const int n = (z * a.ny + y) * a.nx + x;
   const uint8_t t = a.flags[n];
   uint8_t c = 0;
+  uint32_t mask = 0;
   if (t == kSolid) c |= a.periodic_mode ? kClsSolid : (kClsSolid | kClsFrozen);
   if (!a.periodic_mode && t != kSolid) {
-    bool slow = false;
     static_for<L::Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       if constexpr (q > 0) {
@@ -605,15 +653,17 @@ __global__ void k_build_class(const StepArgs a, uint8_t* __restrict__ cls) {
         else in = in && sy >= 0 && sy < a.ny;
         if (a.per_z) sz = wrap(sz, a.nz);
         else in = in && sz >= 0 && sz < a.nz;
-        if (!in) slow = true;
+        if (!in) mask |= 1u << q;
         else {
           const uint8_t nt = a.flags[(sz * a.ny + sy) * a.nx + sx];
-          if (nt == kSolid || nt == kMovingWall) slow = true;
+          if (nt == kSolid || nt == kMovingWall) mask |= 1u << q;
+          if (nt == kMovingWall) mask |= kBounceMovingWall;
         }
       }
     });
-    if (slow) c |= kClsSlow;
+    if (mask) c |= kClsSlow;
   }
+  if (bmask) bmask[n] = mask;
   const bool in_rows = y >= a.by0 && y < a.by1 && z >= a.bz0 && z < a.bz1;
   if (a.inflow && x == 0 && in_rows) c |= kClsInlet;
   if (a.outflow && x == a.nx - 1 && in_rows) c |= kClsOutlet;
@@ -684,6 +734,10 @@ __global__ __launch_bounds__(256) WAKE_STEP_ATTR void k_step(const StepArgs a, c
   float fn[L::Q];
   uint8_t cn = 0;
   int nn = 0;
+#if WAKE_MRT_MFMA
+  constexpr bool kMfma = std::is_same_v<L, D3Q19> && COLL == kMRT && St::kFast;
+  __shared__ float mfma_tile[kMfma ? 4 * kMfmaTile : 1];
+#endif
   if (live) fetch(z0, fn, cn, nn);
   if constexpr (PULL && WAKE_ZK == 1) {
     if (mem_slot >= 0) {
@@ -704,7 +758,25 @@ __global__ __launch_bounds__(256) WAKE_STEP_ATTR void k_step(const StepArgs a, c
     const uint8_t c = cn;
     const int n = nn;
     if (k + 1 < WAKE_ZK && z + 1 < a.nz) fetch(z + 1, fn, cn, nn);
-    if (!(c & kClsSolid)) {
+    bool collided = false;
+#if WAKE_MRT_MFMA
+    if constexpr (kMfma) {
+      // the products need every lane of the wave; solid lanes compute and
+      // keep their own values
+      if (__builtin_amdgcn_read_exec() == ~0ull) {
+        float g[L::Q];
+        static_for<L::Q>([&](auto qc) { g[decltype(qc)::value] = f[decltype(qc)::value]; });
+        const Macro m = collide_mrt_d3q19_mfma<LES, St::kFast>(g, mfma_tile + (threadIdx.x >> 6) * kMfmaTile, a.tau,
+                                                               kTauBulk3D, a.gx, a.gy, a.gz, LES ? a.cs : 0.0f);
+        if (!(c & kClsSolid)) {
+          static_for<L::Q>([&](auto qc) { f[decltype(qc)::value] = g[decltype(qc)::value]; });
+          flag_nonfinite(a, m.rho);
+        }
+        collided = true;
+      }
+    }
+#endif
+    if (!collided && !(c & kClsSolid)) {
       Macro m;
       if constexpr (LES) m = collide_les<L, COLL, St::kFast>(f, a.tau, a.cs, a.gx, a.gy, a.gz);
       else m = collide<L, COLL, false, St::kFast>(f, a.tau, a.gx, a.gy, a.gz);

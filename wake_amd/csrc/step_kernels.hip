@@ -63,11 +63,11 @@ __global__ void k_diag_final(const double* __restrict__ partial, int nblocks, do
 
 
 
-hipError_t launch_build_class(int lattice, const StepArgs& a, uint8_t* cls, hipStream_t s) {
+hipError_t launch_build_class(int lattice, const StepArgs& a, uint8_t* cls, uint32_t* bmask, hipStream_t s) {
   switch (lattice) {
-    case 9: return detail::build_class_d2q9(a, cls, s);
-    case 19: return detail::build_class_d3q19(a, cls, s);
-    case 27: return detail::build_class_d3q27(a, cls, s);
+    case 9: return detail::build_class_d2q9(a, cls, bmask, s);
+    case 19: return detail::build_class_d3q19(a, cls, bmask, s);
+    case 27: return detail::build_class_d3q27(a, cls, bmask, s);
     default: return hipErrorInvalidValue;
   }
 }

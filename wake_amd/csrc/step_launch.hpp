@@ -72,9 +72,9 @@ hipError_t step_dispatch(bool les, bool pull, const StepArgs& a, const void* in,
 
 template <class L>
 struct Launch {
-  static hipError_t build_class(const StepArgs& a, uint8_t* cls, hipStream_t s) {
+  static hipError_t build_class(const StepArgs& a, uint8_t* cls, uint32_t* bmask, hipStream_t s) {
     const int bx = pick_block(a.nx, 0);
-    k_build_class<L><<<grid_of(a, bx), bx, 0, s>>>(a, cls);
+    k_build_class<L><<<grid_of(a, bx), bx, 0, s>>>(a, cls, bmask);
     return hipGetLastError();
   }
 
@@ -160,7 +160,7 @@ struct Launch {
 
 // Per-lattice entry points, defined in step_d2q9.hip / step_d3q19.hip / step_d3q27.hip.
 #define WAKE_DECLARE_LATTICE(NAME)                                                                             \
-  hipError_t build_class_##NAME(const StepArgs& a, uint8_t* cls, hipStream_t s);                               \
+  hipError_t build_class_##NAME(const StepArgs& a, uint8_t* cls, uint32_t* bmask, hipStream_t s);              \
   hipError_t step_##NAME(int storage, int collision, bool les, bool pull, const StepArgs& a, const void* a_in, \
                          void* a_out, int block_x, hipStream_t s);                                             \
   hipError_t materialize_##NAME(int storage, const StepArgs& a, const void* a_in, void* s_state, hipStream_t s); \
@@ -173,8 +173,8 @@ struct Launch {
                              int nrows, hipStream_t s);
 
 #define WAKE_DEFINE_LATTICE(NAME, LAT)                                                                          \
-  hipError_t build_class_##NAME(const StepArgs& a, uint8_t* cls, hipStream_t s) {                               \
-    return Launch<LAT>::build_class(a, cls, s);                                                                 \
+  hipError_t build_class_##NAME(const StepArgs& a, uint8_t* cls, uint32_t* bmask, hipStream_t s) {              \
+    return Launch<LAT>::build_class(a, cls, bmask, s);                                                          \
   }                                                                                                             \
   hipError_t step_##NAME(int storage, int collision, bool les, bool pull, const StepArgs& a, const void* a_in,  \
                          void* a_out, int block_x, hipStream_t s) {                                             \
