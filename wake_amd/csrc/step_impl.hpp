@@ -406,44 +406,66 @@ __device__ __forceinline__ void apply_inlet(const StepArgs& a, float (&f)[L::Q],
     const float rho = (s0 + 2.0f * sm) / (1.0f - ux);
     const float Ny = (ty - (2.0f / 3.0f) * rho * uy) / static_cast<float>(cnt_y);
     const float Nz = (tz - (2.0f / 3.0f) * rho * uz) / static_cast<float>(cnt_z);
-    // non-equilibrium bounce-back completion of the c_x = +1 populations
-    float g[L::Q];
-    static_for<L::Q>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      if constexpr (L::cx[q] == 1) {
-        const float cu = cdot<L, q>(ux, uy, uz);
-        g[q] = f[L::opp[q]] + 6.0f * L::w[q] * rho * cu - (cmul<L::cy[q]>(Ny) + cmul<L::cz[q]>(Nz));
-      } else {
-        g[q] = f[q];
-      }
-    });
-    // regularization: f = feq + 4.5 w (c c - I/3) : Pi_neq
+    // Non-equilibrium bounce-back completion of the c_x = +1 populations,
+    // g_p = f_opp(p) + 6 w rho (c_p . u) - (c_py N_y + c_pz N_z), then the
+    // regularization f = feq + 4.5 w (c c - I/3) : Pi_neq with
+    // Pi_neq = sum_q c c (g - feq).  Both are evaluated per opposite pair:
+    // c c is even in c, so only the pair sums of g and feq enter Pi_neq and
+    // both members share (c c - I/3) : Pi_neq; the pair's feq sum is
+    // 2 w rho (1 - 1.5 u^2 + 4.5 (c.u)^2) and its difference 6 w rho (c.u).
     const float usq = ux * ux + uy * uy + uz * uz;
-    float fe[L::Q];
+    const float K = 1.0f - 1.5f * usq;
+    float fe_s[L::Q], fe_d[L::Q];
     float pxx = 0.0f, pyy = 0.0f, pzz = 0.0f, pxy = 0.0f, pyz = 0.0f, pxz = 0.0f;
     static_for<L::Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
-      constexpr int a = L::cx[q], b = L::cy[q], c = L::cz[q];
-      fe[q] = feq<L, q>(rho, ux, uy, uz, usq);
-      const float n = g[q] - fe[q];
-      if constexpr (a != 0) pxx += n;
-      if constexpr (b != 0) pyy += n;
-      if constexpr (c != 0) pzz += n;
-      if constexpr (a * b == 1) pxy += n;
-      if constexpr (a * b == -1) pxy -= n;
-      if constexpr (b * c == 1) pyz += n;
-      if constexpr (b * c == -1) pyz -= n;
-      if constexpr (a * c == 1) pxz += n;
-      if constexpr (a * c == -1) pxz -= n;
+      constexpr int o = L::opp[q];
+      if constexpr (q > 0 && q < o) {
+        constexpr int a = L::cx[q], b = L::cy[q], c = L::cz[q];
+        const float wr = L::w[q] * rho;
+        const float cu = cdot<L, q>(ux, uy, uz);
+        fe_s[q] = (2.0f * wr) * ((4.5f * cu) * cu + K);
+        fe_d[q] = (6.0f * wr) * cu;
+        float sg;  // g_q + g_o
+        if constexpr (a == 1) {
+          sg = 2.0f * f[o] + (fe_d[q] - (cmul<b>(Ny) + cmul<c>(Nz)));
+        } else if constexpr (a == -1) {
+          sg = 2.0f * f[q] + (-fe_d[q] - (cmul<-b>(Ny) + cmul<-c>(Nz)));
+        } else {
+          sg = f[q] + f[o];
+        }
+        const float ns = sg - fe_s[q];
+        if constexpr (a != 0) pxx += ns;
+        if constexpr (b != 0) pyy += ns;
+        if constexpr (c != 0) pzz += ns;
+        if constexpr (a * b != 0) pxy += cmul<a * b>(ns);
+        if constexpr (b * c != 0) pyz += cmul<b * c>(ns);
+        if constexpr (a * c != 0) pxz += cmul<a * c>(ns);
+      }
     });
-    const float third = 1.0f / 3.0f;
+    const float tr3 = (pxx + pyy + pzz) * (1.0f / 3.0f);
+    {
+      const float wr = L::w[0] * rho;
+      f[0] = wr * K - (4.5f * L::w[0]) * tr3;
+    }
     static_for<L::Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
-      constexpr int a = L::cx[q], b = L::cy[q], c = L::cz[q];
-      const float t = ((static_cast<float>(a * a) - third) * pxx + (static_cast<float>(b * b) - third) * pyy +
-                       (static_cast<float>(c * c) - third) * pzz) +
-                      2.0f * (cmul<a * b>(pxy) + cmul<b * c>(pyz) + cmul<a * c>(pxz));
-      f[q] = fe[q] + 4.5f * L::w[q] * t;
+      constexpr int o = L::opp[q];
+      if constexpr (q > 0 && q < o) {
+        constexpr int a = L::cx[q], b = L::cy[q], c = L::cz[q];
+        float d = -tr3;
+        if constexpr (a != 0) d += pxx;
+        if constexpr (b != 0) d += pyy;
+        if constexpr (c != 0) d += pzz;
+        float od = 0.0f;
+        if constexpr (a * b != 0) od += cmul<a * b>(pxy);
+        if constexpr (b * c != 0) od += cmul<b * c>(pyz);
+        if constexpr (a * c != 0) od += cmul<a * c>(pxz);
+        const float t = d + 2.0f * od;
+        const float base = 0.5f * fe_s[q] + (4.5f * L::w[q]) * t;
+        f[q] = base + 0.5f * fe_d[q];
+        f[o] = base - 0.5f * fe_d[q];
+      }
     });
   }
 }
