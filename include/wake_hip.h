@@ -361,8 +361,15 @@ int wake_domain_set_temperature_field(wake_domain* d, const float* t);
 /* Anti-bounce-back wall temperature for the solid nodes in the box
  * (inclusive); NaN makes them adiabatic (bounce-back) again. */
 int wake_domain_set_wall_temperature(wake_domain* d, int x0, int y0, int x1, int y1, float t_wall);
-/* work-group width along x for the step kernel (64, 128, 256; 0 = auto) */
+/* work-group width along x for the step kernel (64, 128, 192, 256; 0 = auto) */
 int wake_domain_set_block_x(wake_domain* d, int block_x);
+/* Extension (no reference counterpart): on = 1 (default) lets
+ * wake_domain_run take 8 steps per launch on a fully periodic 2D domain
+ * whose cells are all Fluid, with no inlet, outlet, force probe or
+ * temperature, of up to 2^21 nodes (FP32) or 2^17 (16-bit storage), where
+ * single launches are bound by the launch rate; the result is the same as
+ * 8 single steps.  0 keeps one launch per step. */
+int wake_domain_set_multistep(wake_domain* d, int on);
 /* FP16 / BF16 storage: 1 (default) addresses the populations through a
  * buffer descriptor when the buffer is under 4 GiB; 0 always through 64-bit
  * pointers, as larger buffers are.  The results are the same. */

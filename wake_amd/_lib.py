@@ -114,6 +114,7 @@ DOMAIN_SIGS = {
     "wake_domain_count_nonfinite": ([P, P, P], I),
     "wake_domain_info": ([P, P], I),
     "wake_domain_set_block_x": ([P, I], I),
+    "wake_domain_set_multistep": ([P, I], I),
     "wake_hip_last_error": ([], ctypes.c_char_p),
     "wake_hip_clear_error": ([], I),
     "wake_hip_device_count": ([], I),

@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void k_mem_batch(const BatchSlot* __restrict__
     const int z = n / (a.nx * a.ny);
     float f[L::Q];
     form_s_at<L, St, false>(a, buf, x, y, z, n, a.cls[n], false, cur + 1, f);
-    mem_node<L>(a, x, y, z, f, a.mem_c[0], a.mem_c[1], a.mem_c[2], a.mem_rmax, s0, s1, s2);
+    mem_node<L>(a, x, y, z, n, f, a.mem_c[0], a.mem_c[1], a.mem_c[2], a.mem_rmax, s0, s1, s2);
   }
   block_sum3(s0, s1, s2);
   if (threadIdx.x == 0) {

@@ -317,7 +317,14 @@ int wake_domain_set_buffer_addressing(wake_domain* d, int enable) {
   return guard(d, [&](wake_hip::Domain& g) { g.set_buffer_addressing(enable != 0); });
 }
 int wake_domain_set_block_x(wake_domain* d, int block_x) {
-  return guard(d, [&](wake_hip::Domain& g) { g.set_block_x(block_x); });
+  return guard(d, [&](wake_hip::Domain& g) {
+    if (block_x != 0 && block_x != 64 && block_x != 128 && block_x != 192 && block_x != 256)
+      throw wake_hip::HipError("block_x must be 0, 64, 128, 192 or 256");
+    g.set_block_x(block_x);
+  });
+}
+int wake_domain_set_multistep(wake_domain* d, int on) {
+  return guard(d, [&](wake_hip::Domain& g) { g.set_multistep(on != 0); });
 }
 
 }  // extern "C"

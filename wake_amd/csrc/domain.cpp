@@ -719,6 +719,7 @@ void Domain::sync_device_config() {
     cls_dirty_ = true;
   }
   if (cls_dirty_) {
+    all_fluid_ = std::all_of(flags_.begin(), flags_.end(), [](uint8_t t) { return t == kFluid; });
     const StepArgs a = args(false);
     check(launch_build_class(lattice_, a, d_cls_, d_bmask_, stream_), "build class");
     check(launch_mark_bfl(d_bfl_nodes_, n_bfl_nodes_, d_cls_, d_aux_, stream_), "mark bfl");
@@ -743,6 +744,37 @@ void Domain::materialize() {
 }
 
 // ------------------------------------------------------------------ step
+bool Domain::tb_eligible() {
+  if (dim_ != 2 || lattice_ != 9 || thermal_ || probe_ || s_buf_ >= 0 || !all_fluid_ || !tb_enabled_) return false;
+  // Only where single-step launches are bound by the launch rate: measured,
+  // 8 steps per launch reach about 75 G node-steps/s (the tile loads and
+  // the 2.15x redundant halo updates bound it), while single steps of larger
+  // domains run at the HBM rate (FP32: 58.5k vs 71.3k MLUPS at 1024^2,
+  // 82.8k vs 77.5k at 4096^2; FP16: 92.3k vs 65.2k at 1024^2).
+  if (N_ > (storage_ == kStoreF32 ? (1u << 21) : (1u << 17))) return false;
+  const StepArgs a = args(true);
+  return a.periodic_mode && !a.inflow && !a.outflow;
+}
+
+void Domain::step_tb(int collision, float gx, float gy, bool use_les, float smag) {
+  version_ += kTbSteps;
+  StepArgs a = args(true);
+  a.tau = tau_;
+  a.gx = gx;
+  a.gy = gy;
+  a.gz = 0.0f;
+  a.cs = smag;
+  const int out = 1 - a_buf_;
+  check(launch_step_tb(lattice_, storage_, collision, use_les && les_coupled_, a, buf_[a_buf_], buf_[out], stream_),
+        "periodic step");
+  a_buf_ = out;
+  g_last_[0] = gx;
+  g_last_[1] = gy;
+  g_last_[2] = 0.0f;
+  step_count_ += kTbSteps;
+  invalidate_mirrors();
+}
+
 void Domain::step(int collision, float gx, float gy, float gz, bool use_les, float smag) {
   ++version_;
   sync_device_config();
@@ -827,7 +859,13 @@ float Domain::run(int steps, int collision, float gx, float gy, float gz, bool u
   kev_used_ = 0;
   timed_launches_ = 0;
   try {
-    for (int s = 0; s < steps; ++s) step(collision, gx, gy, gz, use_les, smag);
+    int s = 0;
+    // fully periodic D2Q9 without special nodes: kTbSteps steps per launch
+    // (an untimed run only: the kernel timer samples single-step launches)
+    if (!timed && steps >= kTbSteps && tb_eligible()) {
+      for (; steps - s >= kTbSteps; s += kTbSteps) step_tb(collision, gx, gy, use_les, smag);
+    }
+    for (; s < steps; ++s) step(collision, gx, gy, gz, use_les, smag);
   } catch (...) {
     time_kernels_ = false;
     throw;

@@ -90,6 +90,9 @@ class Domain {
 
   // --- stepping
   void step(int collision, float gx, float gy, float gz, bool use_les, float smag);
+  // run() takes kTbSteps steps per launch where tb_eligible() (fully periodic
+  // D2Q9 without special nodes, no force probe); off: one launch per step.
+  void set_multistep(bool on) { tb_enabled_ = on; }
   // `steps` steps; returns the device time in ms measured with events on the
   // domain's stream around the launches.
   float run(int steps, int collision, float gx, float gy, float gz, bool use_les, float smag, bool timed);
@@ -277,6 +280,10 @@ class Domain {
   uint8_t* d_flags_ = nullptr;
   uint8_t* d_cls_ = nullptr;
   uint32_t* d_bmask_ = nullptr;  // per node: bounce mask of the kClsSlow nodes
+  bool tb_eligible();
+  void step_tb(int collision, float gx, float gy, bool use_les, float smag);
+  bool all_fluid_ = false;       // every cell type Fluid (set with the class bytes)
+  bool tb_enabled_ = true;       // run() may take kTbSteps steps per launch
   int* d_aux_ = nullptr;
   BflNode* d_bfl_ = nullptr;
   int* d_bfl_nodes_ = nullptr;

@@ -30,10 +30,13 @@ enum ClassBits : uint8_t {
   kClsOutlet = 32  // Zou-He pressure or convective outlet node
 };
 
-// Per-node bounce mask of a kClsSlow node: bit q set when the pull source
-// along c_q is out of range (non-periodic axis), Solid or a MovingWall;
-// kBounceMovingWall set when one of them is a MovingWall.
+// Per-node bounce mask: bit q (q >= 1) set when the pull source along c_q
+// is out of range (non-periodic axis), Solid or a MovingWall, so that bit
+// opp(q) of an in-range neighbour x + c_q marks a wall link along c_q;
+// kBounceMovingWall set when one of them is a MovingWall; kBounceFluid set
+// when the node itself is Fluid.  All zero in periodic-stream mode.
 enum : uint32_t {
+  kBounceFluid = 1u,
   kBounceMovingWall = 1u << 31
 };
 
@@ -104,6 +107,13 @@ inline int mem_waves_per_row(int nx) { return (nx + 63) / 64; }
 
 // Launchers (step_kernels.hip).  `lattice` is 9, 19 or 27.
 hipError_t launch_build_class(int lattice, const StepArgs& a, uint8_t* cls, uint32_t* bmask, hipStream_t s);
+// Steps per launch of the fully periodic D2Q9 path (launch_step_tb).
+inline constexpr int kTbSteps = 8;
+// kTbSteps pull + collide steps of a fully periodic D2Q9 domain whose class
+// bytes are all zero (no solid, inlet or outlet node), A_{k-1} in `in` to
+// A_{k-1+kTbSteps} in `out`.
+hipError_t launch_step_tb(int lattice, int storage, int collision, bool les, const StepArgs& a, const void* in,
+                          void* out, hipStream_t s);
 hipError_t launch_mark_bfl(const int* nodes, int count, uint8_t* cls, int* aux, hipStream_t s);
 // pull=false: collide-only from the materialised S state.  block_x is the
 // work-group width along x (64, 128 or 256).

@@ -580,11 +580,16 @@ __device__ __forceinline__ void form_s(const StepArgs& a, const typename St::T* 
 // lies within r_max of the centre (r_max <= 0: all), as
 // forces_momentum_exchange.ispc:71-138 walks them.  on_link(qc, rx, ry) is
 // called per link with the midpoint offset from the centre.  The midpoint
-// test rounds as the host's link builder (no fused multiply-add).
+// test rounds as the host's link builder (no fused multiply-add).  n is the
+// node's index: with the bounce masks built, the walls come from its mask
+// (bit opp(q) of an in-range neighbour), not from reading the neighbours'
+// cell types.  Only for kClsSlow nodes (other nodes have no wall neighbour).
 template <class L, class OnLink>
-__device__ __forceinline__ void for_each_mem_link(const StepArgs& a, int x, int y, int z, float ccx, float ccy,
-                                                  float ccz, float r_max, OnLink&& on_link) {
+__device__ __forceinline__ void for_each_mem_link(const StepArgs& a, int x, int y, int z, int n, float ccx,
+                                                  float ccy, float ccz, float r_max, OnLink&& on_link) {
   const float r_max_sq = r_max * r_max;
+  const bool use_bm = a.bmask != nullptr;
+  const uint32_t bm = use_bm ? a.bmask[n] : 0u;
   static_for<L::Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     if constexpr (q > 0) {
@@ -595,9 +600,15 @@ __device__ __forceinline__ void for_each_mem_link(const StepArgs& a, int x, int 
       else in = in && ny_ >= 0 && ny_ < a.ny;
       if (a.per_z) nz_ = wrap(nz_, a.nz);
       else in = in && nz_ >= 0 && nz_ < a.nz;
-      const int nx_ = wrap(x + cx, a.nx);
-      const uint8_t t = in ? a.flags[(nz_ * a.ny + ny_) * a.nx + nx_] : static_cast<uint8_t>(kFluid);
-      if (t == kSolid || t == kMovingWall) {
+      bool wall;
+      if (use_bm) {
+        wall = in && ((bm >> L::opp[q]) & 1u);
+      } else {
+        const int nx_ = wrap(x + cx, a.nx);
+        const uint8_t t = in ? a.flags[(nz_ * a.ny + ny_) * a.nx + nx_] : static_cast<uint8_t>(kFluid);
+        wall = t == kSolid || t == kMovingWall;
+      }
+      if (wall) {
         const float rx = __fsub_rn(__fadd_rn(static_cast<float>(x), 0.5f * static_cast<float>(cx)), ccx);
         const float ry = __fsub_rn(__fadd_rn(static_cast<float>(y), 0.5f * static_cast<float>(cy)), ccy);
         const float rz = __fsub_rn(__fadd_rn(static_cast<float>(z), 0.5f * static_cast<float>(cz)), ccz);
@@ -609,12 +620,18 @@ __device__ __forceinline__ void for_each_mem_link(const StepArgs& a, int x, int 
   });
 }
 
+// Whether node n is Fluid (from its bounce mask when built).
+__device__ __forceinline__ bool is_fluid(const StepArgs& a, int n) {
+  return a.bmask ? (a.bmask[n] & kBounceFluid) != 0 : a.flags[n] == kFluid;
+}
+
 // F += 2 f_q c_q over the links of one Fluid node on its S_k values f; in 2D
 // the third sum is the torque about the centre at the link midpoint.
 template <class L>
-__device__ __forceinline__ void mem_node(const StepArgs& a, int x, int y, int z, const float (&f)[L::Q], float ccx,
-                                         float ccy, float ccz, float r_max, double& s0, double& s1, double& s2) {
-  for_each_mem_link<L>(a, x, y, z, ccx, ccy, ccz, r_max, [&](auto qc, float rx, float ry) {
+__device__ __forceinline__ void mem_node(const StepArgs& a, int x, int y, int z, int n, const float (&f)[L::Q],
+                                         float ccx, float ccy, float ccz, float r_max, double& s0, double& s1,
+                                         double& s2) {
+  for_each_mem_link<L>(a, x, y, z, n, ccx, ccy, ccz, r_max, [&](auto qc, float rx, float ry) {
     constexpr int q = decltype(qc)::value;
     constexpr int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
     const float dfx = 2.0f * f[q] * static_cast<float>(cx);
@@ -639,13 +656,15 @@ __device__ __forceinline__ double wave_sum(double v) {
 // the row's slot.  Every lane of the wave must take part (the sum is a
 // cross-lane exchange); x is the lane's node column (lane 0: 64 w).
 template <class L>
-__device__ __forceinline__ void mem_wave(const StepArgs& a, bool want, int x, int y, int z, const float (&f)[L::Q],
-                                         int slot) {
+__device__ __forceinline__ void mem_wave(const StepArgs& a, bool want, int x, int y, int z, int n,
+                                         const float (&f)[L::Q], int slot) {
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  if (want) mem_node<L>(a, x, y, z, f, a.mem_c[0], a.mem_c[1], a.mem_c[2], a.mem_rmax, s0, s1, s2);
-  s0 = wave_sum(s0);
-  s1 = wave_sum(s1);
-  s2 = wave_sum(s2);
+  if (__any(want)) {  // a wave without linked nodes writes zeros, as its sums are
+    if (want) mem_node<L>(a, x, y, z, n, f, a.mem_c[0], a.mem_c[1], a.mem_c[2], a.mem_rmax, s0, s1, s2);
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+  }
   if (__lane_id() == 0 && x < a.nx) {
     double* p = a.mem_part + 3ll * (slot + x / 64);
     p[0] = s0;
@@ -666,6 +685,7 @@ __global__ void k_build_class(const StepArgs a, uint8_t* __restrict__ cls, uint3
   uint32_t mask = 0;
   if (t == kSolid) c |= a.periodic_mode ? kClsSolid : (kClsSolid | kClsFrozen);
   if (!a.periodic_mode && t != kSolid) {
+    if (t == kFluid) mask |= kBounceFluid;
     static_for<L::Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       if constexpr (q > 0) {
@@ -683,7 +703,7 @@ __global__ void k_build_class(const StepArgs a, uint8_t* __restrict__ cls, uint3
         }
       }
     });
-    if (mask) c |= kClsSlow;
+    if (mask & ~kBounceFluid) c |= kClsSlow;
   }
   if (bmask) bmask[n] = mask;
   const bool in_rows = y >= a.by0 && y < a.by1 && z >= a.bz0 && z < a.bz1;
@@ -764,7 +784,7 @@ __global__ __launch_bounds__(256) WAKE_STEP_ATTR void k_step(const StepArgs a, c
   if constexpr (PULL && WAKE_ZK == 1) {
     if (mem_slot >= 0) {
       // every lane of the row's waves takes part in the sums
-      mem_wave<L>(a, live && (cn & kClsSlow) && a.flags[nn] == kFluid, x, y, z0, fn, mem_slot);
+      mem_wave<L>(a, live && (cn & kClsSlow) && is_fluid(a, nn), x, y, z0, nn, fn, mem_slot);
       if (!live) return;
     }
   }
@@ -818,6 +838,91 @@ __global__ __launch_bounds__(256) WAKE_STEP_ATTR void k_step(const StepArgs a, c
         St::template st<L, q>(aout, q * S + n, f[q]);
       });
     }
+  }
+}
+
+// ------------------------------------------- fully periodic D2Q9, T steps
+// A fully periodic D2Q9 domain without solid, inlet or outlet nodes (every
+// class byte zero) streams by a plain periodic shift, so kTbSteps steps can
+// run in one launch: each work-group loads a (16 + 2 T)^2 tile of A_{k-1}
+// around its 16 x 16 nodes into LDS, pull-collides T times in LDS (the
+// valid region shrinks by one node per step), and stores A_{k-1+T} of its
+// 16 x 16 nodes.  One thread per tile node (1,024 threads, 16 waves), so
+// each SIMD holds four of the work-group's waves to overlap LDS latency.  Every node update is the same pull, collision and storage
+// rounding as one k_step launch, so the result is the same as T launches;
+// the launch cost and the HBM traffic are paid once per T steps.
+inline constexpr int kTbTile = 16;
+
+template <class St, class L, int q>
+__device__ __forceinline__ float tb_get(typename St::T v) {
+  if constexpr (sizeof(typename St::T) == 4) {
+    return v;
+  } else {
+    return St::template from_dev_bits<L, q>(__builtin_bit_cast(unsigned short, v));
+  }
+}
+template <class St, class L, int q>
+__device__ __forceinline__ typename St::T tb_put(float v) {
+  if constexpr (sizeof(typename St::T) == 4) {
+    return v;
+  } else {
+    return __builtin_bit_cast(typename St::T, St::template dev_bits<L, q>(v));
+  }
+}
+
+inline constexpr int kTbWidth = kTbTile + 2 * kTbSteps;  // 32
+inline constexpr int kTbThreads = kTbWidth * kTbWidth;     // one per tile node
+
+template <class St, int COLL, bool LES>
+__global__ __launch_bounds__(kTbThreads) void k_step_tb2d(const StepArgs a, const typename St::T* __restrict__ ain,
+                                                          typename St::T* __restrict__ aout) {
+  using L = D2Q9;
+  using T = typename St::T;
+  constexpr int W = kTbWidth;
+  static_assert((W & (W - 1)) == 0, "tile width");
+  __shared__ T tile[2][L::Q][W * W];
+  const int x0 = blockIdx.x * kTbTile - kTbSteps, y0 = blockIdx.y * kTbTile - kTbSteps;
+  const long long S = a.stride;
+  const int c = threadIdx.x;  // this thread's tile node
+  const int i = c & (W - 1), j = c / W;
+  {
+    const int gx = ((x0 + i) % a.nx + a.nx) % a.nx;
+    const int gy = ((y0 + j) % a.ny + a.ny) % a.ny;
+    const long long n = static_cast<long long>(gy) * a.nx + gx;
+    static_for<L::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      tile[0][q][c] = ain[q * S + n];
+    });
+  }
+  __syncthreads();
+#pragma unroll 1
+  for (int s = 1; s <= kTbSteps; ++s) {
+    const int src = (s - 1) & 1, dst = s & 1;
+    if (i >= s && i < W - s && j >= s && j < W - s) {
+      float f[L::Q];
+      static_for<L::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        f[q] = tb_get<St, L, q>(tile[src][q][(j - L::cy[q]) * W + (i - L::cx[q])]);
+      });
+      Macro m;
+      if constexpr (LES) m = collide_les<L, COLL, St::kFast>(f, a.tau, a.cs, a.gx, a.gy, a.gz);
+      else m = collide<L, COLL, false, St::kFast>(f, a.tau, a.gx, a.gy, a.gz);
+      flag_nonfinite(a, m.rho);
+      static_for<L::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        tile[dst][q][c] = tb_put<St, L, q>(f[q]);
+      });
+    }
+    __syncthreads();
+  }
+  const int li = i - kTbSteps, lj = j - kTbSteps;
+  const int gx = blockIdx.x * kTbTile + li, gy = blockIdx.y * kTbTile + lj;
+  if (li >= 0 && li < kTbTile && lj >= 0 && lj < kTbTile && gx < a.nx && gy < a.ny) {
+    const long long n = static_cast<long long>(gy) * a.nx + gx;
+    static_for<L::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      aout[q * S + n] = tile[kTbSteps & 1][q][c];
+    });
   }
 }
 
@@ -916,7 +1021,7 @@ template <class L>
 __device__ __forceinline__ bool has_mem_link(const StepArgs& a, int x, int y, int z, int n) {
   if (!(a.cls[n] & kClsSlow) || a.flags[n] != kFluid) return false;
   bool any = false;
-  for_each_mem_link<L>(a, x, y, z, a.mem_c[0], a.mem_c[1], a.mem_c[2], a.mem_rmax,
+  for_each_mem_link<L>(a, x, y, z, n, a.mem_c[0], a.mem_c[1], a.mem_c[2], a.mem_rmax,
                        [&](auto, float, float) { any = true; });
   return any;
 }
@@ -971,7 +1076,7 @@ __global__ __launch_bounds__(256) void k_mem_rows(const StepArgs a, const typena
     const int y = row % a.ny, z = row / a.ny;
     const int n = (z * a.ny + y) * a.nx + (live ? x : 0);
     const uint8_t c = live ? a.cls[n] : static_cast<uint8_t>(0);
-    const bool want = live && (c & kClsSlow) && a.flags[n] == kFluid;
+    const bool want = live && (c & kClsSlow) && is_fluid(a, n);
     float f[L::Q];
     static_for<L::Q>([&](auto qc) { f[decltype(qc)::value] = 0.0f; });
     if (want) {
@@ -984,7 +1089,7 @@ __global__ __launch_bounds__(256) void k_mem_rows(const StepArgs a, const typena
         form_s_at<L, St, false>(a, buf, x, y, z, n, c, false, a.s_step, f);
       }
     }
-    mem_wave<L>(a, want, x, y, z, f, a.mem_rows[row]);
+    mem_wave<L>(a, want, x, y, z, n, f, a.mem_rows[row]);
   }
 }
 

@@ -72,6 +72,12 @@ hipError_t launch_build_class(int lattice, const StepArgs& a, uint8_t* cls, uint
   }
 }
 
+hipError_t launch_step_tb(int lattice, int storage, int collision, bool les, const StepArgs& a, const void* in,
+                          void* out, hipStream_t s) {
+  if (lattice != 9) return hipErrorInvalidValue;
+  return detail::step_tb_d2q9(storage, collision, les, a, in, out, s);
+}
+
 hipError_t launch_step(int lattice, int storage, int collision, bool les, bool pull, const StepArgs& a,
                        const void* a_in, void* a_out, int block_x, hipStream_t s) {
   switch (lattice) {

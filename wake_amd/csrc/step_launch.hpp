@@ -96,6 +96,30 @@ struct Launch {
     });
   }
 
+  static hipError_t step_tb(int storage, int collision, bool les, const StepArgs& a, const void* a_in, void* a_out,
+                            hipStream_t s) {
+    if constexpr (L::Q != 9 || L::D != 2) {
+      return hipErrorInvalidValue;
+    } else {
+      const dim3 g((a.nx + kTbTile - 1) / kTbTile, (a.ny + kTbTile - 1) / kTbTile, 1);
+      return with_storage(storage, [&](auto st) {
+        using St = decltype(st);
+        using T = typename St::T;
+        const T* i = static_cast<const T*>(a_in);
+        T* o = static_cast<T*>(a_out);
+        auto go = [&](auto cc) {
+          constexpr int C = decltype(cc)::value;
+          if (les) k_step_tb2d<St, C, true><<<g, kTbThreads, 0, s>>>(a, i, o);
+          else k_step_tb2d<St, C, false><<<g, kTbThreads, 0, s>>>(a, i, o);
+        };
+        if (collision == kBGK) go(std::integral_constant<int, kBGK>{});
+        else if (collision == kTRT) go(std::integral_constant<int, kTRT>{});
+        else go(std::integral_constant<int, kMRT>{});
+        return hipGetLastError();
+      });
+    }
+  }
+
   static hipError_t mem_mark_rows(const StepArgs& a, int* row_has, hipStream_t s) {
     const int bx = pick_block(a.nx, 0);
     k_mem_mark_rows<L><<<grid_of(a, bx), bx, 0, s>>>(a, row_has);
@@ -167,6 +191,8 @@ struct Launch {
   hipError_t prev_from_s_##NAME(int storage, const StepArgs& a, const void* s_state, hipStream_t s);             \
   hipError_t readback_##NAME(int storage, bool from_s, const StepArgs& a, const void* buf, hipStream_t s);      \
   hipError_t mem_mark_rows_##NAME(const StepArgs& a, int* row_has, hipStream_t s);                             \
+  hipError_t step_tb_##NAME(int storage, int collision, bool les, const StepArgs& a, const void* a_in,          \
+                            void* a_out, hipStream_t s);                                                       \
   hipError_t mem_row_nodes_##NAME(const StepArgs& a, const int* row_list, int nrows, int* counts,               \
                                   const int* offsets, int* nodes, hipStream_t s);                              \
   hipError_t mem_rows_##NAME(int storage, bool from_s, const StepArgs& a, const void* buf, const int* row_list, \
@@ -191,6 +217,10 @@ struct Launch {
   }                                                                                                             \
   hipError_t mem_mark_rows_##NAME(const StepArgs& a, int* row_has, hipStream_t s) {                            \
     return Launch<LAT>::mem_mark_rows(a, row_has, s);                                                           \
+  }                                                                                                             \
+  hipError_t step_tb_##NAME(int storage, int collision, bool les, const StepArgs& a, const void* a_in,          \
+                            void* a_out, hipStream_t s) {                                                       \
+    return Launch<LAT>::step_tb(storage, collision, les, a, a_in, a_out, s);                                    \
   }                                                                                                             \
   hipError_t mem_row_nodes_##NAME(const StepArgs& a, const int* row_list, int nrows, int* counts,               \
                                   const int* offsets, int* nodes, hipStream_t s) {                             \

@@ -158,6 +158,14 @@ class _Domain:
     def set_block_x(self, bx):
         self._c(self._L.wake_domain_set_block_x(self._h, bx), "set_block_x")
 
+    def set_multistep(self, on=True):
+        """Let run() take 8 steps per launch where the domain allows it
+        (fully periodic 2D, all cells Fluid, no open boundary, force probe
+        or temperature) and single launches are bound by the launch rate
+        (up to 2^21 nodes in FP32, 2^17 with 16-bit storage); the result is
+        the same as single steps."""
+        self._c(self._L.wake_domain_set_multistep(self._h, int(bool(on))), "set_multistep")
+
     # --------------------------------------------------------------- steps
     def run(self, steps, collision=CollisionType.MRT, gx=0.0, gy=0.0, gz=0.0, use_les=False,
             smag=DEFAULT_SMAGORINSKY_CONSTANT, timed=False):
