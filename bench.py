@@ -401,11 +401,12 @@ def main_wing(args):
 
 def main_tg2d(args):
     """D2Q9 BGK Taylor-Green vortex, 256^2 fully periodic, tau 0.8, u0 0.04,
-    equilibrium plus seeded uniform noise of 1e-6, 1000 steps: the plain
-    single-domain step (one launch per step) and the same domain as a batch
-    of one (graph-captured chunks of 100 launches), against the oracle on
-    the host.  The working set (2 x 2.36 MB) stays in the on-chip caches, so
-    this is not an HBM-roofline figure."""
+    equilibrium plus seeded uniform noise of 1e-6, 1000 steps: the domain's
+    run() (8 steps per launch from an LDS tile, as a fully periodic domain of
+    this size takes), the same with one launch per step, and the domain as a
+    batch of one (graph-captured chunks of 100 launches), against the oracle
+    on the host.  The working set (2 x 2.36 MB) stays in the on-chip caches,
+    so this is not an HBM-roofline figure."""
     import numpy as np
 
     import wake_amd
@@ -424,18 +425,26 @@ def main_tg2d(args):
         g.add_noise(1e-6, 1)
         return g
 
+    def timed_run(g):
+        g.run(16, "BGK")
+        g.synchronize()
+        t0 = time.perf_counter()
+        g.run(S, "BGK")
+        g.synchronize()
+        return time.perf_counter() - t0
+
+    s1 = grid()
+    s1.set_multistep(False)
+    wall_1 = timed_run(s1)
+    mlups_1 = n * n * S / wall_1 / 1e6
+    del s1
     g = grid()
     e0 = g.compute_total_kinetic_energy()
-    g.run(10, "BGK")
-    g.synchronize()
-    t0 = time.perf_counter()
-    g.run(S, "BGK")
-    g.synchronize()
-    wall = time.perf_counter() - t0
+    wall = timed_run(g)
     mlups = n * n * S / wall / 1e6
     nu = (tau - 0.5) / 3.0
     ratio = g.compute_total_kinetic_energy() / e0
-    expect = math.exp(-4 * nu * k * k * (S + 10))
+    expect = math.exp(-4 * nu * k * k * (S + 16))
     h = grid()
     b = wake_amd.Batch(lattice=9, collision="BGK", chunk=100)
     b.add(h, S + 10)
@@ -444,7 +453,8 @@ def main_tg2d(args):
     b.run(S)
     wall_b = time.perf_counter() - t0
     mlups_b = n * n * S / wall_b / 1e6
-    log(f"TG 256^2 BGK: plain {mlups:.0f} MLUPS ({wall / S * 1e6:.2f} us/step), batch of one (graph) "
+    log(f"TG 256^2 BGK: run() {mlups:.0f} MLUPS ({wall / S * 1e6:.2f} us/step), one launch per step {mlups_1:.0f} "
+        f"MLUPS ({wall_1 / S * 1e6:.2f} us/step), batch of one (graph) "
         f"{mlups_b:.0f} MLUPS ({wall_b / S * 1e6:.2f} us/step); KE ratio {ratio:.5f} vs exp(-4 nu k^2 t) {expect:.5f}")
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind as ob
@@ -453,21 +463,23 @@ def main_tg2d(args):
     o.set_periodic(True)
     o.initialize_fields(rho, ux, uy)
     o.add_noise(1e-6, 1)
-    o.run(10, "BGK")
+    o.run(16, "BGK")
     t0 = time.perf_counter()
     o.run(S, "BGK")
     cpu = n * n * S / (time.perf_counter() - t0) / 1e6
     err = float(np.linalg.norm(np.concatenate([g.ux_data(), g.uy_data()]) - np.concatenate([o.ux(), o.uy()]))
                 / np.linalg.norm(np.concatenate([o.ux(), o.uy()])))
     log(f"oracle on the host: {cpu:.0f} MLUPS on {ob.lib(32).oracle_threads()} threads; GPU vs oracle u rel L2 {err:.2e}")
-    best, best_wall, path = max((mlups, wall, "plain"), (mlups_b, wall_b, "batch_of_one_graph"))
+    best, best_wall, path = max((mlups, wall, "run_8_steps_per_launch"), (mlups_1, wall_1, "one_launch_per_step"),
+                                (mlups_b, wall_b, "batch_of_one_graph"))
     out = {
-        "metric": METRIC, "value": round(best, 1), "unit": "MLUPS", "n_gpus": 1, "steps": S, "warmup": 10,
+        "metric": METRIC, "value": round(best, 1), "unit": "MLUPS", "n_gpus": 1, "steps": S, "warmup": 16,
         "ms_per_step": round(best_wall * 1e3 / S, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32", "data": "synthetic",
         "config": {"workload": "D2Q9 BGK Taylor-Green vortex 256^2 periodic, tau 0.8, u0 0.04, noise 1e-6 (seed 1)",
                    "value_path": path,
-                   "plain_mlups": round(mlups, 1), "batch_of_one_graph_mlups": round(mlups_b, 1),
+                   "run_mlups": round(mlups, 1), "one_launch_per_step_mlups": round(mlups_1, 1),
+                   "batch_of_one_graph_mlups": round(mlups_b, 1),
                    "on_chip": "2 x 2.36 MB working set: cache-resident, not an HBM-roofline figure",
                    "ke_ratio": round(ratio, 6), "ke_ratio_analytic": round(expect, 6),
                    "u_rel_l2_vs_oracle": err, "cpu_oracle_mlups": round(cpu, 1),
