@@ -245,6 +245,12 @@ def batched_record(args):
         cds.append(float(np.mean(tail[:, 0])) / q)
         f = analysis.analyze_frequency((F[-4096:, 1] / q).astype(np.float32), 1.0)
         sts.append(analysis.strouhal_number(f, D, U))
+    from wake_amd.hbm_probe import ceiling_d2q9
+
+    pull9 = ceiling_d2q9()
+    bpn9 = 72.0 if storage == 0 else 36.0
+    gbps = bpn9 * nodes * left / (ms * 1e-3) / 1e9  # over the device time of the chunks (forces included)
+    log(f"D2Q9 pull copy on this GPU: {pull9:.0f} GB/s (FP32); batched steps {gbps:.0f} GB/s over the device time")
     log(f"C_D median {np.median(cds):.3f} [{np.min(cds):.3f}, {np.max(cds):.3f}], "
         f"St median {np.median(sts):.3f}, non-finite domains {bad}; {ndom / t_batch:.3f} sims/s batched vs "
         f"{ndom / t_plain:.3f} plain ({t_plain / t_batch:.2f}x)")
@@ -260,6 +266,8 @@ def batched_record(args):
                         f"force of every step, one launch per step over all domains, graph chunks of 100 steps",
             "nodes": nodes, "launch_shape": {"block": shape[0], "grid_factor": shape[1]},
             "batched_mlups": round(mlups, 1), "plain_sims_per_s": round(ndom / t_plain, 4),
+            "batched_gbps": round(gbps, 1), "pull_d2q9_gbps": round(pull9, 1),
+            "frac_of_pull_same_width": round(gbps / pull9, 4) if storage == 0 else None,
             "speedup_vs_plain": round(t_plain / t_batch, 3),
             "plain": "single-domain step, one domain at a time, per-step force inside its step launch, wall time "
                      f"of a {probe}-step run per domain scaled to {S} steps",

@@ -59,9 +59,11 @@ void wake_hip_d3q19_minv(float out[361]);
  *   shape 2: the same pull with 16-bit values, the FP16 / BF16 step's
  *            access (76 bytes per node);
  *   shapes 3, 4: the D3Q27 pull over 27 planes, floats (216 bytes per node)
- *            and 16-bit values (108 bytes per node).
- * Allocates and frees two buffers of 76 (shapes 3, 4: 108) * nx * ny * nz
- * bytes.  Needs
+ *            and 16-bit values (108 bytes per node);
+ *   shape 5: the D2Q9 pull over 9 planes of floats (72 bytes per node), nz
+ *            independent x-y layers.
+ * Allocates and frees two buffers of 76 (shapes 3, 4: 108; shape 5: 36) *
+ * nx * ny * nz bytes.  Needs
  * 1 <= nx <= 1024 and 1 <= nz <= 65535; returns 0 or an error code. */
 int wake_hip_hbm_probe(int shape, int nx, int ny, int nz, int reps, float* ms_per_launch);
 

@@ -66,7 +66,7 @@ __global__ __launch_bounds__(1024) void k_probe_pull(const T* __restrict__ a, T*
 }
 
 hipError_t run_probe(int shape, int nx, int ny, int nz, int reps, float* ms_out) {
-  const long long count = (shape >= 3 ? 27LL : 19LL) * nx * ny * nz;
+  const long long count = (shape == 5 ? 9LL : (shape >= 3 ? 27LL : 19LL)) * nx * ny * nz;
   float* a = nullptr;  // count floats; the 16-bit shapes use the first count halves
   float* b = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -86,9 +86,11 @@ hipError_t run_probe(int shape, int nx, int ny, int nz, int reps, float* ms_out)
                                                                 reinterpret_cast<unsigned short*>(b), nx, ny, nz);
     } else if (shape == 3) {
       k_probe_pull<D3Q27, float><<<dim3(ny, nz), nx>>>(a, b, nx, ny, nz);
-    } else {
+    } else if (shape == 4) {
       k_probe_pull<D3Q27, unsigned short><<<dim3(ny, nz), nx>>>(reinterpret_cast<const unsigned short*>(a),
                                                                 reinterpret_cast<unsigned short*>(b), nx, ny, nz);
+    } else {
+      k_probe_pull<D2Q9, float><<<dim3(ny, nz), nx>>>(a, b, nx, ny, nz);
     }
   };
   if (err == hipSuccess) {
@@ -117,7 +119,7 @@ hipError_t run_probe(int shape, int nx, int ny, int nz, int reps, float* ms_out)
 
 extern "C" int wake_hip_hbm_probe(int shape, int nx, int ny, int nz, int reps, float* ms_per_launch) {
   if (!ms_per_launch) return wake_hip::set_last_error("wake_hip_hbm_probe: null output");
-  if (shape < 0 || shape > 4) return wake_hip::set_last_error("wake_hip_hbm_probe: shape must be 0 to 4");
+  if (shape < 0 || shape > 5) return wake_hip::set_last_error("wake_hip_hbm_probe: shape must be 0 to 5");
   if (nx < 1 || ny < 1 || nz < 1 || nx > 1024 || reps < 1)
     return wake_hip::set_last_error("wake_hip_hbm_probe: need 1 <= nx <= 1024, ny, nz >= 1 and reps >= 1");
   if (nz > 65535) return wake_hip::set_last_error("wake_hip_hbm_probe: nz above the 65,535 grid limit");

@@ -10,6 +10,18 @@ import ctypes
 BYTES_PER_NODE = 152  # 19 floats read and written per node (FP32)
 
 
+def ceiling_d2q9(nx=1024, ny=256, nz=128, reps=20):
+    """The D2Q9 pull over 9 SoA planes of floats (72 bytes per node) on the
+    current GPU, in GB/s, over nz independent nx x ny layers."""
+    from . import _lib
+
+    L = _lib.lib()
+    ms = ctypes.c_float()
+    if L.wake_hip_hbm_probe(5, nx, ny, nz, reps, ctypes.byref(ms)) != 0:
+        raise RuntimeError(L.wake_hip_last_error().decode())
+    return 72 * nx * ny * nz / (ms.value * 1e-3) / 1e9
+
+
 def ceilings_d3q27(nx=384, ny=192, nz=192, reps=20):
     """The D3Q27 pull over 27 SoA planes on the current GPU, in GB/s:
     floats (216 bytes per node) and 16-bit values (108 bytes per node)."""
