@@ -57,8 +57,8 @@ void oracle_set_threads(int n) { (void)n; }
  * Lattice tables.  include/wake/core/lattice_models.hpp:14-110 and the ISPC
  * copies include/wake/ispc/lbm_d2q9.isph:8-17, lbm_d3q19.isph:8-28,
  * lbm_d3q27.isph:8-42.  The direction order is the population-plane order.
- * pinned: test_oracle_tables (opposite involution, sum w = 1, sum w c = 0,
- * sum w c c = 1/3 delta).
+ * pinned: test_oracle_tables (the velocity sets, opposite involution with
+ * c_opp = -c, sum w = 1, sum w c = 0, sum w c c = 1/3 delta).
  * ---------------------------------------------------------------------- */
 static const int c9x[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
 static const int c9y[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
@@ -115,7 +115,7 @@ static lat_t lattice_of(int lattice) {
 /* Counter-based RNG for seeded noise and inlet turbulence (no reference
  * counterpart: the reference has no RNG).  splitmix64 finaliser of
  * seed ^ counter, top 24 bits to [0,1).  The product uses the same published
- * splitmix64 constants. pinned: test_oracle_rng (known splitmix64 outputs). */
+ * splitmix64 constants. pinned: test_oracle_rng_* (known splitmix64 outputs). */
 uint64_t oracle_rng_splitmix64(uint64_t x) {
   x += 0x9E3779B97F4A7C15ull;
   x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -131,8 +131,11 @@ double oracle_rng_u01(uint64_t seed, uint64_t counter) {
  * src/ispc/d3q19_collision_mrt.ispc:66-105 (explicit sums).  The inverse is
  * the reference's table d3q19_Minv (d3q19_collision_mrt.ispc:4-24), which is
  * inv(M) rounded to 7 decimals; it is regenerated here from M rather than
- * copied.  pinned: test_oracle_mrt_table (round(inv(M),7) == table
- * characteristics, M*Minv ~ I, mass-drift signature at rest).
+ * copied.  pinned: test_oracle_mrt_forward_matches_dhumieres_basis,
+ * test_oracle_mrt_inverse_is_rounded_inverse,
+ * test_oracle_mrt_inverse_equals_reference_literals (round(inv(M),7) == the
+ * reference's table),
+ * test_oracle_mrt_mass_drift_signature.
  * ---------------------------------------------------------------------- */
 static void d3q19_forward(const real* f, real* m) {
   m[0] = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8] + f[9] + f[10] + f[11] + f[12] + f[13] +
@@ -218,7 +221,7 @@ void oracle_d3q19_m(real out[19 * 19]) {
  * D2Q9
  * ===================================================================== */
 
-/* src/ispc/d2q9_moments.ispc:4-32.  pinned: test_oracle_moments (analytic
+/* src/ispc/d2q9_moments.ispc:4-32.  pinned: test_oracle_moments_* (analytic
  * feq round trip from tests/unit/test_d2q9_moments.cpp:28-61). */
 void oracle_d2q9_compute_macroscopic_moments(const real* f, real* rho, real* ux, real* uy, int count) {
   for (int i = 0; i < count; ++i) {
@@ -235,7 +238,7 @@ void oracle_d2q9_compute_macroscopic_moments(const real* f, real* rho, real* ux,
   }
 }
 
-/* src/ispc/d2q9_moments.ispc:35-65.  pinned: test_oracle_moments (g shift). */
+/* src/ispc/d2q9_moments.ispc:35-65.  pinned: test_oracle_moments_* (g shift). */
 void oracle_d2q9_compute_macroscopic_moments_forced(const real* f, real* rho, real* ux, real* uy, real gx,
                                                     real gy, int count) {
   for (int i = 0; i < count; ++i) {
@@ -252,7 +255,7 @@ void oracle_d2q9_compute_macroscopic_moments_forced(const real* f, real* rho, re
   }
 }
 
-/* src/ispc/d2q9_moments.ispc:68-79. pinned: test_oracle_moments. */
+/* src/ispc/d2q9_moments.ispc:68-79. pinned: test_oracle_moments_*. */
 void oracle_d2q9_compute_velocity_magnitude(const real* ux, const real* uy, real* speed, int count) {
   for (int i = 0; i < count; ++i) speed[i] = RSQRT(ux[i] * ux[i] + uy[i] * uy[i]);
 }
@@ -261,7 +264,7 @@ void oracle_d2q9_compute_velocity_magnitude(const real* ux, const real* uy, real
  * LES-coupled EXTENSION (per-node shear tau); otherwise the scalar tau of the
  * ISPC signature is used. */
 
-/* src/ispc/d2q9_collision_bgk.ispc:5-86. pinned: test_oracle_collision
+/* src/ispc/d2q9_collision_bgk.ispc:5-86. pinned: test_oracle_collision_*
  * (conservation, solid copy from tests/unit/test_d2q9_collision.cpp;
  * (1-omega) contraction of f-feq; Poiseuille). */
 static void d2q9_bgk_node(const real* f_in, real* f_out, size_t n, size_t i, real* rho, real* ux, real* uy,
@@ -302,7 +305,7 @@ void oracle_d2q9_collide_bgk(const real* f_in, real* f_out, const uint8_t* cell_
   }
 }
 
-/* src/ispc/d2q9_collision_trt.ispc:7-92. pinned: test_oracle_collision
+/* src/ispc/d2q9_collision_trt.ispc:7-92. pinned: test_oracle_collision_*
  * (conservation; TRT with tau_a == tau_s equals BGK). */
 static void d2q9_trt_node(const real* f_in, real* f_out, size_t n, size_t i, real* rho, real* ux, real* uy,
                           real tau_s, real tau_a, real gx, real gy) {
@@ -355,7 +358,7 @@ void oracle_d2q9_collide_trt(const real* f_in, real* f_out, const uint8_t* cell_
 }
 
 /* src/ispc/d2q9_collision_mrt.ispc:6-137 (Lallemand-Luo basis, exact
- * fraction inverse).  pinned: test_oracle_collision (conservation; all rates
+ * fraction inverse).  pinned: test_oracle_collision_* (conservation; all rates
  * 1 except conserved gives f' = feq up to the m2/m1 closure; TG decay). */
 static void d2q9_mrt_node(const real* f_in, real* f_out, size_t n, size_t i, real* rho, real* ux, real* uy,
                           real tau_shear, real tau_bulk, real gx, real gy) {
@@ -429,7 +432,7 @@ void oracle_d2q9_collide_mrt(const real* f_in, real* f_out, const uint8_t* cell_
   }
 }
 
-/* src/ispc/d2q9_streaming.ispc:4-29.  pinned: test_oracle_streaming
+/* src/ispc/d2q9_streaming.ispc:4-29.  pinned: test_oracle_streaming_*
  * (permutation; a delta moves by c_q). */
 void oracle_d2q9_stream_periodic(const real* f_src, real* f_dst, int nx, int ny) {
   size_t total = (size_t)nx * ny;
@@ -449,8 +452,8 @@ void oracle_d2q9_stream_periodic(const real* f_src, real* f_dst, int nx, int ny)
 
 /* src/ispc/d2q9_streaming.ispc:33-95 (x periodic, y walls, solid halfway
  * bounce-back, moving-wall momentum).  With per_y = 1 (EXTENSION) y wraps
- * instead of reflecting.  pinned: test_oracle_streaming (closed-box mass
- * conservation), test_oracle_poiseuille (analytic profile, reference
+ * instead of reflecting.  pinned: test_oracle_streaming_* (closed-box mass
+ * conservation), test_oracle_poiseuille_* (analytic profile, reference
  * tests/validation/test_poiseuille_convergence.cpp). */
 void oracle_d2q9_stream_general(const real* f_src, real* f_dst, const uint8_t* cell_type, int nx, int ny,
                                 int per_y, real uw_x, real uw_y) {
@@ -510,7 +513,7 @@ void oracle_d2q9_apply_bounce_back(real* f, const uint8_t* cell_type, real uw_x,
   }
 }
 
-/* src/ispc/d2q9_boundaries.ispc:43-81.  pinned: test_oracle_zou_he (after the
+/* src/ispc/d2q9_boundaries.ispc:43-81.  pinned: test_oracle_zou_he_* (after the
  * update sum f c_x = rho_in u_x and sum f = rho_in at every inlet node). */
 static void d2q9_zh_inflow_node(real* f, size_t total, size_t idx, real u_in_x, real u_in_y, real* rho_o) {
   real f0 = f[0 * total + idx], f2 = f[2 * total + idx], f3 = f[3 * total + idx];
@@ -535,7 +538,7 @@ void oracle_d2q9_apply_zou_he_inflow_west(real* f, real* rho, real* ux, real* uy
   }
 }
 
-/* src/ispc/d2q9_boundaries.ispc:85-121.  pinned: test_oracle_zou_he (sum f =
+/* src/ispc/d2q9_boundaries.ispc:85-121.  pinned: test_oracle_zou_he_* (sum f =
  * rho_out at every outlet node). */
 void oracle_d2q9_apply_zou_he_outflow_east(real* f, real* rho, real* ux, real* uy, real rho_out, int nx,
                                            int ny) {
@@ -604,7 +607,7 @@ void oracle_d2q9_compute_smagorinsky_tau(const real* f, const real* rho, const r
 /* src/ispc/forces_momentum_exchange.ispc:6-68.  Sums are accumulated in
  * double (the reference sums FP32 lanes with a gang-width-dependent
  * reduce_add; the order is not part of its contract).  pinned:
- * test_oracle_mem (rest equilibrium gives 0; +0.05 on f1 at (6,7) gives
+ * test_oracle_mem_* (rest equilibrium gives 0; +0.05 on f1 at (6,7) gives
  * F_x = 0.1, tests/unit/test_forces_momentum_exchange.cpp:33-68). */
 static void d2q9_mem_d(const real* f, const uint8_t* cell_type, int nx, int ny, real center_x, real center_y,
                        real max_radius, double* out) {
@@ -650,7 +653,7 @@ void oracle_d2q9_compute_momentum_exchange_forces(const real* f, const uint8_t* 
  * ===================================================================== */
 
 /* src/ispc/d3q19_moments.ispc:4-66, d3q27_moments.ispc:4-32.  pinned:
- * test_oracle_moments (analytic feq from tests/unit/test_d3q19_moments.cpp). */
+ * test_oracle_moments_* (analytic feq from tests/unit/test_d3q19_moments.cpp). */
 static void moments3d(const lat_t* L, const real* f, real* rho, real* ux, real* uy, real* uz, real gx, real gy,
                       real gz, int forced, int count) {
   size_t n = (size_t)count;
@@ -698,7 +701,7 @@ void oracle_d3q19_compute_velocity_magnitude(const real* ux, const real* uy, con
   for (int i = 0; i < count; ++i) speed[i] = RSQRT(ux[i] * ux[i] + uy[i] * uy[i] + uz[i] * uz[i]);
 }
 
-/* src/ispc/d3q19_collision_bgk.ispc:4-75.  pinned: test_oracle_collision. */
+/* src/ispc/d3q19_collision_bgk.ispc:4-75.  pinned: test_oracle_collision_*. */
 static void bgk3d_node(const lat_t* L, const real* f_in, real* f_out, size_t n, size_t i, real* rho, real* ux,
                        real* uy, real* uz, real tau, real gx, real gy, real gz) {
   real omega = R(1) / tau;
@@ -731,7 +734,7 @@ static void bgk3d_node(const lat_t* L, const real* f_in, real* f_out, size_t n, 
 }
 
 /* src/ispc/d3q19_collision_trt.ispc:5-93 and d3q27_collision_mrt.ispc:5-91
- * (the D3Q27 export is TRT).  pinned: test_oracle_collision. */
+ * (the D3Q27 export is TRT).  pinned: test_oracle_collision_*. */
 static void trt3d_node(const lat_t* L, const real* f_in, real* f_out, size_t n, size_t i, real* rho, real* ux,
                        real* uy, real* uz, real tau_s, real tau_a, real gx, real gy, real gz) {
   real omega_s = R(1) / tau_s, omega_a = R(1) / tau_a;
@@ -775,9 +778,9 @@ static void trt3d_node(const lat_t* L, const real* f_in, real* f_out, size_t n, 
 }
 
 /* src/ispc/d3q19_collision_mrt.ispc:26-176 with the rounded inverse table.
- * pinned: test_oracle_collision (rho, jx conservation as in
- * tests/unit/test_d3q19_collision.cpp:237-254), test_oracle_mrt_table
- * (-1.1e-6 per step mass drift at rest), test_oracle_taylor_green_3d
+ * pinned: test_oracle_collision_* (rho, jx conservation as in
+ * tests/unit/test_d3q19_collision.cpp:237-254), test_oracle_mrt_mass_drift_signature
+ * (-1.1e-6 per step mass drift at rest), test_oracle_taylor_green_3d_*
  * (tests/validation/test_taylor_green_decay.cpp). */
 static void mrt19_node(const real* f_in, real* f_out, size_t n, size_t i, real* rho, real* ux, real* uy,
                        real* uz, real tau_shear, real tau_bulk, real gx, real gy, real gz) {
@@ -905,7 +908,7 @@ static real trt_tau_a(real tau_s) {
 }
 
 /* EXTENSION: LES-coupled collision, per-node shear tau.  pinned:
- * test_oracle_les (tau_node == tau everywhere reproduces the scalar kernels
+ * test_oracle_les_* (tau_node == tau everywhere reproduces the scalar kernels
  * bit for bit). */
 void oracle_collide_tau_node(int lattice, int collision, const real* f_in, real* f_out,
                              const uint8_t* cell_type, const real* tau_node, real gx, real gy, real gz,
@@ -938,7 +941,7 @@ void oracle_collide_tau_node(int lattice, int collision, const real* f_in, real*
 }
 
 /* src/ispc/d3q19_streaming.ispc:4-35, d3q27_streaming.ispc:4-35.  pinned:
- * test_oracle_streaming (permutation, plane sums, delta moves by c_q). */
+ * test_oracle_streaming_* (permutation, plane sums, delta moves by c_q). */
 static void stream3d_periodic(const lat_t* L, const real* f_src, real* f_dst, int nx, int ny, int nz) {
   size_t total = (size_t)nx * ny * nz;
   for (int q = 0; q < L->Q; ++q) {
@@ -968,7 +971,7 @@ void oracle_d3q27_stream_periodic(const real* f_src, real* f_dst, int nx, int ny
 /* src/ispc/d3q19_streaming.ispc:39-104 (x periodic; y/z out of range and
  * solid neighbours reflect; moving-wall neighbours add 2 w 3 c.u_w).  With
  * per_y / per_z (EXTENSION) the axis wraps instead, and lattice 27 applies the
- * same rules to D3Q27.  pinned: test_oracle_streaming (closed-box mass
+ * same rules to D3Q27.  pinned: test_oracle_streaming_* (closed-box mass
  * conservation; all-fluid periodic equals the periodic kernel). */
 void oracle_stream3d_general(int lattice, const real* f_src, real* f_dst, const uint8_t* cell_type, int nx,
                              int ny, int nz, int per_y, int per_z, real uw_x, real uw_y, real uw_z) {
@@ -1015,7 +1018,7 @@ void oracle_d3q19_stream_with_boundaries(const real* f_src, real* f_dst, const u
   oracle_stream3d_general(19, f_src, f_dst, cell_type, nx, ny, nz, 0, 0, uw_x, uw_y, uw_z);
 }
 
-/* src/ispc/d3q19_les.ispc:7-66.  pinned: test_oracle_les. */
+/* src/ispc/d3q19_les.ispc:7-66.  pinned: test_oracle_les_*. */
 static void les3d(const lat_t* L, const real* f, const real* rho, const real* ux, const real* uy,
                   const real* uz, real* tau_eff, real tau_0, real cs, int count) {
   size_t n = (size_t)count;
@@ -1054,7 +1057,7 @@ void oracle_d3q19_compute_smagorinsky_tau(const real* f, const real* rho, const 
 
 /* EXTENSION: Smagorinsky tau from the populations alone, for any lattice;
  * rho and u come from f with the Guo shift, exactly as the forced moments
- * the reference passes in.  pinned: test_oracle_les (agrees bit for bit
+ * the reference passes in.  pinned: test_oracle_les_* (agrees bit for bit
  * with moments_forced + the reference LES kernel on D3Q19). */
 void oracle_smagorinsky_tau_from_f(int lattice, const real* f, real* tau_eff, real tau_0, real cs, real gx,
                                    real gy, real gz, int count) {
@@ -1072,8 +1075,9 @@ void oracle_smagorinsky_tau_from_f(int lattice, const real* f, real* tau_eff, re
 }
 
 /* src/ispc/d3q19_curved.ispc:7-71; per_y/per_z wrap x_ff (EXTENSION) the way
- * the streaming does.  pinned: test_bfl_reproduces_uniform_equilibrium_of_the
- * _wall_velocity (Galilean consistency of both wall terms, delta 0.01..0.99),
+ * the streaming does.  pinned:
+ * test_bfl_reproduces_uniform_equilibrium_of_the_wall_velocity (Galilean
+ * consistency of both wall terms, delta 0.01..0.99),
  * test_bfl_interpolates_a_linear_field_at_the_departure_point (interpolation
  * weights and the upstream neighbour in both branches),
  * test_bfl_limits_and_upstream_neighbour (delta -> 0 and 1, the y-edge
@@ -1351,7 +1355,7 @@ static int imax(int a, int b) { return a > b ? a : b; }
 static int imin(int a, int b) { return a < b ? a : b; }
 
 /* lattice_grid_3d.cpp:160-182 (bit-exact flag rule).  pinned:
- * test_oracle_flags (voxel count equals an exact-integer count of lattice
+ * test_oracle_sphere_flags_integer_count (voxel count equals an exact-integer count of lattice
  * points in the ball when the centre and radius are integers). */
 void oracle_grid_set_solid_sphere(oracle_grid* g, real cx, real cy, real cz, real radius) {
   g->is_periodic = 0;
@@ -1533,7 +1537,7 @@ void oracle_grid_set_les_coupled(oracle_grid* g, int coupled) { g->les_coupled =
 /* lattice_grid_3d.cpp:217-290: links in z -> y -> x -> q order, Fluid nodes
  * only, neighbour x periodic and y/z in range (wrapped under EXTENSION
  * per-axis periodicity), FP32 ray-sphere quadratic, clamp [0.01, 0.99].
- * pinned: test_oracle_links (brute-force segment/sphere intersection on small
+ * pinned: test_oracle_links_* (brute-force segment/sphere intersection on small
  * R; every link joins a Fluid node to a Solid neighbour). */
 int oracle_grid_build_curved_sphere_links(oracle_grid* g, real cx, real cy, real cz, real radius) {
   lat_t L = lattice_of(g->lattice);

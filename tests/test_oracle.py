@@ -53,6 +53,60 @@ def _d3q19_m_closed_form():
     return np.array(rows, dtype=np.float64).T  # M[k][q]
 
 
+def _oracle_velocities(lattice):
+    """The oracle's own c table, read back through its moments kernel: node q
+    holds a unit population q and nothing else, so its u is c_q exactly."""
+    Q = {9: 9, 19: 19, 27: 27}[lattice]
+    f = np.eye(Q, dtype=np.float64).reshape(Q, Q)  # f[q, node] = (q == node)
+    rho, ux, uy, uz = (np.zeros(Q, np.float64) for _ in range(4))
+    if lattice == 9:
+        call(64, "d2q9_compute_macroscopic_moments", f.ravel(), rho, ux, uy, Q)
+        return np.stack([ux, uy], 1)
+    call(64, f"d3q{lattice}_compute_macroscopic_moments", f.ravel(), rho, ux, uy, uz, Q)
+    return np.stack([ux, uy, uz], 1)
+
+
+@pytest.mark.parametrize("lattice", [9, 19, 27])
+def test_oracle_tables(lattice):
+    """lattice_models.hpp:14-110: the velocity set (distinct, rest first;
+    D3Q19 = the 19 of |c|^2 <= 2), the weights (read back as the rest
+    equilibrium) with sum w = 1, sum w c = 0, sum w c c = I/3, and the
+    opposite table as an involution with c_opp = -c, read back through
+    halfway bounce-back (a populated node whose every non-periodic pull source
+    is out of range receives its own opposite populations)."""
+    C = _oracle_velocities(lattice)
+    Q, D = C.shape
+    assert np.array_equal(C, np.round(C)) and np.all(np.abs(C) <= 1)
+    assert np.array_equal(C[0], np.zeros(D))
+    assert len({tuple(c) for c in C}) == Q
+    full = {tuple(v - 1 for v in c) for c in np.ndindex(*([3] * D))}
+    want = {c for c in full if sum(v * v for v in c) <= 2} if lattice == 19 else full
+    assert {tuple(int(v) for v in c) for c in C} == want
+    g = OracleGrid(2 if lattice == 9 else 3, 2, 2, 1 if lattice == 9 else 2, tau=0.6,
+                   lattice=None if lattice in (9, 19) else 27, bits=64)
+    g.initialize(1.0, 0.0, 0.0, 0.0)
+    w = g.f_src()[:, 0]
+    assert abs(w.sum() - 1.0) < 1e-15
+    assert np.all(np.abs(w @ C) < 1e-15)
+    assert np.allclose(C.T @ (w[:, None] * C), np.eye(D) / 3.0, atol=1e-15)
+    # opposites: distinct values on one node of a 1 x 1 (x 1) domain with
+    # y (and z) walls; x is always periodic, so only the +-x pair streams in
+    A = np.arange(1, Q + 1, dtype=np.float64)
+    S = np.zeros(Q, np.float64)
+    ct = np.zeros(1, np.uint8)
+    if lattice == 9:
+        call(64, "d2q9_stream_general", A, S, ct, 1, 1, 0, 0.0, 0.0)
+    else:
+        call(64, "stream3d_general", lattice, A, S, ct, 1, 1, 1, 0, 0, 0.0, 0.0, 0.0)
+    for q in range(Q):
+        if np.any(C[q, 1:] != 0):
+            o = int(S[q]) - 1
+            assert np.array_equal(C[o], -C[q]), (q, o)
+            assert int(S[o]) - 1 == q
+        else:
+            assert S[q] == A[q]
+
+
 def test_oracle_mrt_forward_matches_dhumieres_basis(oracle64):
     m = np.zeros(361, np.float64)
     oracle64.oracle_d3q19_m(ob.ptr(m))
