@@ -48,7 +48,7 @@ def _grid(wa, nx, ny, tau, storage, multistep):
 
 
 @pytest.mark.parametrize("storage", [0, 1, 2])
-@pytest.mark.parametrize("nx,ny", [(53, 37), (64, 48), (5, 7)])
+@pytest.mark.parametrize("nx,ny", [(53, 37), (64, 48), (5, 7), (2, 9), (17, 1)])
 def test_multistep_run_equals_single_steps(wa, storage, nx, ny):
     for coll, les, g in (("BGK", False, (0.0, 0.0)), ("TRT", False, (1e-6, -2e-6)), ("MRT", False, (0.0, 0.0)),
                          ("MRT", True, (2e-6, 0.0))):
