@@ -455,13 +455,173 @@ __device__ __forceinline__ void mrt_inverse_d3q19(const float (&m)[19], float (&
   });
 }
 
+// The two transforms for 16-bit population storage.  The storage rounds
+// f - w_q to 11 (FP16) or 8 (BF16) significant bits, far above the rounding of
+// any order of FP32 operations, so, as in collide_trt_fast27, the sums are
+// reorganised for issue count (these kernels are bound by VALU issue):
+//  * forward: each moment row is even or odd under c -> -c, so the even
+//    moments are sums of the nine opposite-pair sums s and the odd moments
+//    sums of the pair differences d (about 85 operations, against about 130
+//    row by row);
+//  * inverse: the same rounded table as mrt_inverse_d3q19, each entry applied
+//    once, but the rows of the faces and of the edges share their partial
+//    sums (one rho-and-energy sum per velocity shell, one stress sum per
+//    plane, one odd sum per axis), and each opposite pair is formed as
+//    even part +- odd part (about 75 operations, against about 110).
+struct D3Q19PairLayout {
+  // the velocity order both functions below are written for
+  static constexpr bool ok =
+      D3Q19::cx[1] == 1 && D3Q19::opp[1] == 2 && D3Q19::cy[3] == 1 && D3Q19::opp[3] == 4 && D3Q19::cz[5] == 1 &&
+      D3Q19::opp[5] == 6 && D3Q19::cx[7] == 1 && D3Q19::cy[7] == 1 && D3Q19::opp[7] == 10 && D3Q19::cx[8] == -1 &&
+      D3Q19::cy[8] == 1 && D3Q19::opp[8] == 9 && D3Q19::cx[11] == 1 && D3Q19::cz[11] == 1 && D3Q19::opp[11] == 14 &&
+      D3Q19::cx[12] == -1 && D3Q19::cz[12] == 1 && D3Q19::opp[12] == 13 && D3Q19::cy[15] == 1 &&
+      D3Q19::cz[15] == 1 && D3Q19::opp[15] == 18 && D3Q19::cy[16] == -1 && D3Q19::cz[16] == 1 &&
+      D3Q19::opp[16] == 17;
+};
+static_assert(D3Q19PairLayout::ok, "D3Q19 velocity order changed: rewrite the pair transforms");
+
+__device__ __forceinline__ void mrt_forward_d3q19_pairs(const float (&f)[19], float (&m)[19]) {
+  const float s1 = f[1] + f[2], d1 = f[1] - f[2];          // (+-1, 0, 0)
+  const float s3 = f[3] + f[4], d3 = f[3] - f[4];          // (0, +-1, 0)
+  const float s5 = f[5] + f[6], d5 = f[5] - f[6];          // (0, 0, +-1)
+  const float s7 = f[7] + f[10], d7 = f[7] - f[10];        // +-(1, 1, 0)
+  const float s8 = f[8] + f[9], d8 = f[8] - f[9];          // +-(-1, 1, 0)
+  const float s11 = f[11] + f[14], d11 = f[11] - f[14];    // +-(1, 0, 1)
+  const float s12 = f[12] + f[13], d12 = f[12] - f[13];    // +-(-1, 0, 1)
+  const float s15 = f[15] + f[18], d15 = f[15] - f[18];    // +-(0, 1, 1)
+  const float s16 = f[16] + f[17], d16 = f[16] - f[17];    // +-(0, -1, 1)
+  const float syz = s3 + s5;
+  const float faces = s1 + syz;
+  const float exy = s7 + s8, exz = s11 + s12, eyz = s15 + s16;
+  const float exyz = exy + exz;
+  const float edges = exyz + eyz;
+  m[0] = f[0] + faces + edges;
+  m[1] = -30.0f * f[0] - 11.0f * faces + 8.0f * edges;
+  m[2] = 12.0f * f[0] - 4.0f * faces + edges;
+  // odd parts of the edges: c_x over the xy and xz edges, c_y over xy and
+  // yz, c_z over xz and yz
+  const float xp = d7 - d8, xq = d11 - d12;
+  const float yp = d7 + d8, yq = d15 - d16;
+  const float zp = d11 + d12, zq = d15 + d16;
+  const float ex = xp + xq, ey = yp + yq, ez = zp + zq;
+  m[3] = d1 + ex;
+  m[4] = -4.0f * d1 + ex;
+  m[5] = d3 + ey;
+  m[6] = -4.0f * d3 + ey;
+  m[7] = d5 + ez;
+  m[8] = -4.0f * d5 + ez;
+  const float t = exyz - 2.0f * eyz;
+  m[9] = 2.0f * s1 - syz + t;
+  m[10] = -4.0f * s1 + 2.0f * syz + t;
+  const float u = s3 - s5, v = exy - exz;
+  m[11] = u + v;
+  m[12] = -2.0f * u + v;
+  m[13] = s7 - s8;
+  m[14] = s15 - s16;
+  m[15] = s11 - s12;
+  m[16] = xp - xq;
+  m[17] = yq - yp;
+  m[18] = zq - zp;
+}
+
+template <int q, int k>
+inline constexpr float kMinv = kD3Q19Minv.minv[q][k];
+
+// The inverse, written once for floats and for MomentForm, a linear form over
+// the 19 moments with which the compiler checks below that output q applies
+// exactly the entries of table row q.
+template <class V>
+__host__ __device__ constexpr void mrt_inverse_d3q19_shared(const V (&m)[19], V (&f)[19]) {
+  auto pair = [&](int q, int o, const V& even, const V& odd) {
+    f[q] = even + odd;
+    f[o] = even - odd;
+  };
+  f[0] = kMinv<0, 0> * m[0] + kMinv<0, 1> * m[1] + kMinv<0, 2> * m[2];
+  // faces: one shell sum; the x faces weigh m9, m10; the y and z faces weigh
+  // m9, m10 alike and m11, m12 with opposite signs
+  const V F = kMinv<1, 0> * m[0] + kMinv<1, 1> * m[1] + kMinv<1, 2> * m[2];
+  const V G = F + kMinv<3, 9> * m[9] + kMinv<3, 10> * m[10];
+  const V B = kMinv<3, 11> * m[11] + kMinv<3, 12> * m[12];
+  pair(1, 2, F + kMinv<1, 9> * m[9] + kMinv<1, 10> * m[10], kMinv<1, 3> * m[3] + kMinv<1, 4> * m[4]);
+  pair(3, 4, G + B, kMinv<3, 5> * m[5] + kMinv<3, 6> * m[6]);
+  pair(5, 6, G - B, kMinv<5, 7> * m[7] + kMinv<5, 8> * m[8]);
+  // edges: one shell sum; xy and xz edges share m9, m10 and take m11, m12
+  // with opposite signs; one odd sum per axis (c = +1) and per third-order
+  // moment
+  const V E = kMinv<7, 0> * m[0] + kMinv<7, 1> * m[1] + kMinv<7, 2> * m[2];
+  const V EP = E + kMinv<7, 9> * m[9] + kMinv<7, 10> * m[10];
+  const V R = kMinv<7, 11> * m[11] + kMinv<7, 12> * m[12];
+  const V X = kMinv<7, 3> * m[3] + kMinv<7, 4> * m[4];
+  const V Y = kMinv<7, 5> * m[5] + kMinv<7, 6> * m[6];
+  const V Z = kMinv<11, 7> * m[7] + kMinv<11, 8> * m[8];
+  {  // xy edges: (1, 1, 0) and (-1, 1, 0)
+    const V b = EP + R;
+    const V t13 = kMinv<7, 13> * m[13];
+    const V xa = X + kMinv<7, 16> * m[16], yb = Y + kMinv<7, 17> * m[17];
+    pair(7, 10, b + t13, xa + yb);
+    pair(8, 9, b - t13, yb - xa);
+  }
+  {  // xz edges: (1, 0, 1) and (-1, 0, 1)
+    const V b = EP - R;
+    const V t15 = kMinv<11, 15> * m[15];
+    const V xc = X + kMinv<11, 16> * m[16], zc = Z + kMinv<11, 18> * m[18];
+    pair(11, 14, b + t15, xc + zc);
+    pair(12, 13, b - t15, zc - xc);
+  }
+  {  // yz edges: (0, 1, 1) and (0, -1, 1)
+    const V b = E + kMinv<15, 9> * m[9] + kMinv<15, 10> * m[10];
+    const V t14 = kMinv<15, 14> * m[14];
+    const V yd = Y + kMinv<15, 17> * m[17], zd = Z + kMinv<15, 18> * m[18];
+    pair(15, 18, b + t14, yd + zd);
+    pair(16, 17, b - t14, zd - yd);
+  }
+}
+
+struct MomentForm {
+  double c[19] = {};
+  constexpr MomentForm operator+(const MomentForm& o) const {
+    MomentForm r;
+    for (int k = 0; k < 19; ++k) r.c[k] = c[k] + o.c[k];
+    return r;
+  }
+  constexpr MomentForm operator-(const MomentForm& o) const {
+    MomentForm r;
+    for (int k = 0; k < 19; ++k) r.c[k] = c[k] - o.c[k];
+    return r;
+  }
+};
+constexpr MomentForm operator*(float s, const MomentForm& v) {
+  MomentForm r;
+  for (int k = 0; k < 19; ++k) r.c[k] = static_cast<double>(s) * v.c[k];
+  return r;
+}
+
+// Every output row equals its table row, entry for entry (each entry enters
+// once, so the comparison is exact).
+constexpr bool mrt_inverse_shared_matches_table() {
+  MomentForm m[19], f[19];
+  for (int k = 0; k < 19; ++k) m[k].c[k] = 1.0;
+  mrt_inverse_d3q19_shared(m, f);
+  for (int q = 0; q < 19; ++q)
+    for (int k = 0; k < 19; ++k)
+      if (f[q].c[k] != static_cast<double>(kD3Q19Minv.minv[q][k])) return false;
+  return true;
+}
+static_assert(mrt_inverse_shared_matches_table(), "shared-sum inverse does not apply the rounded table");
+
+__device__ __forceinline__ void mrt_inverse_d3q19_pairs(const float (&m)[19], float (&f)[19]) {
+  mrt_inverse_d3q19_shared(m, f);
+}
+
 template <bool LES = false, bool FAST = false>
 __device__ __forceinline__ Macro collide_mrt_d3q19(float (&f)[19], float tau_shear, float tau_bulk, float gx,
                                                    float gy, float gz, float cs = 0.0f) {
   float m[19];
-  mrt_forward_d3q19(f, m);
+  if constexpr (FAST) mrt_forward_d3q19_pairs(f, m);
+  else mrt_forward_d3q19(f, m);
   const Macro r = mrt_relax_d3q19<LES, FAST>(m, tau_shear, tau_bulk, gx, gy, gz, cs);
-  mrt_inverse_d3q19(m, f);
+  if constexpr (FAST) mrt_inverse_d3q19_pairs(m, f);
+  else mrt_inverse_d3q19(m, f);
   return r;
 }
 
@@ -473,12 +633,9 @@ __device__ __forceinline__ Macro collide_mrt_d3q19(float (&f)[19], float tau_she
 //  * the moments rho, j and sum c_a c_b f come from summing the 27
 //    populations along x, then y, then z (66 additions, against about 160
 //    for the separate sums);
-//  * each opposite pair (q, o) relaxes through its sum s and difference d:
-//      f_q' = f_q - D_s - D_a,  f_o' = f_o - D_s + D_a,
-//      D_s = omega_s (s / 2 - w rho (1 - 1.5 u^2 + 4.5 (c u)^2)),
-//      D_a = omega_a (d / 2 - 3 w rho c u),
-//    which is f - omega_s (f+ - feq+) - omega_a (f- - feq-) with the
-//    symmetric and antisymmetric equilibria formed once per pair;
+//  * each opposite pair (q, o) relaxes through its sum s and difference d
+//    (trt_pairs), which is f - omega_s (f+ - feq+) - omega_a (f- - feq-) with
+//    the symmetric and antisymmetric equilibria formed once per pair;
 //  * the Guo force terms are compiled out when the body force is zero.
 
 // Index of velocity (a, b, c) in lattice L, or -1.
@@ -561,7 +718,14 @@ __device__ __forceinline__ void trt_pairs(float (&f)[L::Q], float rho, float ux,
                                           float omega_a, float gx, float gy, float gz) {
   const float usq = ux * ux + uy * uy + uz * uz;
   const float K = 1.0f - 1.5f * usq;
-  const float hs = 0.5f * omega_s, ha = 0.5f * omega_a;
+  // With s = f_q + f_o and d = f_q - f_o, the pair relaxes to
+  //   f_q' = S + A,  f_o' = S - A,
+  //   S = (1 - omega_s) s / 2 + omega_s w rho (K + 4.5 (c u)^2),
+  //   A = (1 - omega_a) d / 2 + 3 omega_a w rho (c u),
+  // the per-shell factors omega_s w rho, its K and 4.5 multiples and
+  // 3 omega_a w rho formed once per weight (three per node).
+  const float as = 0.5f - 0.5f * omega_s, aa = 0.5f - 0.5f * omega_a;
+  const float srho = omega_s * rho, arho = 3.0f * omega_a * rho;
   float pre_s = 0.0f, pre_a = 0.0f, ug = 0.0f;
   if constexpr (FORCE) {
     pre_s = 1.0f - 0.5f * omega_s;
@@ -578,25 +742,22 @@ __device__ __forceinline__ void trt_pairs(float (&f)[L::Q], float rho, float ux,
     constexpr int q = decltype(qc)::value;
     constexpr int o = L::opp[q];
     if constexpr (q > 0 && q < o) {
-      const float wr = L::w[q] * rho;
+      constexpr float w = L::w[q];
+      const float ws = srho * w;
+      const float beta = ws * K, gam = ws * 4.5f, del = arho * w;
       const float cu = cdot<L, q>(ux, uy, uz);
       const float s = f[q] + f[o];
       const float d = f[q] - f[o];
-      const float esym = wr * ((4.5f * cu) * cu + K);
-      const float easym = (3.0f * wr) * cu;
-      const float Ds = hs * s - omega_s * esym;
-      const float Da = ha * d - omega_a * easym;
-      float fq = f[q] - Ds - Da;
-      float fo = f[o] - Ds + Da;
+      float S = as * s + ((gam * cu) * cu + beta);
+      float A = aa * d + del * cu;
       if constexpr (FORCE) {
+        const float wr = w * rho;
         const float cg = cdot<L, q>(gx, gy, gz);
-        const float Fa = pre_a * (3.0f * wr) * cg;
-        const float Fs = pre_s * wr * (9.0f * cu * cg - 3.0f * ug);
-        fq += Fs + Fa;
-        fo += Fs - Fa;
+        A += pre_a * (3.0f * wr) * cg;
+        S += pre_s * wr * (9.0f * cu * cg - 3.0f * ug);
       }
-      f[q] = fq;
-      f[o] = fo;
+      f[q] = S + A;
+      f[o] = S - A;
     }
   });
 }
