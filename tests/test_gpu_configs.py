@@ -169,7 +169,7 @@ def test_config2_boundary_set_1000_steps(wa):
     assert np.all(e <= np.maximum(FORCE_TOL, 1.25 * s))
 
 
-def _les_member(target, nx, re, oracle_bits=None):
+def _les_member(target, nx, re, oracle_bits=None, storage=0):
     ny = nx // 4
     D = ny / 8.0
     umax = 0.05
@@ -178,7 +178,7 @@ def _les_member(target, nx, re, oracle_bits=None):
         g = ob.OracleGrid(2, nx, ny, 1, tau=tau, lattice=9, bits=oracle_bits)
         cell = lambda x, y, t: g.set_cell_type(x, y, 0, t)
     else:
-        g = target.LatticeGrid2D(nx, ny, tau)
+        g = target.LatticeGrid2D(nx, ny, tau, storage=storage)
         cell = lambda x, y, t: g.set_cell_type(x, y, t)
     H = ny - 2
     s = np.arange(ny, dtype=np.float64) - 0.5
@@ -223,3 +223,35 @@ def test_config4_les_coupled_mrt_batch_member_vs_oracle(wa):
     e, s = np.abs(fg - f32) / np.abs(f32), np.abs(ffm - f32) / np.abs(f32)
     print(f"LES-coupled member drag every 50 steps: max error {e.max():.2e}, FP32-build spread {s.max():.2e}")
     assert np.all(e <= np.maximum(FORCE_TOL, 1.25 * s))
+
+
+@pytest.mark.parametrize("storage,name", [(1, "fp16"), (2, "bf16")])
+def test_config4_batch_member_reduced_precision_vs_fp64(wa, storage, name):
+    """The batched-wake member with 16-bit deviation storage in a batch,
+    against the FP64 oracle (NX 128, D 4, Re 300, 400 steps).  tau is 0.50133
+    here, next to the stability limit as in the wing channel, and the flow
+    amplifies the storage rounding the same way (measured on MI355X: FP16
+    3.0e-3 in u and 8e-4 in the drag, BF16 2.9e-2 and 9e-3), so the wing's
+    bounds apply.  FP16 storage
+    misses the 1e-3 field bound on this flow, which is why the batched-wake
+    workload keeps FP32 storage."""
+    nx, re, steps = 128, 300.0, 400
+    g, (cx, cy, D) = _les_member(wa, nx, re, storage=storage)
+    b = wa.Batch(lattice=9, storage=storage, collision="MRT", use_les=True, chunk=50)
+    slot = b.add(g, steps, smag=0.17, forces=(cx, cy, 0.0, 1.5 * D))
+    b.run(steps)
+    F = b.forces(slot)
+    o64, _ = _les_member(wa, nx, re, oracle_bits=64)
+    fo = []
+    for _ in range(steps // 50):
+        o64.run(50, "MRT", use_les=True, smag=0.17)
+        fo.append(o64.momentum_exchange(cx, cy, 0.0, 1.5 * D))
+    fl = o64.cell_type() == 0
+    eu = rel_l2(np.concatenate([g.ux_data()[fl], g.uy_data()[fl]]), np.concatenate([o64.ux()[fl], o64.uy()[fl]]))
+    f64 = np.array(fo)[:, 0]
+    ef = float(np.max(np.abs(F[49::50, 0] - f64) / np.abs(f64)))
+    print(f"LES-coupled D2Q9 member, {name} storage vs FP64 after {steps} steps: u rel L2 {eu:.2e}, "
+          f"drag max rel error {ef:.2e}")
+    bounds = PRECISION_BOUNDS[storage]
+    assert eu <= bounds["u"]
+    assert ef <= bounds["force"]
