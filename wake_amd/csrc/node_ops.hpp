@@ -458,7 +458,7 @@ __device__ __forceinline__ void mrt_inverse_d3q19(const float (&m)[19], float (&
 // The two transforms for 16-bit population storage.  The storage rounds
 // f - w_q to 11 (FP16) or 8 (BF16) significant bits, far above the rounding of
 // any order of FP32 operations, so, as in collide_trt_fast27, the sums are
-// reorganised for issue count (these kernels are bound by VALU issue):
+// reorganised for issue count (VALU issue, not HBM, limited these kernels):
 //  * forward: each moment row is even or odd under c -> -c, so the even
 //    moments are sums of the nine opposite-pair sums s and the odd moments
 //    sums of the pair differences d (about 85 operations, against about 130
@@ -629,7 +629,7 @@ __device__ __forceinline__ Macro collide_mrt_d3q19(float (&f)[19], float tau_she
 // D3Q27 TRT for 16-bit population storage.  The storage rounds f - w_q to 11
 // (FP16) or 8 (BF16) significant bits, far above the rounding of any order
 // of FP32 operations, so this path is free to reorganise the arithmetic for
-// issue count (the 16-bit kernels are bound by VALU issue, not HBM):
+// issue count (VALU issue, not HBM, limited the 16-bit kernels):
 //  * the moments rho, j and sum c_a c_b f come from summing the 27
 //    populations along x, then y, then z (66 additions, against about 160
 //    for the separate sums);
