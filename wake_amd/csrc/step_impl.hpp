@@ -215,10 +215,29 @@ __device__ __forceinline__ float rng_sym(unsigned long long seed, unsigned long 
 
 __device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
+// A work-group-uniform value (row indices, sizes, the plane stride) as an
+// opaque scalar operand.  The bulk pull forms its offsets from such copies,
+// so the compiler cannot share them with the slow paths' address arithmetic:
+// shared, they stay live across the whole node update and spill to VGPR
+// lanes (a v_writelane per value on every node, VALU issue the 16-bit
+// kernels are short of).  Only for wave-uniform values: gather takes this
+// path only when its caller's lanes hold nodes of one row (UNIFORM_ROW).
+__device__ __forceinline__ int opaque_uniform(int v) {
+  v = __builtin_amdgcn_readfirstlane(v);
+  asm("" : "+s"(v));
+  return v;
+}
+__device__ __forceinline__ long long opaque_uniform(long long v) {
+  const int lo = opaque_uniform(static_cast<int>(v));
+  const int hi = opaque_uniform(static_cast<int>(v >> 32));
+  return static_cast<long long>((static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) |
+                                static_cast<unsigned>(lo));
+}
+
 // ------------------------------------------------------------------ gather
 // S of node (x, y, z) from the post-collision buffer A: streaming with the
 // reference's bounce-back rules, then the BFL overrides.
-template <class L, class St>
+template <class L, class St, bool UNIFORM_ROW = false>
 __device__ __forceinline__ void gather(const StepArgs& a, const typename St::T* __restrict__ A, int x, int y,
                                        int z, int n, uint8_t c, float (&f)[L::Q]) {
   const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -234,22 +253,34 @@ __device__ __forceinline__ void gather(const StepArgs& a, const typename St::T* 
     constexpr unsigned E = sizeof(typename St::T);
     const unsigned vx = static_cast<unsigned>(x) * E, vm = static_cast<unsigned>(xm) * E,
                    vp = static_cast<unsigned>(xp) * E;
+    // (D3Q27 keeps the shared offsets: opaque copies raise its FP32 kernel
+    // to 132 VGPRs and spill its 16-bit ones to scratch)
+    constexpr bool kOpaque = UNIFORM_ROW && L::Q <= 19;
+    const int by = kOpaque ? opaque_uniform(y) : y, bz = kOpaque ? opaque_uniform(z) : z;
+    const int bnx = kOpaque ? opaque_uniform(nx) : nx, bny = kOpaque ? opaque_uniform(ny) : ny;
+    const int bnz = kOpaque ? opaque_uniform(nz) : nz;
+    const long long bS = kOpaque ? opaque_uniform(S) : S;
     static_for<L::Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       constexpr int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
-      const int sy = cy == 0 ? y : wrap(y - cy, ny);
-      const int sz = cz == 0 ? z : wrap(z - cz, nz);
-      const unsigned so = static_cast<unsigned>(q * S + static_cast<long long>((sz * ny + sy) * nx)) * E;
+      const int sy = cy == 0 ? by : wrap(by - cy, bny);
+      const int sz = cz == 0 ? bz : wrap(bz - cz, bnz);
+      const unsigned so = static_cast<unsigned>(q * bS + static_cast<long long>((sz * bny + sy) * bnx)) * E;
       f[q] = St::template ldb<L, q>(r, cx == 0 ? vx : (cx > 0 ? vm : vp), so);
     });
   } else {
+    constexpr bool kOpaque = UNIFORM_ROW && L::Q <= 19;
+    const int by = kOpaque ? opaque_uniform(y) : y, bz = kOpaque ? opaque_uniform(z) : z;
+    const int bnx = kOpaque ? opaque_uniform(nx) : nx, bny = kOpaque ? opaque_uniform(ny) : ny;
+    const int bnz = kOpaque ? opaque_uniform(nz) : nz;
+    const long long bS = kOpaque ? opaque_uniform(S) : S;
     static_for<L::Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       constexpr int cx = L::cx[q], cy = L::cy[q], cz = L::cz[q];
       const int sx = cx == 0 ? x : (cx > 0 ? xm : xp);
-      const int sy = cy == 0 ? y : wrap(y - cy, ny);
-      const int sz = cz == 0 ? z : wrap(z - cz, nz);
-      f[q] = St::template ld<L, q>(A, q * S + static_cast<long long>((sz * ny + sy) * nx + sx));
+      const int sy = cy == 0 ? by : wrap(by - cy, bny);
+      const int sz = cz == 0 ? bz : wrap(bz - cz, bnz);
+      f[q] = St::template ld<L, q>(A, q * bS + static_cast<long long>((sz * bny + sy) * bnx + sx));
     });
   }
   if (!(c & (kClsFrozen | kClsSlow | kClsBfl))) return;
@@ -528,7 +559,7 @@ __device__ __forceinline__ void form_s_at(const StepArgs& a, const typename St::
       rec_loaded = true;
     }
   }
-  gather<L, St>(a, A, x, y, z, n, c, f);
+  gather<L, St, ROW_LANES>(a, A, x, y, z, n, c, f);
   if (c & kClsInlet) apply_inlet<L>(a, f, y, n, s_step);
   if constexpr (L::Q == 9) {
     if ((c & kClsOutlet) && a.outflow == 1) apply_zh_outflow_d2q9(a, f);
@@ -556,7 +587,7 @@ __device__ __forceinline__ void form_s_at(const StepArgs& a, const typename St::
   // the upstream neighbour is not in this wave: gather it
   float nb[L::Q];
   const uint8_t cn = a.cls[n - 1];
-  gather<L, St>(a, A, x - 1, y, z, n - 1, cn, nb);
+  gather<L, St, ROW_LANES>(a, A, x - 1, y, z, n - 1, cn, nb);
   if (cn & kClsInlet) apply_inlet<L>(a, nb, y, n - 1, s_step);
   load_prev(prev);  // (the shuffle pass above, if any, overwrote prev)
   static_for<L::Q>([&](auto qc) {
