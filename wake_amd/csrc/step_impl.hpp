@@ -253,9 +253,7 @@ __device__ __forceinline__ void gather(const StepArgs& a, const typename St::T* 
     constexpr unsigned E = sizeof(typename St::T);
     const unsigned vx = static_cast<unsigned>(x) * E, vm = static_cast<unsigned>(xm) * E,
                    vp = static_cast<unsigned>(xp) * E;
-    // (D3Q27 keeps the shared offsets: opaque copies raise its FP32 kernel
-    // to 132 VGPRs and spill its 16-bit ones to scratch)
-    constexpr bool kOpaque = UNIFORM_ROW && L::Q <= 19;
+    constexpr bool kOpaque = UNIFORM_ROW;
     const int by = kOpaque ? opaque_uniform(y) : y, bz = kOpaque ? opaque_uniform(z) : z;
     const int bnx = kOpaque ? opaque_uniform(nx) : nx, bny = kOpaque ? opaque_uniform(ny) : ny;
     const int bnz = kOpaque ? opaque_uniform(nz) : nz;
@@ -269,6 +267,8 @@ __device__ __forceinline__ void gather(const StepArgs& a, const typename St::T* 
       f[q] = St::template ldb<L, q>(r, cx == 0 ? vx : (cx > 0 ? vm : vp), so);
     });
   } else {
+    // (the D3Q27 pointer-addressed kernel keeps the shared offsets: opaque
+    // copies raise it from 90 to 132 VGPRs, 5 to 3 waves per SIMD)
     constexpr bool kOpaque = UNIFORM_ROW && L::Q <= 19;
     const int by = kOpaque ? opaque_uniform(y) : y, bz = kOpaque ? opaque_uniform(z) : z;
     const int bnx = kOpaque ? opaque_uniform(nx) : nx, bny = kOpaque ? opaque_uniform(ny) : ny;
