@@ -480,40 +480,44 @@ struct D3Q19PairLayout {
 };
 static_assert(D3Q19PairLayout::ok, "D3Q19 velocity order changed: rewrite the pair transforms");
 
-__device__ __forceinline__ void mrt_forward_d3q19_pairs(const float (&f)[19], float (&m)[19]) {
-  const float s1 = f[1] + f[2], d1 = f[1] - f[2];          // (+-1, 0, 0)
-  const float s3 = f[3] + f[4], d3 = f[3] - f[4];          // (0, +-1, 0)
-  const float s5 = f[5] + f[6], d5 = f[5] - f[6];          // (0, 0, +-1)
-  const float s7 = f[7] + f[10], d7 = f[7] - f[10];        // +-(1, 1, 0)
-  const float s8 = f[8] + f[9], d8 = f[8] - f[9];          // +-(-1, 1, 0)
-  const float s11 = f[11] + f[14], d11 = f[11] - f[14];    // +-(1, 0, 1)
-  const float s12 = f[12] + f[13], d12 = f[12] - f[13];    // +-(-1, 0, 1)
-  const float s15 = f[15] + f[18], d15 = f[15] - f[18];    // +-(0, 1, 1)
-  const float s16 = f[16] + f[17], d16 = f[16] - f[17];    // +-(0, -1, 1)
-  const float syz = s3 + s5;
-  const float faces = s1 + syz;
-  const float exy = s7 + s8, exz = s11 + s12, eyz = s15 + s16;
-  const float exyz = exy + exz;
-  const float edges = exyz + eyz;
+// Both transforms are written once for floats and for LinearForm19, a linear
+// form over 19 values, with which the compiler checks below that the forward
+// transform is the basis M and the inverse applies the rounded table.
+template <class V>
+__host__ __device__ constexpr void mrt_forward_d3q19_shared(const V (&f)[19], V (&m)[19]) {
+  const V s1 = f[1] + f[2], d1 = f[1] - f[2];          // (+-1, 0, 0)
+  const V s3 = f[3] + f[4], d3 = f[3] - f[4];          // (0, +-1, 0)
+  const V s5 = f[5] + f[6], d5 = f[5] - f[6];          // (0, 0, +-1)
+  const V s7 = f[7] + f[10], d7 = f[7] - f[10];        // +-(1, 1, 0)
+  const V s8 = f[8] + f[9], d8 = f[8] - f[9];          // +-(-1, 1, 0)
+  const V s11 = f[11] + f[14], d11 = f[11] - f[14];    // +-(1, 0, 1)
+  const V s12 = f[12] + f[13], d12 = f[12] - f[13];    // +-(-1, 0, 1)
+  const V s15 = f[15] + f[18], d15 = f[15] - f[18];    // +-(0, 1, 1)
+  const V s16 = f[16] + f[17], d16 = f[16] - f[17];    // +-(0, -1, 1)
+  const V syz = s3 + s5;
+  const V faces = s1 + syz;
+  const V exy = s7 + s8, exz = s11 + s12, eyz = s15 + s16;
+  const V exyz = exy + exz;
+  const V edges = exyz + eyz;
   m[0] = f[0] + faces + edges;
   m[1] = -30.0f * f[0] - 11.0f * faces + 8.0f * edges;
   m[2] = 12.0f * f[0] - 4.0f * faces + edges;
   // odd parts of the edges: c_x over the xy and xz edges, c_y over xy and
   // yz, c_z over xz and yz
-  const float xp = d7 - d8, xq = d11 - d12;
-  const float yp = d7 + d8, yq = d15 - d16;
-  const float zp = d11 + d12, zq = d15 + d16;
-  const float ex = xp + xq, ey = yp + yq, ez = zp + zq;
+  const V xp = d7 - d8, xq = d11 - d12;
+  const V yp = d7 + d8, yq = d15 - d16;
+  const V zp = d11 + d12, zq = d15 + d16;
+  const V ex = xp + xq, ey = yp + yq, ez = zp + zq;
   m[3] = d1 + ex;
   m[4] = -4.0f * d1 + ex;
   m[5] = d3 + ey;
   m[6] = -4.0f * d3 + ey;
   m[7] = d5 + ez;
   m[8] = -4.0f * d5 + ez;
-  const float t = exyz - 2.0f * eyz;
+  const V t = exyz - 2.0f * eyz;
   m[9] = 2.0f * s1 - syz + t;
   m[10] = -4.0f * s1 + 2.0f * syz + t;
-  const float u = s3 - s5, v = exy - exz;
+  const V u = s3 - s5, v = exy - exz;
   m[11] = u + v;
   m[12] = -2.0f * u + v;
   m[13] = s7 - s8;
@@ -527,9 +531,6 @@ __device__ __forceinline__ void mrt_forward_d3q19_pairs(const float (&f)[19], fl
 template <int q, int k>
 inline constexpr float kMinv = kD3Q19Minv.minv[q][k];
 
-// The inverse, written once for floats and for MomentForm, a linear form over
-// the 19 moments with which the compiler checks below that output q applies
-// exactly the entries of table row q.
 template <class V>
 __host__ __device__ constexpr void mrt_inverse_d3q19_shared(const V (&m)[19], V (&f)[19]) {
   auto pair = [&](int q, int o, const V& even, const V& odd) {
@@ -577,29 +578,42 @@ __host__ __device__ constexpr void mrt_inverse_d3q19_shared(const V (&m)[19], V 
   }
 }
 
-struct MomentForm {
+struct LinearForm19 {
   double c[19] = {};
-  constexpr MomentForm operator+(const MomentForm& o) const {
-    MomentForm r;
+  constexpr LinearForm19 operator+(const LinearForm19& o) const {
+    LinearForm19 r;
     for (int k = 0; k < 19; ++k) r.c[k] = c[k] + o.c[k];
     return r;
   }
-  constexpr MomentForm operator-(const MomentForm& o) const {
-    MomentForm r;
+  constexpr LinearForm19 operator-(const LinearForm19& o) const {
+    LinearForm19 r;
     for (int k = 0; k < 19; ++k) r.c[k] = c[k] - o.c[k];
     return r;
   }
 };
-constexpr MomentForm operator*(float s, const MomentForm& v) {
-  MomentForm r;
+constexpr LinearForm19 operator*(float s, const LinearForm19& v) {
+  LinearForm19 r;
   for (int k = 0; k < 19; ++k) r.c[k] = static_cast<double>(s) * v.c[k];
   return r;
 }
 
-// Every output row equals its table row, entry for entry (each entry enters
-// once, so the comparison is exact).
+// Moment k of the forward transform is sum_q M[k][q] f_q (small integers,
+// exact in double).
+constexpr bool mrt_forward_shared_matches_basis() {
+  LinearForm19 f[19], m[19];
+  for (int q = 0; q < 19; ++q) f[q].c[q] = 1.0;
+  mrt_forward_d3q19_shared(f, m);
+  for (int k = 0; k < 19; ++k)
+    for (int q = 0; q < 19; ++q)
+      if (m[k].c[q] != d3q19_m(k, q)) return false;
+  return true;
+}
+static_assert(mrt_forward_shared_matches_basis(), "pair-sum forward transform is not the basis M");
+
+// Every output row of the inverse equals its table row, entry for entry
+// (each entry enters once, so the comparison is exact).
 constexpr bool mrt_inverse_shared_matches_table() {
-  MomentForm m[19], f[19];
+  LinearForm19 m[19], f[19];
   for (int k = 0; k < 19; ++k) m[k].c[k] = 1.0;
   mrt_inverse_d3q19_shared(m, f);
   for (int q = 0; q < 19; ++q)
@@ -609,6 +623,9 @@ constexpr bool mrt_inverse_shared_matches_table() {
 }
 static_assert(mrt_inverse_shared_matches_table(), "shared-sum inverse does not apply the rounded table");
 
+__device__ __forceinline__ void mrt_forward_d3q19_pairs(const float (&f)[19], float (&m)[19]) {
+  mrt_forward_d3q19_shared(f, m);
+}
 __device__ __forceinline__ void mrt_inverse_d3q19_pairs(const float (&m)[19], float (&f)[19]) {
   mrt_inverse_d3q19_shared(m, f);
 }
