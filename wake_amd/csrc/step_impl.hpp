@@ -215,8 +215,8 @@ __device__ __forceinline__ float rng_sym(unsigned long long seed, unsigned long 
 
 __device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
-// A work-group-uniform value (row indices, sizes, the plane stride) as an
-// opaque scalar operand.  The bulk pull forms its offsets from such copies,
+// A wave-uniform value (row indices, sizes, the plane stride) as an opaque
+// scalar operand.  The bulk pull forms its offsets from such copies,
 // so the compiler cannot share them with the slow paths' address arithmetic:
 // shared, they stay live across the whole node update and spill to VGPR
 // lanes (a v_writelane per value on every node, VALU issue the 16-bit
@@ -519,10 +519,11 @@ __device__ __forceinline__ void apply_zh_outflow_d2q9(const StepArgs& a, float (
 // (the outlet store, one 16-byte aligned record of prev_stride<Q> floats per
 // (y, z) row, read and written with vector accesses) with the post-stream
 // state of its upstream neighbour x - 1.  With ROW_LANES (lanes hold
-// consecutive nodes, as in every row- or tile-shaped kernel) that state is
-// taken from lane - 1 by a cross-lane shuffle of the values the neighbour
-// has just formed; otherwise (and for a lane 0 outlet) the outlet lane
-// gathers it itself.
+// consecutive nodes of one row, as in every row- or tile-shaped kernel) that
+// state is taken from lane - 1 by a cross-lane shuffle of the values the
+// neighbour has just formed, and the bulk pull forms its offsets from opaque
+// copies of the wave's row indices; otherwise (and for a lane 0 outlet) the
+// outlet lane gathers it itself.
 template <class L, class St, bool ROW_LANES = true>
 __device__ __forceinline__ void form_s_at(const StepArgs& a, const typename St::T* __restrict__ A, int x, int y,
                                           int z, int n, uint8_t c, bool commit, unsigned long long s_step,
