@@ -480,9 +480,9 @@ struct D3Q19PairLayout {
 };
 static_assert(D3Q19PairLayout::ok, "D3Q19 velocity order changed: rewrite the pair transforms");
 
-// Both transforms are written once for floats and for LinearForm19, a linear
-// form over 19 values, with which the compiler checks below that the forward
-// transform is the basis M and the inverse applies the rounded table.
+// Both transforms are written once for floats and for LinearForm<19>, with
+// which the compiler checks below that the forward transform is the basis M
+// and the inverse applies the rounded table.
 template <class V>
 __host__ __device__ constexpr void mrt_forward_d3q19_shared(const V (&f)[19], V (&m)[19]) {
   const V s1 = f[1] + f[2], d1 = f[1] - f[2];          // (+-1, 0, 0)
@@ -578,29 +578,33 @@ __host__ __device__ constexpr void mrt_inverse_d3q19_shared(const V (&m)[19], V 
   }
 }
 
-struct LinearForm19 {
-  double c[19] = {};
-  constexpr LinearForm19 operator+(const LinearForm19& o) const {
-    LinearForm19 r;
-    for (int k = 0; k < 19; ++k) r.c[k] = c[k] + o.c[k];
+// A linear form over N values, exact in double for the small coefficients
+// the transforms use.
+template <int N>
+struct LinearForm {
+  double c[N] = {};
+  constexpr LinearForm operator+(const LinearForm& o) const {
+    LinearForm r;
+    for (int k = 0; k < N; ++k) r.c[k] = c[k] + o.c[k];
     return r;
   }
-  constexpr LinearForm19 operator-(const LinearForm19& o) const {
-    LinearForm19 r;
-    for (int k = 0; k < 19; ++k) r.c[k] = c[k] - o.c[k];
+  constexpr LinearForm operator-(const LinearForm& o) const {
+    LinearForm r;
+    for (int k = 0; k < N; ++k) r.c[k] = c[k] - o.c[k];
     return r;
   }
 };
-constexpr LinearForm19 operator*(float s, const LinearForm19& v) {
-  LinearForm19 r;
-  for (int k = 0; k < 19; ++k) r.c[k] = static_cast<double>(s) * v.c[k];
+template <int N>
+constexpr LinearForm<N> operator*(float s, const LinearForm<N>& v) {
+  LinearForm<N> r;
+  for (int k = 0; k < N; ++k) r.c[k] = static_cast<double>(s) * v.c[k];
   return r;
 }
 
 // Moment k of the forward transform is sum_q M[k][q] f_q (small integers,
 // exact in double).
 constexpr bool mrt_forward_shared_matches_basis() {
-  LinearForm19 f[19], m[19];
+  LinearForm<19> f[19], m[19];
   for (int q = 0; q < 19; ++q) f[q].c[q] = 1.0;
   mrt_forward_d3q19_shared(f, m);
   for (int k = 0; k < 19; ++k)
@@ -613,7 +617,7 @@ static_assert(mrt_forward_shared_matches_basis(), "pair-sum forward transform is
 // Every output row of the inverse equals its table row, entry for entry
 // (each entry enters once, so the comparison is exact).
 constexpr bool mrt_inverse_shared_matches_table() {
-  LinearForm19 m[19], f[19];
+  LinearForm<19> m[19], f[19];
   for (int k = 0; k < 19; ++k) m[k].c[k] = 1.0;
   mrt_inverse_d3q19_shared(m, f);
   for (int q = 0; q < 19; ++q)
@@ -663,32 +667,47 @@ constexpr int q_of(int a, int b, int c) {
   return -1;
 }
 
-struct Moments2 {
-  float rho, jx, jy, jz;        // sum f, sum c f
-  float sxx, syy, szz;          // sum c_a^2 f
-  float sxy, syz, sxz;          // sum c_a c_b f
+template <class V>
+struct Moments2T {
+  V rho, jx, jy, jz;        // sum f, sum c f
+  V sxx, syy, szz;          // sum c_a^2 f
+  V sxy, syz, sxz;          // sum c_a c_b f
 };
+using Moments2 = Moments2T<float>;
 
+// The x lines of lattice L: line i = 3 (b + 1) + (c + 1) holds the
+// velocities (-1, b, c), (0, b, c), (1, b, c).
+struct XLines {
+  int q[9][3];
+};
 template <class L>
-__device__ __forceinline__ Moments2 moments_d3q27(const float (&f)[L::Q]) {
+constexpr XLines make_x_lines() {
+  XLines t = {};
+  for (int i = 0; i < 9; ++i)
+    for (int a = -1; a <= 1; ++a) t.q[i][a + 1] = q_of<L>(a, i / 3 - 1, i % 3 - 1);
+  return t;
+}
+
+// Written once for floats and for LinearForm<27>, with which the compiler
+// checks below that every moment is its sum over the 27 populations.
+template <class L, class V>
+__host__ __device__ constexpr Moments2T<V> moments_d3q27_t(const V (&f)[L::Q]) {
   static_assert(L::Q == 27, "the x-y-z cascade needs every velocity of {-1,0,1}^3");
+  constexpr XLines xl = make_x_lines<L>();
   // x lines (y = b, z = c): orders 0, 1, 2 in c_x
-  float k0[3][3], k1[3][3], k2[3][3];
-  static_for<9>([&](auto ic) {
-    constexpr int i = decltype(ic)::value;
-    constexpr int b = i / 3 - 1, c = i % 3 - 1;
-    constexpr int qm = q_of<L>(-1, b, c), q0 = q_of<L>(0, b, c), qp = q_of<L>(1, b, c);
-    static_assert(qm >= 0 && q0 >= 0 && qp >= 0, "missing velocity");
-    const float t = f[qp] + f[qm];
-    k2[b + 1][c + 1] = t;
-    k1[b + 1][c + 1] = f[qp] - f[qm];
-    k0[b + 1][c + 1] = t + f[q0];
-  });
+  V k0[3][3], k1[3][3], k2[3][3];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const V t = f[xl.q[i][2]] + f[xl.q[i][0]];
+    k2[i / 3][i % 3] = t;
+    k1[i / 3][i % 3] = f[xl.q[i][2]] - f[xl.q[i][0]];
+    k0[i / 3][i % 3] = t + f[xl.q[i][1]];
+  }
   // y: for each z, orders (x0 y0, x0 y1, x0 y2, x1 y0, x1 y1, x2 y0)
-  float a0[3], a1[3], a2[3], b0[3], b1[3], c0[3];
+  V a0[3], a1[3], a2[3], b0[3], b1[3], c0[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    const float t = k0[2][c] + k0[0][c];
+    const V t = k0[2][c] + k0[0][c];
     a2[c] = t;
     a0[c] = t + k0[1][c];
     a1[c] = k0[2][c] - k0[0][c];
@@ -697,8 +716,8 @@ __device__ __forceinline__ Moments2 moments_d3q27(const float (&f)[L::Q]) {
     c0[c] = (k2[2][c] + k2[0][c]) + k2[1][c];
   }
   // z
-  Moments2 m;
-  const float t = a0[2] + a0[0];
+  Moments2T<V> m;
+  const V t = a0[2] + a0[0];
   m.szz = t;
   m.rho = t + a0[1];
   m.jz = a0[2] - a0[0];
@@ -710,6 +729,29 @@ __device__ __forceinline__ Moments2 moments_d3q27(const float (&f)[L::Q]) {
   m.sxy = (b1[2] + b1[0]) + b1[1];
   m.sxx = (c0[2] + c0[0]) + c0[1];
   return m;
+}
+
+constexpr bool moments_d3q27_cascade_matches_sums() {
+  for (int i = 0; i < 9; ++i)
+    for (int a = 0; a < 3; ++a)
+      if (make_x_lines<D3Q27>().q[i][a] < 0) return false;
+  LinearForm<27> f[27];
+  for (int q = 0; q < 27; ++q) f[q].c[q] = 1.0;
+  const Moments2T<LinearForm<27>> m = moments_d3q27_t<D3Q27>(f);
+  for (int q = 0; q < 27; ++q) {
+    const double x = D3Q27::cx[q], y = D3Q27::cy[q], z = D3Q27::cz[q];
+    if (m.rho.c[q] != 1.0 || m.jx.c[q] != x || m.jy.c[q] != y || m.jz.c[q] != z || m.sxx.c[q] != x * x ||
+        m.syy.c[q] != y * y || m.szz.c[q] != z * z || m.sxy.c[q] != x * y || m.syz.c[q] != y * z ||
+        m.sxz.c[q] != x * z)
+      return false;
+  }
+  return true;
+}
+static_assert(moments_d3q27_cascade_matches_sums(), "x-y-z cascade is not the moment sums");
+
+template <class L>
+__device__ __forceinline__ Moments2 moments_d3q27(const float (&f)[L::Q]) {
+  return moments_d3q27_t<L, float>(f);
 }
 
 // Smagorinsky tau from the second moments (as smagorinsky_tau_moments).
