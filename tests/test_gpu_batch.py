@@ -188,6 +188,36 @@ def test_batch_3d_les_bfl_matches_plain(wa, storage, shape):
         assert rel_l2(g.f_src_data(), p.f_src_data()) <= {0: 1e-6, 1: 1e-5, 2: 1e-4}[storage]
 
 
+def _wing(wa, nx, storage):
+    from bench import wing_setup
+
+    ny = nz = nx // 2
+    return wing_setup(wa.LatticeGrid3D(nx, ny, nz, 0.52, lattice=27, storage=storage), nx, ny, nx / 4.0)
+
+
+@pytest.mark.parametrize("shape", [None, (64, 4)], ids=["tiles", "persistent"])
+@pytest.mark.parametrize("storage", [0, 1, 2])
+def test_batch_d3q27_wing_matches_plain(wa, storage, shape):
+    """D3Q27 TRT + coupled LES wing channels of two sizes (walls, a NACA
+    section, velocity inlet, convective outlet, rows of 48 and 80 nodes)
+    stepped in a batch match the plain step, under both launch shapes."""
+    steps = 30
+    sizes = (48, 80)
+    batch = wa.Batch(lattice=27, storage=storage, collision="TRT", use_les=True, chunk=16)
+    gs = [_wing(wa, nx, storage) for nx in sizes]
+    for g in gs:
+        batch.add(g, steps, smag=0.17)
+    if shape is not None:
+        batch.set_launch(*shape)
+    batch.run(steps)
+    for g, nx in zip(gs, sizes):
+        p = _wing(wa, nx, storage)
+        for _ in range(steps):
+            p.step("TRT", 0.0, 0.0, 0.0, True, 0.17)
+        assert np.all(np.isfinite(p.f_src_data()))
+        assert rel_l2(g.f_src_data(), p.f_src_data()) <= {0: 1e-6, 1: 1e-5, 2: 1e-4}[storage]
+
+
 def test_forces_member_without_links_records_zeros(wa):
     """A member with forces but no wall link (no body, or r_max too small)
     records zero forces, also after an earlier member left the batch and the
